@@ -1,0 +1,257 @@
+/*
+ * atmo: MI355X (gfx950) precomputation of Bruneton-style atmospheric
+ * scattering LUTs and the sky-shading lookups that read them.
+ *
+ * C ABI of libatmo.so. It replaces the OpenGL-backed atmosphere::Model of the
+ * reference (atmosphere/model.h:160-337, atmosphere/model.cc:613-1215) and the
+ * GLSL rendering API it exposes (atmosphere/model.h:51-89,
+ * atmosphere/model.cc:221-281). Conventions:
+ *   - every entry point returns an atmo_status; atmo_last_error() gives the
+ *     message of the last failure on the calling thread;
+ *   - a model handle owns its device memory; one handle must not be used from
+ *     two threads at once (distinct handles are independent);
+ *   - 'stream' is a hipStream_t (NULL = the default stream); all device work
+ *     is enqueued on it and the call returns without waiting, except where a
+ *     function says otherwise;
+ *   - array arguments of the lookup functions are device pointers, n x 3
+ *     floats (AoS) unless stated; lengths are in the model's length unit, as
+ *     in the reference's shader API.
+ */
+#ifndef ATMO_H_
+#define ATMO_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum atmo_status {
+  ATMO_OK = 0,
+  ATMO_ERROR_INVALID_ARGUMENT = 1,
+  ATMO_ERROR_OUT_OF_MEMORY = 2,
+  ATMO_ERROR_DEVICE = 3,       /* a HIP runtime call failed */
+  ATMO_ERROR_NOT_INITIALIZED = 4,
+  ATMO_ERROR_UNSUPPORTED = 5,
+} atmo_status;
+
+/* DensityProfileLayer (atmosphere/model.h:165-178): width in m, exp_scale and
+ * linear_term in 1/m; density = clamp(exp_term * exp(exp_scale * h) +
+ * linear_term * h + constant_term, 0, 1). */
+typedef struct atmo_density_layer {
+  double width;
+  double exp_term;
+  double exp_scale;
+  double linear_term;
+  double constant_term;
+} atmo_density_layer;
+
+/* The 17 Model constructor parameters (atmosphere/model.h:182-281), same
+ * meaning and units. Spectra have num_wavelengths samples; layer lists have
+ * at most 2 entries (fewer are padded with null layers at the front, as
+ * atmosphere/model.cc:653-658 does). */
+typedef struct atmo_model_params {
+  int num_wavelengths;
+  const double* wavelengths;          /* nm, increasing */
+  const double* solar_irradiance;     /* W/m^2/nm */
+  double sun_angular_radius;          /* rad */
+  double bottom_radius;               /* m */
+  double top_radius;                  /* m */
+  int num_rayleigh_layers;
+  const atmo_density_layer* rayleigh_density;
+  const double* rayleigh_scattering;  /* 1/m */
+  int num_mie_layers;
+  const atmo_density_layer* mie_density;
+  const double* mie_scattering;       /* 1/m */
+  const double* mie_extinction;       /* 1/m */
+  double mie_phase_function_g;
+  int num_absorption_layers;
+  const atmo_density_layer* absorption_density;
+  const double* absorption_extinction; /* 1/m */
+  const double* ground_albedo;
+  double max_sun_zenith_angle;        /* rad */
+  double length_unit_in_meters;
+  unsigned num_precomputed_wavelengths;
+  int combine_scattering_textures;
+  int half_precision;
+} atmo_model_params;
+
+/* LUT resolution. The reference fixes these at compile time
+ * (atmosphere/constants.h:47-61); here they are per model. NULL selects the
+ * reference values: 256x64, (32, 128, 32, 8), 64x16. scattering_mu must be
+ * even and every size >= 2. */
+typedef struct atmo_lut_sizes {
+  int transmittance_width;
+  int transmittance_height;
+  int scattering_r;
+  int scattering_mu;
+  int scattering_mu_s;
+  int scattering_nu;
+  int irradiance_width;
+  int irradiance_height;
+} atmo_lut_sizes;
+
+typedef enum atmo_lut_id {
+  ATMO_LUT_TRANSMITTANCE = 0,
+  ATMO_LUT_SCATTERING = 1,
+  ATMO_LUT_SINGLE_MIE_SCATTERING = 2, /* absent when textures are combined */
+  ATMO_LUT_IRRADIANCE = 3,
+  /* Temporaries of the last Init pass, kept only when the model was created
+   * with ATMO_OPTION_RETAIN_INTERMEDIATES (checks and debugging). */
+  ATMO_LUT_DELTA_IRRADIANCE = 4,
+  ATMO_LUT_DELTA_RAYLEIGH = 5,        /* also delta multiple scattering */
+  ATMO_LUT_DELTA_MIE = 6,
+  ATMO_LUT_DELTA_SCATTERING_DENSITY = 7,
+} atmo_lut_id;
+
+typedef enum atmo_dtype { ATMO_DTYPE_F32 = 0, ATMO_DTYPE_F16 = 1 } atmo_dtype;
+
+/* A LUT in device memory: width x height x depth texels of 4 channels (RGBA),
+ * x fastest, in the GL texture layout (the packed scattering LUT has
+ * x = nu_index * MU_S + mu_s_index, y = mu_index, z = r_index). A stored
+ * value equals the physical value times scale[c] (scale is 1 for F32). */
+typedef struct atmo_lut_view {
+  void* data;
+  int width;
+  int height;
+  int depth;
+  int channels;
+  atmo_dtype dtype;
+  float scale[4];
+} atmo_lut_view;
+
+typedef enum atmo_option {
+  ATMO_OPTION_RETAIN_INTERMEDIATES = 0, /* keep the delta LUTs after Init */
+} atmo_option;
+
+typedef struct atmo_model atmo_model;
+
+/* Model::Model (atmosphere/model.h:182-281, model.cc:613-795). Validates and
+ * copies the parameters, allocates the result LUTs on the current device. */
+atmo_status atmo_model_create(const atmo_model_params* params,
+                              const atmo_lut_sizes* sizes, atmo_model** out);
+/* Model::~Model (atmosphere/model.cc:801-811). */
+void atmo_model_destroy(atmo_model* model);
+atmo_status atmo_model_set_option(atmo_model* model, atmo_option option,
+                                  int value);
+/* Model::Init (atmosphere/model.h:285, model.cc:866-975): precomputes the
+ * LUTs with num_scattering_orders scattering orders (>= 1). */
+atmo_status atmo_model_init(atmo_model* model, unsigned num_scattering_orders,
+                            void* stream);
+/* Init of several models in one batched submission (no counterpart in the
+ * reference, which precomputes one Model at a time): all models' passes of
+ * one kind run in a single launch, the launch sequence is captured once in a
+ * HIP graph and replayed. orders[i] belongs to models[i]. */
+atmo_status atmo_model_init_batch(atmo_model* const* models,
+                                  const unsigned* orders, int count,
+                                  void* stream);
+/* A reusable batched precompute plan (no counterpart in the reference): the
+ * pass sequence of several models, one launch per pass level over all of
+ * them, recorded once. With use_graph the launches are captured in a HIP
+ * graph and atmo_batch_launch replays the graph. The models keep their
+ * temporaries until the batch is destroyed; a model must not be in two live
+ * batches. atmo_model_init_batch is create + launch + wait + destroy. */
+typedef struct atmo_batch atmo_batch;
+atmo_status atmo_batch_create(atmo_model* const* models,
+                              const unsigned* orders, int count, int use_graph,
+                              atmo_batch** out);
+atmo_status atmo_batch_launch(atmo_batch* batch, void* stream);
+int atmo_batch_num_launches(const atmo_batch* batch);
+void atmo_batch_destroy(atmo_batch* batch);
+
+/* The device LUTs (the reference binds them to texture units instead,
+ * Model::SetProgramUniforms, atmosphere/model.h:289-294). */
+atmo_status atmo_model_get_lut(const atmo_model* model, atmo_lut_id id,
+                               atmo_lut_view* view);
+/* Synchronous copy of a LUT to host memory as float RGBA, decoded to
+ * physical values (width*height*depth*4 floats). */
+atmo_status atmo_model_copy_lut(const atmo_model* model, atmo_lut_id id,
+                                float* host_rgba);
+/* Device bytes held by the model's result LUTs, and the peak extra bytes
+ * Init allocates for its temporaries. */
+atmo_status atmo_model_footprint(const atmo_model* model, size_t* resident,
+                                 size_t* init_scratch);
+
+/* GetSolarRadiance (atmosphere/model.cc:227-230): host-side, 3 floats. */
+atmo_status atmo_get_solar_radiance(const atmo_model* model, float* rgb);
+/* GetSkyRadiance (atmosphere/functions.glsl:1705-1769, wrapped at
+ * model.cc:231-237). camera, view_ray, sun_direction: n x 3; shadow_length:
+ * n floats. Outputs radiance and transmittance: n x 3. */
+atmo_status atmo_get_sky_radiance(const atmo_model* model, int64_t n,
+                                  const float* camera, const float* view_ray,
+                                  const float* shadow_length,
+                                  const float* sun_direction, float* radiance,
+                                  float* transmittance, void* stream);
+/* GetSkyRadianceToPoint (functions.glsl:1787-1863, model.cc:238-244). */
+atmo_status atmo_get_sky_radiance_to_point(
+    const atmo_model* model, int64_t n, const float* camera,
+    const float* point, const float* shadow_length, const float* sun_direction,
+    float* radiance, float* transmittance, void* stream);
+/* GetSunAndSkyIrradiance (functions.glsl:1878-1896, model.cc:245-250).
+ * Returns the sun irradiance in sun_irradiance and the sky irradiance in
+ * sky_irradiance, n x 3 each. */
+atmo_status atmo_get_sun_and_sky_irradiance(
+    const atmo_model* model, int64_t n, const float* point,
+    const float* normal, const float* sun_direction, float* sun_irradiance,
+    float* sky_irradiance, void* stream);
+
+/* AtmosphereParameters (atmosphere/definitions.glsl:205-255) at three
+ * wavelengths, as the constructor lowers them (atmosphere/model.cc:636-666,
+ * 718-734) but in exact binary values: lengths divided by the length unit,
+ * coefficients multiplied by it, profiles padded to two layers. Host-only. */
+typedef struct atmo_atmosphere {
+  double solar_irradiance[3];
+  double sun_angular_radius;
+  double bottom_radius;
+  double top_radius;
+  atmo_density_layer rayleigh_density[2];
+  double rayleigh_scattering[3];
+  atmo_density_layer mie_density[2];
+  double mie_scattering[3];
+  double mie_extinction[3];
+  double mie_phase_function_g;
+  atmo_density_layer absorption_density[2];
+  double absorption_extinction[3];
+  double ground_albedo[3];
+  double mu_s_min;
+} atmo_atmosphere;
+atmo_status atmo_lower_parameters(const atmo_model_params* params,
+                                  const double* lambdas3,
+                                  atmo_atmosphere* out);
+/* Number of Precompute iterations (1, or ceil(num_precomputed_wavelengths/3)
+ * in illuminance mode) and, for iteration i, its 3 wavelengths and
+ * luminance_from_radiance matrix (row-major), atmosphere/model.cc:907-943.
+ * Host-only. */
+int atmo_num_iterations(const atmo_model_params* params);
+atmo_status atmo_iteration(const atmo_model_params* params, int i,
+                           double* lambdas3, float* matrix9);
+
+/* Model::ConvertSpectrumToLinearSrgb (atmosphere/model.h:301-304,
+ * model.cc:1020-1040). Host-only. */
+atmo_status atmo_convert_spectrum_to_linear_srgb(int n,
+                                                 const double* wavelengths,
+                                                 const double* spectrum,
+                                                 double* r, double* g,
+                                                 double* b);
+
+const char* atmo_last_error(void);
+
+/* Minimal device-memory and stream helpers, so that callers without their
+ * own HIP runtime (ctypes, cgo) can drive the lookups. */
+atmo_status atmo_malloc(void** ptr, size_t bytes);
+atmo_status atmo_free(void* ptr);
+atmo_status atmo_memcpy_to_device(void* dst, const void* src, size_t bytes,
+                                  void* stream);
+atmo_status atmo_memcpy_to_host(void* dst, const void* src, size_t bytes,
+                                void* stream);
+atmo_status atmo_stream_create(void** stream);
+atmo_status atmo_stream_destroy(void* stream);
+atmo_status atmo_stream_synchronize(void* stream);
+atmo_status atmo_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATMO_H_ */

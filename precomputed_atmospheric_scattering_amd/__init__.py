@@ -1,0 +1,372 @@
+"""MI355X precomputation of Bruneton-style atmospheric scattering LUTs.
+
+Python host-side mirror of the reference's atmosphere::Model
+(atmosphere/model.h:160-337) over the C ABI of libatmo.so (include/atmo.h):
+the same constructor parameters, init(num_scattering_orders) (alias Init),
+ConvertSpectrumToLinearSrgb and kLambdaR/G/B. The precomputed LUTs live in
+device memory; lut() copies one to the host, and the sky lookups run on
+device buffers (DeviceArray) or on host arrays for convenience.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import AtmoError, check, lib
+
+kLambdaR = 680.0
+kLambdaG = 550.0
+kLambdaB = 440.0
+
+DEFAULT_SIZES = dict(transmittance_width=256, transmittance_height=64,
+                     scattering_r=32, scattering_mu=128, scattering_mu_s=32,
+                     scattering_nu=8, irradiance_width=64, irradiance_height=16)
+
+
+class DensityProfileLayer(tuple):
+    """(width, exp_term, exp_scale, linear_term, constant_term), in m and 1/m
+    (atmosphere/model.h:165-178)."""
+
+    def __new__(cls, width=0.0, exp_term=0.0, exp_scale=0.0, linear_term=0.0,
+                constant_term=0.0):
+        return super().__new__(cls, (float(width), float(exp_term), float(exp_scale),
+                                     float(linear_term), float(constant_term)))
+
+
+def _f64(v):
+    return np.ascontiguousarray(v, dtype=np.float64)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Stream:
+    """A HIP stream owned by libatmo (for callers without a HIP runtime)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().atmo_stream_create(C.byref(h)), "atmo_stream_create")
+        self.handle = h
+
+    def synchronize(self):
+        check(lib().atmo_stream_synchronize(self.handle), "atmo_stream_synchronize")
+
+    def close(self):
+        if self.handle:
+            check(lib().atmo_stream_destroy(self.handle), "atmo_stream_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+class DeviceArray:
+    """A float32 device buffer allocated through libatmo."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().atmo_malloc(C.byref(p), max(self.nbytes, 4)), "atmo_malloc")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        check(lib().atmo_memcpy_to_device(d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes,
+                                          _stream_handle(stream)), "atmo_memcpy_to_device")
+        if stream is None:
+            check(lib().atmo_stream_synchronize(None), "atmo_stream_synchronize")
+        return d
+
+    def to_host(self, stream=None):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().atmo_memcpy_to_host(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes,
+                                        _stream_handle(stream)), "atmo_memcpy_to_host")
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().atmo_free(self.ptr), "atmo_free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class ModelParameters:
+    """The Model constructor parameters (atmosphere/model.h:182-281), checked
+    and packed into the C struct atmo_model_params. Host-only: also gives the
+    lowered AtmosphereParameters and the illuminance-mode iterations."""
+
+    def __init__(self, wavelengths, solar_irradiance, sun_angular_radius, bottom_radius,
+                 top_radius, rayleigh_density, rayleigh_scattering, mie_density,
+                 mie_scattering, mie_extinction, mie_phase_function_g, absorption_density,
+                 absorption_extinction, ground_albedo, max_sun_zenith_angle,
+                 length_unit_in_meters, num_precomputed_wavelengths,
+                 combine_scattering_textures, half_precision):
+        self.wavelengths = _f64(wavelengths)
+        n = len(self.wavelengths)
+        spectra = dict(solar_irradiance=solar_irradiance,
+                       rayleigh_scattering=rayleigh_scattering,
+                       mie_scattering=mie_scattering, mie_extinction=mie_extinction,
+                       absorption_extinction=absorption_extinction,
+                       ground_albedo=ground_albedo)
+        for k, v in spectra.items():
+            v = _f64(v)
+            if v.shape != (n,):
+                raise ValueError("%s must have one value per wavelength" % k)
+            setattr(self, k, v)
+        mk = lambda ls: [DensityProfileLayer(*l) for l in ls]
+        self.rayleigh_density = mk(rayleigh_density)
+        self.mie_density = mk(mie_density)
+        self.absorption_density = mk(absorption_density)
+        for name in ("rayleigh_density", "mie_density", "absorption_density"):
+            if len(getattr(self, name)) > 2:
+                raise ValueError("%s: at most 2 layers" % name)
+        self.sun_angular_radius = float(sun_angular_radius)
+        self.bottom_radius = float(bottom_radius)
+        self.top_radius = float(top_radius)
+        self.mie_phase_function_g = float(mie_phase_function_g)
+        self.max_sun_zenith_angle = float(max_sun_zenith_angle)
+        self.length_unit_in_meters = float(length_unit_in_meters)
+        self.num_precomputed_wavelengths = int(num_precomputed_wavelengths)
+        self.combine_scattering_textures = bool(combine_scattering_textures)
+        self.half_precision = bool(half_precision)
+        carr = lambda ls: (_capi.DensityLayer * max(1, len(ls)))(
+            *[_capi.DensityLayer(*l) for l in ls])
+        self._layers = (carr(self.rayleigh_density), carr(self.mie_density),
+                        carr(self.absorption_density))
+        self.c = _capi.ModelParams(
+            n, _dptr(self.wavelengths), _dptr(self.solar_irradiance),
+            self.sun_angular_radius, self.bottom_radius, self.top_radius,
+            len(self.rayleigh_density), self._layers[0], _dptr(self.rayleigh_scattering),
+            len(self.mie_density), self._layers[1], _dptr(self.mie_scattering),
+            _dptr(self.mie_extinction), self.mie_phase_function_g,
+            len(self.absorption_density), self._layers[2],
+            _dptr(self.absorption_extinction), _dptr(self.ground_albedo),
+            self.max_sun_zenith_angle, self.length_unit_in_meters,
+            self.num_precomputed_wavelengths, int(self.combine_scattering_textures),
+            int(self.half_precision))
+
+    def lower(self, lambdas):
+        """AtmosphereParameters at the 3 wavelengths (atmo_lower_parameters)."""
+        out = _capi.Atmosphere()
+        check(lib().atmo_lower_parameters(C.byref(self.c), (C.c_double * 3)(*lambdas),
+                                          C.byref(out)), "lower_parameters")
+        return out
+
+    def num_iterations(self):
+        return lib().atmo_num_iterations(C.byref(self.c))
+
+    def iteration(self, i):
+        """(3 wavelengths, 3x3 luminance_from_radiance) of iteration i."""
+        lam = (C.c_double * 3)()
+        mat = (C.c_float * 9)()
+        check(lib().atmo_iteration(C.byref(self.c), i, lam, mat), "iteration")
+        return list(lam), np.array(list(mat), dtype=np.float32).reshape(3, 3)
+
+
+class Model:
+    """atmosphere::Model (atmosphere/model.h:182-281) on the GPU."""
+
+    kLambdaR = kLambdaR
+    kLambdaG = kLambdaG
+    kLambdaB = kLambdaB
+
+    def __init__(self, wavelengths, solar_irradiance, sun_angular_radius, bottom_radius,
+                 top_radius, rayleigh_density, rayleigh_scattering, mie_density,
+                 mie_scattering, mie_extinction, mie_phase_function_g, absorption_density,
+                 absorption_extinction, ground_albedo, max_sun_zenith_angle,
+                 length_unit_in_meters, num_precomputed_wavelengths,
+                 combine_scattering_textures, half_precision, lut_sizes=None,
+                 retain_intermediates=False):
+        self._handle = None
+        self.params = ModelParameters(
+            wavelengths, solar_irradiance, sun_angular_radius, bottom_radius, top_radius,
+            rayleigh_density, rayleigh_scattering, mie_density, mie_scattering,
+            mie_extinction, mie_phase_function_g, absorption_density, absorption_extinction,
+            ground_albedo, max_sun_zenith_angle, length_unit_in_meters,
+            num_precomputed_wavelengths, combine_scattering_textures, half_precision)
+        self.combine_scattering_textures = self.params.combine_scattering_textures
+        self.half_precision = self.params.half_precision
+        self.lut_sizes = dict(DEFAULT_SIZES)
+        if lut_sizes:
+            self.lut_sizes.update(lut_sizes)
+        z = _capi.LutSizes(**self.lut_sizes)
+        h = C.c_void_p()
+        check(lib().atmo_model_create(C.byref(self.params.c), C.byref(z), C.byref(h)),
+              "Model")
+        self._handle = h
+        if retain_intermediates:
+            check(lib().atmo_model_set_option(h, _capi.OPTION_RETAIN_INTERMEDIATES, 1),
+                  "set_option")
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def close(self):
+        if self._handle:
+            lib().atmo_model_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # atmosphere/model.h:285
+    def init(self, num_scattering_orders=4, stream=None):
+        check(lib().atmo_model_init(self._handle, int(num_scattering_orders),
+                                    _stream_handle(stream)), "Init")
+
+    Init = init
+
+    def lut_view(self, name):
+        v = _capi.LutView()
+        check(lib().atmo_model_get_lut(self._handle, _capi.LUT_IDS[name], C.byref(v)),
+              "lut_view")
+        return v
+
+    def lut(self, name):
+        """Copy of a LUT as float32 (depth, height, width, 4) (2D LUTs:
+        (height, width, 4)), decoded to physical values."""
+        v = self.lut_view(name)
+        out = np.empty((v.depth, v.height, v.width, 4), dtype=np.float32)
+        check(lib().atmo_model_copy_lut(self._handle, _capi.LUT_IDS[name],
+                                        out.ctypes.data_as(C.POINTER(C.c_float))), "lut")
+        return out[0] if v.depth == 1 and name in (
+            "transmittance", "irradiance", "delta_irradiance") else out
+
+    def footprint(self):
+        a, b = C.c_size_t(), C.c_size_t()
+        check(lib().atmo_model_footprint(self._handle, C.byref(a), C.byref(b)), "footprint")
+        return a.value, b.value
+
+    def solar_radiance(self):
+        out = (C.c_float * 3)()
+        check(lib().atmo_get_solar_radiance(self._handle, out), "GetSolarRadiance")
+        return np.array(list(out), dtype=np.float32)
+
+    # ---- lookups on device arrays ----
+    def sky_radiance_device(self, n, camera, view_ray, shadow_length, sun_direction,
+                            radiance, transmittance, stream=None):
+        check(lib().atmo_get_sky_radiance(self._handle, n, camera.ptr, view_ray.ptr,
+                                          shadow_length.ptr, sun_direction.ptr,
+                                          radiance.ptr, transmittance.ptr,
+                                          _stream_handle(stream)), "GetSkyRadiance")
+
+    def sky_radiance_to_point_device(self, n, camera, point, shadow_length, sun_direction,
+                                     radiance, transmittance, stream=None):
+        check(lib().atmo_get_sky_radiance_to_point(self._handle, n, camera.ptr, point.ptr,
+                                                   shadow_length.ptr, sun_direction.ptr,
+                                                   radiance.ptr, transmittance.ptr,
+                                                   _stream_handle(stream)),
+              "GetSkyRadianceToPoint")
+
+    def sun_and_sky_irradiance_device(self, n, point, normal, sun_direction,
+                                      sun_irradiance, sky_irradiance, stream=None):
+        check(lib().atmo_get_sun_and_sky_irradiance(self._handle, n, point.ptr, normal.ptr,
+                                                    sun_direction.ptr, sun_irradiance.ptr,
+                                                    sky_irradiance.ptr,
+                                                    _stream_handle(stream)),
+              "GetSunAndSkyIrradiance")
+
+    # ---- host-array conveniences (copy in, run, copy out) ----
+    def sky_radiance(self, camera, view_ray, shadow_length, sun_direction):
+        n = len(camera)
+        d = [DeviceArray.from_host(np.asarray(a, np.float32))
+             for a in (camera, view_ray, shadow_length, sun_direction)]
+        rad, tr = DeviceArray((n, 3)), DeviceArray((n, 3))
+        self.sky_radiance_device(n, *d, rad, tr)
+        return rad.to_host(), tr.to_host()
+
+    def sky_radiance_to_point(self, camera, point, shadow_length, sun_direction):
+        n = len(camera)
+        d = [DeviceArray.from_host(np.asarray(a, np.float32))
+             for a in (camera, point, shadow_length, sun_direction)]
+        rad, tr = DeviceArray((n, 3)), DeviceArray((n, 3))
+        self.sky_radiance_to_point_device(n, *d, rad, tr)
+        return rad.to_host(), tr.to_host()
+
+    def sun_and_sky_irradiance(self, point, normal, sun_direction):
+        n = len(point)
+        d = [DeviceArray.from_host(np.asarray(a, np.float32))
+             for a in (point, normal, sun_direction)]
+        sun, sky = DeviceArray((n, 3)), DeviceArray((n, 3))
+        self.sun_and_sky_irradiance_device(n, *d, sun, sky)
+        return sun.to_host(), sky.to_host()
+
+    # atmosphere/model.h:301-304
+    @staticmethod
+    def ConvertSpectrumToLinearSrgb(wavelengths, spectrum):
+        w, s = _f64(wavelengths), _f64(spectrum)
+        if w.shape != s.shape:
+            raise ValueError("spectrum must have one value per wavelength")
+        r, g, b = C.c_double(), C.c_double(), C.c_double()
+        check(lib().atmo_convert_spectrum_to_linear_srgb(len(w), _dptr(w), _dptr(s),
+                                                         C.byref(r), C.byref(g),
+                                                         C.byref(b)),
+              "ConvertSpectrumToLinearSrgb")
+        return r.value, g.value, b.value
+
+
+class Batch:
+    """Reusable batched precompute of several models (atmo_batch_*)."""
+
+    def __init__(self, models, orders, use_graph=True):
+        self.models = list(models)
+        n = len(self.models)
+        hs = (C.c_void_p * n)(*[m.handle for m in self.models])
+        os_ = (C.c_uint * n)(*[int(o) for o in orders])
+        h = C.c_void_p()
+        check(lib().atmo_batch_create(hs, os_, n, int(bool(use_graph)), C.byref(h)),
+              "atmo_batch_create")
+        self.handle = h
+
+    @property
+    def num_launches(self):
+        return lib().atmo_batch_num_launches(self.handle)
+
+    def launch(self, stream=None):
+        check(lib().atmo_batch_launch(self.handle, _stream_handle(stream)),
+              "atmo_batch_launch")
+
+    def close(self):
+        if self.handle:
+            lib().atmo_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def init_batch(models, orders, stream=None):
+    n = len(models)
+    hs = (C.c_void_p * n)(*[m.handle for m in models])
+    os_ = (C.c_uint * n)(*[int(o) for o in orders])
+    check(lib().atmo_model_init_batch(hs, os_, n, _stream_handle(stream)), "init_batch")
+
+
+def synchronize():
+    check(lib().atmo_device_synchronize(), "atmo_device_synchronize")

@@ -1,0 +1,429 @@
+// Device-side parameter block and the scalar mathematics shared by the
+// precompute and lookup kernels (gfx950).
+//
+// Precision plan. Everything that sets up a texel or a ray-march sample (the
+// LUT-coordinate mappings, ray/sphere intersections, ground flags, the
+// transmittance LUT coordinates) runs in FP64: those quantities are shared by
+// many inner-loop samples, so the cost is amortised, and they are where FP32
+// cancellation shifts LUT coordinates near the horizon and near the ground.
+// The inner quadrature loops (phase functions, LUT interpolation, sums) run in
+// FP32 with FP32 accumulation.
+#ifndef ATMO_DEVICE_H_
+#define ATMO_DEVICE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ATMO_HD __host__ __device__ inline
+
+namespace atmo {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct LayerD {
+  double width, exp_term, exp_scale, linear_term, constant_term;
+};
+
+// One precompute job: an atmosphere lowered at three wavelengths (one
+// Model::Precompute call of atmosphere/model.cc:1048-1215), with its LUTs.
+struct Job {
+  // AtmosphereParameters (atmosphere/definitions.glsl:205-255), FP64, lengths
+  // in the model's length unit.
+  double solar[3];
+  double rayleigh[3];
+  double mie_sca[3];
+  double mie_ext[3];
+  double absorption[3];
+  double albedo[3];
+  double sun_angular_radius;
+  double rb, rt;
+  double mu_s_min;
+  double g;
+  LayerD prof_r[2], prof_m[2], prof_a[2];
+  // Derived constants (host, FP64): H = sqrt(rt^2 - rb^2) and the A of the
+  // mu_s mapping (functions.glsl:819-821).
+  double H;
+  double A;
+  // luminance_from_radiance, row-major (atmosphere/model.cc:909, 939-943).
+  float lum[9];
+  int blend;     // add into S, S_Mie, E (illuminance iterations after the first)
+  int combined;  // COMBINED_SCATTERING_TEXTURES: no separate S_Mie output
+  int orders;    // number of scattering orders of this job
+  // LUT sizes.
+  int tw, th;
+  int nr, nmu, nmus, nnu;
+  int ew, eh;
+  // LUTs (RGBA FP32, GL layout, x fastest).
+  float4* T;
+  float4* S;
+  float4* SM;
+  float4* E;
+  float4* dE;
+  float4* dR;  // delta Rayleigh; delta multiple scattering from order 2 on
+  float4* dM;
+  float4* dJ;
+};
+
+// ---------------------------------------------------------------------------
+// FP64 scalar functions (atmosphere/functions.glsl).
+// ---------------------------------------------------------------------------
+
+ATMO_HD double clamp_d(double x, double lo, double hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+ATMO_HD double clamp_cos(double mu) { return clamp_d(mu, -1.0, 1.0); }
+ATMO_HD double safe_sqrt(double a) { return sqrt(a > 0.0 ? a : 0.0); }
+ATMO_HD double clamp_radius(const Job& j, double r) {
+  return clamp_d(r, j.rb, j.rt);
+}
+
+// functions.glsl:207-214
+ATMO_HD double dist_top(const Job& j, double r, double mu) {
+  double disc = r * r * (mu * mu - 1.0) + j.rt * j.rt;
+  double d = -r * mu + safe_sqrt(disc);
+  return d > 0.0 ? d : 0.0;
+}
+// functions.glsl:222-229
+ATMO_HD double dist_bottom(const Job& j, double r, double mu) {
+  double disc = r * r * (mu * mu - 1.0) + j.rb * j.rb;
+  double d = -r * mu - safe_sqrt(disc);
+  return d > 0.0 ? d : 0.0;
+}
+// functions.glsl:240-246
+ATMO_HD bool hits_ground(const Job& j, double r, double mu) {
+  return mu < 0.0 && r * r * (mu * mu - 1.0) + j.rb * j.rb >= 0.0;
+}
+// functions.glsl:680-687
+ATMO_HD double dist_nearest(const Job& j, double r, double mu, bool ground) {
+  return ground ? dist_bottom(j, r, mu) : dist_top(j, r, mu);
+}
+// functions.glsl:263-273. A null exponential term is skipped (its product
+// with a finite exp is 0).
+ATMO_HD double layer_density(const LayerD& l, double h) {
+  double v = l.linear_term * h + l.constant_term;
+  if (l.exp_term != 0.0) v += l.exp_term * exp(l.exp_scale * h);
+  return clamp_d(v, 0.0, 1.0);
+}
+ATMO_HD double profile_density(const LayerD* p, double h) {
+  return h < p[0].width ? layer_density(p[0], h) : layer_density(p[1], h);
+}
+
+// GL LINEAR + CLAMP_TO_EDGE taps along one axis of n texels, from the
+// texel-space coordinate x = u * n - 1/2 (GL 3.3 core, section 3.8.11).
+struct Taps {
+  int i0, i1;
+  float w;
+};
+ATMO_HD Taps axis_taps(double x, int n) {
+  // Out-of-range and NaN coordinates are pinned to [-1, n] first, so the
+  // integer taps below are always in bounds.
+  if (!(x >= -1.0)) x = -1.0;
+  if (x > static_cast<double>(n)) x = static_cast<double>(n);
+  double f = floor(x);
+  Taps t;
+  t.w = static_cast<float>(x - f);
+  int i = static_cast<int>(f);
+  t.i0 = i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+  t.i1 = i + 1 < 0 ? 0 : (i + 1 > n - 1 ? n - 1 : i + 1);
+  return t;
+}
+// functions.glsl:342-348, composed with the GL texel-space coordinate.
+ATMO_HD double unit_to_texel(double x, int n) {
+  double u = 0.5 / n + x * (1.0 - 1.0 / n);
+  return u * n - 0.5;
+}
+ATMO_HD double unit_from_tex(double u, int n) {
+  return (u - 0.5 / n) / (1.0 - 1.0 / n);
+}
+
+// ---------------------------------------------------------------------------
+// Transmittance LUT (functions.glsl:402-519).
+// ---------------------------------------------------------------------------
+
+// functions.glsl:402-421, as GL texel-space (x, y).
+ATMO_HD void t_texel_coords(const Job& j, double r, double mu, double* x,
+                            double* y) {
+  double rho = safe_sqrt(r * r - j.rb * j.rb);
+  double d = dist_top(j, r, mu);
+  double d_min = j.rt - r;
+  double d_max = rho + j.H;
+  *x = unit_to_texel((d - d_min) / (d_max - d_min), j.tw);
+  *y = unit_to_texel(rho / j.H, j.th);
+}
+
+// functions.glsl:427-447
+ATMO_HD void t_rmu_from_uv(const Job& j, double u, double v, double* r,
+                           double* mu) {
+  double x_mu = unit_from_tex(u, j.tw);
+  double x_r = unit_from_tex(v, j.th);
+  double rho = j.H * x_r;
+  *r = sqrt(rho * rho + j.rb * j.rb);
+  double d_min = j.rt - *r;
+  double d_max = rho + j.H;
+  double d = d_min + x_mu * (d_max - d_min);
+  double m = d == 0.0 ? 1.0 : (j.H * j.H - rho * rho - d * d) / (2.0 * *r * d);
+  *mu = clamp_cos(m);
+}
+
+ATMO_HD float3 make3(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// Bilinear read of a 2D RGBA LUT at GL texel-space (x, y).
+ATMO_HD float3 bilinear(const float4* lut, int w, int h, double x, double y) {
+  Taps tx = axis_taps(x, w), ty = axis_taps(y, h);
+  float4 a = lut[ty.i0 * w + tx.i0], b = lut[ty.i0 * w + tx.i1];
+  float4 c = lut[ty.i1 * w + tx.i0], d = lut[ty.i1 * w + tx.i1];
+  float wx1 = tx.w, wx0 = 1.0f - tx.w, wy1 = ty.w, wy0 = 1.0f - ty.w;
+  return make_float3(wy0 * (wx0 * a.x + wx1 * b.x) + wy1 * (wx0 * c.x + wx1 * d.x),
+                     wy0 * (wx0 * a.y + wx1 * b.y) + wy1 * (wx0 * c.y + wx1 * d.y),
+                     wy0 * (wx0 * a.z + wx1 * b.z) + wy1 * (wx0 * c.z + wx1 * d.z));
+}
+
+// functions.glsl:473-480
+ATMO_HD float3 t_lookup(const Job& j, const float4* T, double r, double mu) {
+  double x, y;
+  t_texel_coords(j, r, mu, &x, &y);
+  return bilinear(T, j.tw, j.th, x, y);
+}
+
+ATMO_HD float min1(float v) { return v < 1.0f ? v : 1.0f; }
+
+// Clamp to [0, hi]; NaN maps to 0. Used before every float -> index
+// conversion so that indices stay in bounds whatever the input.
+ATMO_HD float clamp_index_coord(float t, float hi) {
+  t = t >= 0.0f ? t : 0.0f;
+  return t <= hi ? t : hi;
+}
+
+// functions.glsl:493-519
+ATMO_HD float3 transmittance(const Job& j, const float4* T, double r,
+                             double mu, double d, bool ground) {
+  double r_d = clamp_radius(j, sqrt(d * d + 2.0 * r * mu * d + r * r));
+  double mu_d = clamp_cos((r * mu + d) / r_d);
+  float3 num, den;
+  if (ground) {
+    num = t_lookup(j, T, r_d, -mu_d);
+    den = t_lookup(j, T, r, -mu);
+  } else {
+    num = t_lookup(j, T, r, mu);
+    den = t_lookup(j, T, r_d, mu_d);
+  }
+  return make_float3(min1(num.x / den.x), min1(num.y / den.y),
+                     min1(num.z / den.z));
+}
+
+ATMO_HD double smoothstep_d(double e0, double e1, double x) {
+  double t = clamp_d((x - e0) / (e1 - e0), 0.0, 1.0);
+  return t * t * (3.0 - 2.0 * t);
+}
+
+// functions.glsl:552-563
+ATMO_HD float3 transmittance_to_sun(const Job& j, const float4* T, double r,
+                                    double mu_s) {
+  double sin_h = j.rb / r;
+  double cos_h = -sqrt(1.0 - sin_h * sin_h > 0.0 ? 1.0 - sin_h * sin_h : 0.0);
+  float f = static_cast<float>(smoothstep_d(-sin_h * j.sun_angular_radius,
+                                            sin_h * j.sun_angular_radius,
+                                            mu_s - cos_h));
+  float3 t = t_lookup(j, T, r, mu_s);
+  return make_float3(t.x * f, t.y * f, t.z * f);
+}
+
+// ---------------------------------------------------------------------------
+// Scattering LUT (functions.glsl:773-976).
+// ---------------------------------------------------------------------------
+
+// Texel-space coordinates of the packed 4D lookup: z (r), y (mu), the mu_s
+// coordinate within a nu slice (x_in), and the nu slice coordinate.
+struct SCoords {
+  double z, y, x_in, nu;
+};
+
+// functions.glsl:781-787 (u_r), as GL texel-space z.
+ATMO_HD double s_r_texel(const Job& j, double r) {
+  double rho = safe_sqrt(r * r - j.rb * j.rb);
+  return unit_to_texel(rho / j.H, j.nr);
+}
+
+// functions.glsl:789-812 (u_mu), as GL texel-space y. The ground and sky
+// halves each map to MU/2 texels.
+ATMO_HD double s_mu_texel(const Job& j, double r, double mu, bool ground) {
+  double rho = safe_sqrt(r * r - j.rb * j.rb);
+  double r_mu = r * mu;
+  double disc = r_mu * r_mu - r * r + j.rb * j.rb;
+  const int half = j.nmu / 2;
+  double u_mu;
+  if (ground) {
+    double d = -r_mu - safe_sqrt(disc);
+    double d_min = r - j.rb;
+    double d_max = rho;
+    double x = d_max == d_min ? 0.0 : (d - d_min) / (d_max - d_min);
+    u_mu = 0.5 - 0.5 * (0.5 / half + x * (1.0 - 1.0 / half));
+  } else {
+    double d = -r_mu + safe_sqrt(disc + j.H * j.H);
+    double d_min = j.rt - r;
+    double d_max = rho + j.H;
+    double x = (d - d_min) / (d_max - d_min);
+    u_mu = 0.5 + 0.5 * (0.5 / half + x * (1.0 - 1.0 / half));
+  }
+  return u_mu * j.nmu - 0.5;
+}
+
+// functions.glsl:814-827 (u_mu_s), as texel-space x within a nu slice.
+ATMO_HD double s_mus_texel(const Job& j, double mu_s) {
+  double d = dist_top(j, j.rb, mu_s);
+  double d_min = j.rt - j.rb;
+  double d_max = j.H;
+  double a = (d - d_min) / (d_max - d_min);
+  double x_mu_s = (1.0 - a / j.A > 0.0 ? 1.0 - a / j.A : 0.0) / (1.0 + a);
+  return unit_to_texel(x_mu_s, j.nmus);
+}
+
+// functions.glsl:773-831, with u_nu expressed as the nu slice coordinate
+// u_nu * (NU - 1) of functions.glsl:967.
+ATMO_HD SCoords s_coords(const Job& j, double r, double mu, double mu_s,
+                         double nu, bool ground) {
+  SCoords c;
+  c.z = s_r_texel(j, r);
+  c.y = s_mu_texel(j, r, mu, ground);
+  c.x_in = s_mus_texel(j, mu_s);
+  c.nu = (nu + 1.0) / 2.0 * (j.nnu - 1);
+  return c;
+}
+
+// GetScattering (functions.glsl:958-976) with the GL texture() of each slice:
+// the two nu slices are read with their own trilinear (mu_s, mu, r) filter.
+// A slice index past the last one only ever gets weight 0 (the reference's
+// x > 1 coordinate is clamped by the sampler); it is clamped here too.
+ATMO_HD float4 s_lookup(const Job& j, const float4* lut, const SCoords& c) {
+  double cnu = c.nu;
+  if (!(cnu >= -1.0)) cnu = -1.0;
+  if (cnu > static_cast<double>(j.nnu)) cnu = static_cast<double>(j.nnu);
+  double f = floor(cnu);
+  float ws = static_cast<float>(cnu - f);
+  int s0 = static_cast<int>(f);
+  int s1 = s0 + 1;
+  s0 = s0 < 0 ? 0 : (s0 > j.nnu - 1 ? j.nnu - 1 : s0);
+  s1 = s1 < 0 ? 0 : (s1 > j.nnu - 1 ? j.nnu - 1 : s1);
+  Taps tz = axis_taps(c.z, j.nr), ty = axis_taps(c.y, j.nmu),
+       tx = axis_taps(c.x_in, j.nmus);
+  const int w = j.nnu * j.nmus;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int zs[2] = {tz.i0, tz.i1};
+  const int ys[2] = {ty.i0, ty.i1};
+  const int xs[2] = {tx.i0, tx.i1};
+  const float wz[2] = {1.0f - tz.w, tz.w};
+  const float wy[2] = {1.0f - ty.w, ty.w};
+  const float wx[2] = {1.0f - tx.w, tx.w};
+  const int ss[2] = {s0, s1};
+  const float wsl[2] = {1.0f - ws, ws};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int e = 0; e < 2; ++e)
+        for (int q = 0; q < 2; ++q) {
+          float wt = wz[a] * wy[b] * wx[e] * wsl[q];
+          float4 v = lut[(static_cast<int64_t>(zs[a]) * j.nmu + ys[b]) * w +
+                         ss[q] * j.nmus + xs[e]];
+          acc.x += wt * v.x;
+          acc.y += wt * v.y;
+          acc.z += wt * v.z;
+          acc.w += wt * v.w;
+        }
+  return acc;
+}
+
+// Inverse mappings of a scattering texel (functions.glsl:837-890 and
+// :905-926, frag_coord at the texel centre), split per axis.
+
+// r of layer z; rho = sqrt(r^2 - rb^2).
+ATMO_HD void s_layer_r(const Job& j, int z, double* r, double* rho) {
+  *rho = j.H * unit_from_tex((z + 0.5) / j.nr, j.nr);
+  *r = sqrt(*rho * *rho + j.rb * j.rb);
+}
+
+// mu and ground flag of row y at radius r.
+ATMO_HD void s_row_mu(const Job& j, int y, double r, double rho, double* mu,
+                      bool* ground) {
+  const double u_mu = (y + 0.5) / j.nmu;
+  const int half = j.nmu / 2;
+  if (u_mu < 0.5) {
+    double d_min = r - j.rb;
+    double d_max = rho;
+    double d = d_min + (d_max - d_min) * unit_from_tex(1.0 - 2.0 * u_mu, half);
+    *mu = d == 0.0 ? -1.0 : clamp_cos(-(rho * rho + d * d) / (2.0 * r * d));
+    *ground = true;
+  } else {
+    double d_min = j.rt - r;
+    double d_max = rho + j.H;
+    double d = d_min + (d_max - d_min) * unit_from_tex(2.0 * u_mu - 1.0, half);
+    *mu = d == 0.0 ? 1.0
+                   : clamp_cos((j.H * j.H - rho * rho - d * d) / (2.0 * r * d));
+    *ground = false;
+  }
+}
+
+// mu_s of column i (within a nu slice).
+ATMO_HD double s_col_mus(const Job& j, int i) {
+  double x_mu_s = unit_from_tex((i + 0.5) / j.nmus, j.nmus);
+  double d_min = j.rt - j.rb;
+  double d_max = j.H;
+  double a = (j.A - x_mu_s * j.A) / (1.0 + x_mu_s * j.A);
+  double d = d_min + (a < j.A ? a : j.A) * (d_max - d_min);
+  return d == 0.0 ? 1.0 : clamp_cos((j.H * j.H - d * d) / (2.0 * j.rb * d));
+}
+
+// nu of slice k (sampled endpoint-exact over NU - 1), clamped to the range
+// allowed by mu and mu_s.
+ATMO_HD double s_slice_nu(const Job& j, int k, double mu, double mu_s) {
+  double n = clamp_cos(static_cast<double>(k) / (j.nnu - 1) * 2.0 - 1.0);
+  double half_range = sqrt((1.0 - mu * mu) * (1.0 - mu_s * mu_s));
+  return clamp_d(n, mu * mu_s - half_range, mu * mu_s + half_range);
+}
+
+ATMO_HD void s_texel_params(const Job& j, int x, int y, int z, double* r,
+                            double* mu, double* mu_s, double* nu,
+                            bool* ground) {
+  double rho;
+  s_layer_r(j, z, r, &rho);
+  s_row_mu(j, y, *r, rho, mu, ground);
+  *mu_s = s_col_mus(j, x % j.nmus);
+  *nu = s_slice_nu(j, x / j.nmus, *mu, *mu_s);
+}
+
+// ---------------------------------------------------------------------------
+// Irradiance LUT (functions.glsl:1524-1601).
+// ---------------------------------------------------------------------------
+
+ATMO_HD void e_rmus_from_texel(const Job& j, int i, int k, double* r,
+                               double* mu_s) {
+  double x_mu_s = unit_from_tex((i + 0.5) / j.ew, j.ew);
+  double x_r = unit_from_tex((k + 0.5) / j.eh, j.eh);
+  *r = j.rb + x_r * (j.rt - j.rb);
+  *mu_s = clamp_cos(2.0 * x_mu_s - 1.0);
+}
+
+ATMO_HD float3 e_lookup(const Job& j, const float4* E, double r, double mu_s) {
+  double x_r = (r - j.rb) / (j.rt - j.rb);
+  double x_mu_s = mu_s * 0.5 + 0.5;
+  return bilinear(E, j.ew, j.eh, unit_to_texel(x_mu_s, j.ew),
+                  unit_to_texel(x_r, j.eh));
+}
+
+// functions.glsl:739-747
+ATMO_HD float rayleigh_phase(float nu) {
+  return 3.0f / (16.0f * 3.14159265358979f) * (1.0f + nu * nu);
+}
+ATMO_HD float mie_phase(float g, float nu) {
+  float k = 3.0f / (8.0f * 3.14159265358979f) * (1.0f - g * g) / (2.0f + g * g);
+  float y = 1.0f + g * g - 2.0f * g * nu;
+  return k * (1.0f + nu * nu) / (y * sqrtf(y));
+}
+
+ATMO_HD float3 apply_lum(const float* L, float3 v) {
+  return make_float3(L[0] * v.x + L[1] * v.y + L[2] * v.z,
+                     L[3] * v.x + L[4] * v.y + L[5] * v.z,
+                     L[6] * v.x + L[7] * v.y + L[8] * v.z);
+}
+
+}  // namespace atmo
+
+#endif  // ATMO_DEVICE_H_

@@ -1,0 +1,65 @@
+// Host-side launchers of the precompute and lookup kernels.
+#ifndef ATMO_KERNELS_H_
+#define ATMO_KERNELS_H_
+
+#include <hip/hip_runtime.h>
+
+#include "atmo_device.h"
+
+namespace atmo {
+
+// One launch of a pass over a set of jobs. Block b of the launch belongs to
+// job i with first_block[i] <= b < first_block[i + 1] (device arrays).
+struct PassLaunch {
+  const Job* jobs;
+  const int* first_block;
+  int njobs;
+  int total_blocks;
+  int max_nnu;  // largest nu size among the jobs (LDS sizing)
+  int max_ew;   // largest irradiance width among the jobs (LDS sizing)
+};
+
+constexpr int kBlock = 256;
+
+// Blocks one job needs for each pass.
+int blocks_transmittance(const Job& j);
+int blocks_direct_irradiance(const Job& j);
+int blocks_single_scattering(const Job& j);
+int blocks_scattering_density(const Job& j);
+int blocks_indirect_irradiance(const Job& j);
+int blocks_multiple_scattering(const Job& j);
+
+hipError_t launch_transmittance(const PassLaunch& p, hipStream_t s);
+hipError_t launch_direct_irradiance(const PassLaunch& p, hipStream_t s);
+hipError_t launch_single_scattering(const PassLaunch& p, hipStream_t s);
+// 'order' is the scattering order being computed (>= 2).
+hipError_t launch_scattering_density(const PassLaunch& p, int order,
+                                     hipStream_t s);
+// 'order' is the order of the scattering read by the pass (>= 1).
+hipError_t launch_indirect_irradiance(const PassLaunch& p, int order,
+                                      hipStream_t s);
+hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s);
+
+// Lookups (one ray / point per thread). 'job' is a device pointer.
+hipError_t launch_sky_radiance(const Job* job, int64_t n, const float* camera,
+                               const float* view_ray,
+                               const float* shadow_length,
+                               const float* sun_direction, float* radiance,
+                               float* transmittance, hipStream_t s);
+hipError_t launch_sky_radiance_to_point(const Job* job, int64_t n,
+                                        const float* camera,
+                                        const float* point,
+                                        const float* shadow_length,
+                                        const float* sun_direction,
+                                        float* radiance, float* transmittance,
+                                        hipStream_t s);
+hipError_t launch_sun_and_sky_irradiance(const Job* job, int64_t n,
+                                         const float* point,
+                                         const float* normal,
+                                         const float* sun_direction,
+                                         float* sun_irradiance,
+                                         float* sky_irradiance, hipStream_t s);
+
+}  // namespace atmo
+
+#endif  // ATMO_KERNELS_H_

@@ -1,0 +1,882 @@
+// Precompute kernels for gfx950: the six passes of Model::Precompute
+// (atmosphere/model.cc:1048-1215) as HIP kernels.
+//
+// Every kernel runs over the blocks of one or more jobs (atmosphere x
+// wavelength triple): block b belongs to the job whose block range contains
+// it (first_block prefix sums), so one launch serves a single model or a
+// batch of models with different LUT sizes. Blocks are 256 threads.
+#include <hip/hip_runtime.h>
+
+#include "atmo_device.h"
+#include "kernels.h"
+
+namespace atmo {
+namespace {
+
+// Block b's job: the i with first_block[i] <= b < first_block[i + 1]. The
+// search is uniform across the block.
+__device__ __forceinline__ int find_job(const int* __restrict__ first_block,
+                                        int njobs, int b) {
+  int lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (first_block[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ float rcp_f(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+__device__ __forceinline__ float rsq_f(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+__device__ __forceinline__ float sqrt_f(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
+__device__ __forceinline__ float clampf(float x, float lo, float hi) {
+  return fminf(fmaxf(x, lo), hi);
+}
+
+// cos and sin of (m + 1/2) * pi / 16 for m = 0..15, evaluated at compile time
+// (Taylor series in FP64, converged to the last bit for |x| <= pi).
+constexpr double cx_sin(double x) {
+  double term = x, sum = x;
+  for (int n = 1; n < 24; ++n) {
+    term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int n = 1; n < 24; ++n) {
+    term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+#define ATMO_PHI(m) ((m + 0.5) * 3.14159265358979323846 / 16.0)
+__device__ constexpr float kCosPhi[16] = {
+    float(cx_cos(ATMO_PHI(0))),  float(cx_cos(ATMO_PHI(1))),
+    float(cx_cos(ATMO_PHI(2))),  float(cx_cos(ATMO_PHI(3))),
+    float(cx_cos(ATMO_PHI(4))),  float(cx_cos(ATMO_PHI(5))),
+    float(cx_cos(ATMO_PHI(6))),  float(cx_cos(ATMO_PHI(7))),
+    float(cx_cos(ATMO_PHI(8))),  float(cx_cos(ATMO_PHI(9))),
+    float(cx_cos(ATMO_PHI(10))), float(cx_cos(ATMO_PHI(11))),
+    float(cx_cos(ATMO_PHI(12))), float(cx_cos(ATMO_PHI(13))),
+    float(cx_cos(ATMO_PHI(14))), float(cx_cos(ATMO_PHI(15)))};
+__device__ constexpr float kSinPhi[16] = {
+    float(cx_sin(ATMO_PHI(0))),  float(cx_sin(ATMO_PHI(1))),
+    float(cx_sin(ATMO_PHI(2))),  float(cx_sin(ATMO_PHI(3))),
+    float(cx_sin(ATMO_PHI(4))),  float(cx_sin(ATMO_PHI(5))),
+    float(cx_sin(ATMO_PHI(6))),  float(cx_sin(ATMO_PHI(7))),
+    float(cx_sin(ATMO_PHI(8))),  float(cx_sin(ATMO_PHI(9))),
+    float(cx_sin(ATMO_PHI(10))), float(cx_sin(ATMO_PHI(11))),
+    float(cx_sin(ATMO_PHI(12))), float(cx_sin(ATMO_PHI(13))),
+    float(cx_sin(ATMO_PHI(14))), float(cx_sin(ATMO_PHI(15)))};
+#undef ATMO_PHI
+
+// ---------------------------------------------------------------------------
+// Transmittance (functions.glsl:275-320, :454-463). FP64 throughout: the
+// 501-sample trapezoid of each texel is split over the 64 lanes of a wave and
+// reduced with cross-lane shuffles. 4 texels per block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_transmittance(
+    const Job* __restrict__ jobs, const int* __restrict__ first_block,
+    int njobs) {
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int lb = blockIdx.x - first_block[ji];
+  const int lane = threadIdx.x & 63;
+  const int texel = lb * (kBlock / 64) + (threadIdx.x >> 6);
+  if (texel >= J.tw * J.th) return;
+  const int i = texel % J.tw, k = texel / J.tw;
+  double r, mu;
+  t_rmu_from_uv(J, (i + 0.5) / J.tw, (k + 0.5) / J.th, &r, &mu);
+  constexpr int kIntervals = 500;
+  const double dx = dist_top(J, r, mu) / kIntervals;
+  double lr = 0.0, lm = 0.0, la = 0.0;
+  for (int s = lane; s <= kIntervals; s += 64) {
+    const double d = s * dx;
+    const double h = sqrt(d * d + 2.0 * r * mu * d + r * r) - J.rb;
+    const double w = (s == 0 || s == kIntervals) ? 0.5 : 1.0;
+    lr += w * profile_density(J.prof_r, h);
+    lm += w * profile_density(J.prof_m, h);
+    la += w * profile_density(J.prof_a, h);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lr += __shfl_xor(lr, off);
+    lm += __shfl_xor(lm, off);
+    la += __shfl_xor(la, off);
+  }
+  if (lane == 0) {
+    lr *= dx;
+    lm *= dx;
+    la *= dx;
+    float4 t;
+    t.x = static_cast<float>(exp(-(J.rayleigh[0] * lr + J.mie_ext[0] * lm +
+                                   J.absorption[0] * la)));
+    t.y = static_cast<float>(exp(-(J.rayleigh[1] * lr + J.mie_ext[1] * lm +
+                                   J.absorption[1] * la)));
+    t.z = static_cast<float>(exp(-(J.rayleigh[2] * lr + J.mie_ext[2] * lm +
+                                   J.absorption[2] * la)));
+    t.w = 0.0f;
+    J.T[texel] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Direct irradiance (functions.glsl:1443-1461, :1558-1567) into delta E; the
+// irradiance LUT is cleared unless blending (atmosphere/model.cc:132-140).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_direct_irradiance(
+    const Job* __restrict__ jobs, const int* __restrict__ first_block,
+    int njobs) {
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int texel = (blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
+  if (texel >= J.ew * J.eh) return;
+  double r, mu_s;
+  e_rmus_from_texel(J, texel % J.ew, texel / J.ew, &r, &mu_s);
+  const double alpha = J.sun_angular_radius;
+  const double avg_cos =
+      mu_s < -alpha ? 0.0
+                    : (mu_s > alpha ? mu_s
+                                    : (mu_s + alpha) * (mu_s + alpha) / (4.0 * alpha));
+  const float3 t = t_lookup(J, J.T, r, mu_s);
+  J.dE[texel] = make_float4(static_cast<float>(J.solar[0] * avg_cos) * t.x,
+                            static_cast<float>(J.solar[1] * avg_cos) * t.y,
+                            static_cast<float>(J.solar[2] * avg_cos) * t.z, 0.0f);
+  if (!J.blend) J.E[texel] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Single scattering (functions.glsl:650-730, :933-945) and the output packing
+// of atmosphere/model.cc:142-157.
+//
+// A block covers up to 256 texels of one packed row (fixed r layer and mu
+// row). Along a row only mu_s and nu change, so every quantity of the 51
+// ray-march samples that depends on (r, mu, d_i) alone - the view
+// transmittance, the densities, the transmittance-LUT row of the sample
+// radius - is computed once per block in FP64 and kept in LDS. Per texel and
+// sample what remains is the sun-side transmittance lookup at
+// (r_d, mu_s_d) and the visible-disc smoothstep.
+// ---------------------------------------------------------------------------
+constexpr int kMarch = 51;  // 50 trapezoid intervals
+
+struct SsSample {
+  double d;        // distance along the view ray
+  double inv_r;    // 1 / r_d
+  double rho;      // sqrt(r_d^2 - rb^2)
+  float rf;        // r_d
+  float c;         // rt^2 - r_d^2
+  float dmin;      // rt - r_d
+  float inv_range; // 1 / (rho + H - dmin)
+  float e;         // sin(theta_h) * sun angular radius
+  float inv_span;  // 1 / (2 e)
+  int y0, y1;      // transmittance LUT rows of r_d
+  float wy;
+  float fr[3];     // trapezoid weight * view transmittance * Rayleigh density
+  float fm[3];     // same with the Mie density
+};
+
+__device__ __forceinline__ float3 tsun_lookup(const Job& J, const SsSample& S,
+                                              float mus) {
+  // functions.glsl:402-421 at (r_d, mu_s_d): DistanceToTopAtmosphereBoundary
+  // in the cancellation-free form, then the LUT column coordinate.
+  const float tt = S.rf * mus;
+  const float sq = sqrt_f(fmaf(tt, tt, S.c));
+  const float dd = tt > 0.0f ? S.c * rcp_f(sq + tt) : sq - tt;
+  const float xm = (dd - S.dmin) * S.inv_range;
+  const float ut = clamp_index_coord(xm * static_cast<float>(J.tw - 1),
+                                     static_cast<float>(J.tw - 1));
+  const float fl = floorf(ut);
+  const float w = ut - fl;
+  const int i0 = static_cast<int>(fl);
+  const int i1 = min(i0 + 1, J.tw - 1);
+  const float4* __restrict__ T = J.T;
+  const float4 a = T[S.y0 * J.tw + i0], b = T[S.y0 * J.tw + i1];
+  const float4 c = T[S.y1 * J.tw + i0], d = T[S.y1 * J.tw + i1];
+  const float wy1 = S.wy, wy0 = 1.0f - S.wy;
+  float3 o;
+  o.x = wy0 * fmaf(w, b.x - a.x, a.x) + wy1 * fmaf(w, d.x - c.x, c.x);
+  o.y = wy0 * fmaf(w, b.y - a.y, a.y) + wy1 * fmaf(w, d.y - c.y, c.y);
+  o.z = wy0 * fmaf(w, b.z - a.z, a.z) + wy1 * fmaf(w, d.z - c.z, c.z);
+  return o;
+}
+
+__global__ __launch_bounds__(kBlock) void k_single_scattering(
+    const Job* __restrict__ jobs, const int* __restrict__ first_block,
+    int njobs) {
+  __shared__ SsSample smp[kMarch];
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int lb = blockIdx.x - first_block[ji];
+  const int W = J.nnu * J.nmus;
+  const int tiles = (W + kBlock - 1) / kBlock;
+  const int tile = lb % tiles;
+  const int row = lb / tiles;  // z * nmu + y
+  const int y = row % J.nmu, z = row / J.nmu;
+  const int tid = threadIdx.x;
+
+  double r, rho, mu;
+  bool ground;
+  s_layer_r(J, z, &r, &rho);
+  s_row_mu(J, y, r, rho, &mu, &ground);
+  const double dx = dist_nearest(J, r, mu, ground) / (kMarch - 1);
+
+  if (tid < kMarch) {
+    SsSample& S = smp[tid];
+    const double d = tid * dx;
+    const double r_d = clamp_radius(J, sqrt(d * d + 2.0 * r * mu * d + r * r));
+    const float3 tv = transmittance(J, J.T, r, mu, d, ground);
+    const double rho_d = safe_sqrt(r_d * r_d - J.rb * J.rb);
+    S.d = d;
+    S.inv_r = 1.0 / r_d;
+    S.rho = rho_d;
+    S.rf = static_cast<float>(r_d);
+    S.c = static_cast<float>(J.rt * J.rt - r_d * r_d);
+    S.dmin = static_cast<float>(J.rt - r_d);
+    S.inv_range = static_cast<float>(1.0 / (rho_d + J.H - (J.rt - r_d)));
+    const double e = J.rb / r_d * J.sun_angular_radius;
+    S.e = static_cast<float>(e);
+    S.inv_span = static_cast<float>(0.5 / e);
+    const Taps ty = axis_taps(unit_to_texel(rho_d / J.H, J.th), J.th);
+    S.y0 = ty.i0;
+    S.y1 = ty.i1;
+    S.wy = ty.w;
+    const double w = (tid == 0 || tid == kMarch - 1) ? 0.5 : 1.0;
+    const float dr = static_cast<float>(w * profile_density(J.prof_r, r_d - J.rb));
+    const float dm = static_cast<float>(w * profile_density(J.prof_m, r_d - J.rb));
+    S.fr[0] = dr * tv.x;
+    S.fr[1] = dr * tv.y;
+    S.fr[2] = dr * tv.z;
+    S.fm[0] = dm * tv.x;
+    S.fm[1] = dm * tv.y;
+    S.fm[2] = dm * tv.z;
+  }
+  __syncthreads();
+
+  const int x = tile * kBlock + tid;
+  if (x >= W) return;
+  const double mu_s = s_col_mus(J, x % J.nmus);
+  const double nu = s_slice_nu(J, x / J.nmus, mu, mu_s);
+  const double rmus = r * mu_s;
+  float ar0 = 0.f, ar1 = 0.f, ar2 = 0.f, am0 = 0.f, am1 = 0.f, am2 = 0.f;
+  for (int s = 0; s < kMarch; ++s) {
+    const SsSample& S = smp[s];
+    const double t = fma(S.d, nu, rmus);  // r * mu_s + d * nu = r_d * mu_s_d
+    const float xs = static_cast<float>((t + S.rho) * S.inv_r);  // mu_s_d - cos(theta_h)
+    const float mus = clampf(static_cast<float>(t * S.inv_r), -1.0f, 1.0f);
+    const float u = clampf((xs + S.e) * S.inv_span, 0.0f, 1.0f);
+    const float vis = u * u * (3.0f - 2.0f * u);
+    const float3 ts = tsun_lookup(J, S, mus);
+    const float f0 = ts.x * vis, f1 = ts.y * vis, f2 = ts.z * vis;
+    ar0 = fmaf(f0, S.fr[0], ar0);
+    ar1 = fmaf(f1, S.fr[1], ar1);
+    ar2 = fmaf(f2, S.fr[2], ar2);
+    am0 = fmaf(f0, S.fm[0], am0);
+    am1 = fmaf(f1, S.fm[1], am1);
+    am2 = fmaf(f2, S.fm[2], am2);
+  }
+  const float3 ray = make_float3(ar0 * static_cast<float>(dx * J.solar[0] * J.rayleigh[0]),
+                                 ar1 * static_cast<float>(dx * J.solar[1] * J.rayleigh[1]),
+                                 ar2 * static_cast<float>(dx * J.solar[2] * J.rayleigh[2]));
+  const float3 mie = make_float3(am0 * static_cast<float>(dx * J.solar[0] * J.mie_sca[0]),
+                                 am1 * static_cast<float>(dx * J.solar[1] * J.mie_sca[1]),
+                                 am2 * static_cast<float>(dx * J.solar[2] * J.mie_sca[2]));
+  const int64_t t_idx = static_cast<int64_t>(row) * W + x;
+  J.dR[t_idx] = make_float4(ray.x, ray.y, ray.z, 0.0f);
+  J.dM[t_idx] = make_float4(mie.x, mie.y, mie.z, 0.0f);
+  const float3 lr = apply_lum(J.lum, ray);
+  const float3 lm = apply_lum(J.lum, mie);
+  float4 s = make_float4(lr.x, lr.y, lr.z, lm.x);
+  if (J.blend) {
+    const float4 o = J.S[t_idx];
+    s.x += o.x;
+    s.y += o.y;
+    s.z += o.z;
+    s.w += o.w;
+  }
+  J.S[t_idx] = s;
+  if (!J.combined) {
+    float4 m = make_float4(lm.x, lm.y, lm.z, 0.0f);
+    if (J.blend) {
+      const float4 o = J.SM[t_idx];
+      m.x += o.x;
+      m.y += o.y;
+      m.z += o.z;
+    }
+    J.SM[t_idx] = m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scattering density (functions.glsl:1163-1260, :1348-1367).
+//
+// A block covers up to 256 texels (mu, nu) of one (r layer, mu_s column)
+// plane. All lookups of the 16 x 32 incident directions are made at the
+// texel's own r and mu_s (whose LUT coordinates round-trip to the texel
+// centres), and for incident zenith angle theta_l at the fixed mu coordinate
+// u_mu(r, cos theta_l). The 4D lookup therefore reduces to a linear
+// interpolation in nu of a per-theta table of NU values (the two mu rows
+// already blended), built once per block in LDS. The 32 azimuths pair up as
+// phi and 2 pi - phi, which share the phase function of the outgoing
+// direction. Ground-hitting theta_l add the ground-reflected term read from a
+// table of delta-irradiance row 0 (r = rb).
+// ---------------------------------------------------------------------------
+constexpr int kTheta = 16;
+
+// Table entry i: (v.x, v.y, v.z, dv.x), (dv.y, dv.z, -, -) with
+// dv = v[i + 1] - v[i] (0 for the last entry).
+__device__ __forceinline__ float3 table_lerp(const float4* __restrict__ tab,
+                                             float t_max, float t) {
+  t = clamp_index_coord(t, t_max);
+  const float fl = floorf(t);
+  const float w = t - fl;
+  const int i = static_cast<int>(fl);
+  const float4 e0 = tab[2 * i];
+  const float4 e1 = tab[2 * i + 1];
+  return make_float3(fmaf(w, e0.w, e0.x), fmaf(w, e1.x, e0.y),
+                     fmaf(w, e1.y, e0.z));
+}
+
+struct SdTheta {
+  float cos_t, sin_t;
+  float ga, gb;   // ground normal . sun = ga + gb * nu1
+  float gf[3];    // transmittance to ground * albedo / pi * d_omega
+  int ground;
+};
+
+template <bool kOrder2, bool kGround>
+__device__ __forceinline__ void sd_theta(
+    const float4* __restrict__ tab_a, const float4* __restrict__ tab_b,
+    const float4* __restrict__ etab, float nu_max, float e_max,
+    float A, float B, float C, float P, float Q, float g1, float mg2,
+    const float cR[3], const float cM[3], const SdTheta& th, float& acc0,
+    float& acc1, float& acc2) {
+  const float hn = 0.5f * nu_max;
+  const float he = 0.5f * e_max;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const float c = kCosPhi[m], s = kSinPhi[m];
+    // Phase function of the scattering towards -omega, shared by phi and
+    // 2 pi - phi.
+    const float nu2 = fmaf(Q, c, P);
+    const float x2 = fmaf(nu2, nu2, 1.0f);
+    const float rs = rsq_f(fmaf(mg2, nu2, g1));
+    const float rs3 = rs * rs * rs;
+    const float p0 = x2 * fmaf(cM[0], rs3, cR[0]);
+    const float p1 = x2 * fmaf(cM[1], rs3, cR[1]);
+    const float p2 = x2 * fmaf(cM[2], rs3, cR[2]);
+    const float base = fmaf(B, c, A);
+    float l0 = 0.f, l1 = 0.f, l2 = 0.f;
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const float nu1 = sgn == 0 ? fmaf(C, s, base) : fmaf(-C, s, base);
+      float3 v = table_lerp(tab_a, nu_max, fmaf(nu1, hn, hn));
+      if (kOrder2) {
+        // Order-1 incident radiance: Rayleigh and Mie single scattering
+        // times their phase functions of nu1 (constants folded in tables).
+        const float3 vm = table_lerp(tab_b, nu_max, fmaf(nu1, hn, hn));
+        const float r1 = rsq_f(fmaf(mg2, nu1, g1));
+        const float q = r1 * r1 * r1;
+        const float x1 = fmaf(nu1, nu1, 1.0f);
+        v.x = x1 * fmaf(vm.x, q, v.x);
+        v.y = x1 * fmaf(vm.y, q, v.y);
+        v.z = x1 * fmaf(vm.z, q, v.z);
+      }
+      if (kGround) {
+        const float mug = fmaf(th.gb, nu1, th.ga);
+        const float3 e = table_lerp(etab, e_max, fmaf(mug, he, he));
+        v.x = fmaf(th.gf[0], e.x, v.x);
+        v.y = fmaf(th.gf[1], e.y, v.y);
+        v.z = fmaf(th.gf[2], e.z, v.z);
+      }
+      l0 += v.x;
+      l1 += v.y;
+      l2 += v.z;
+    }
+    acc0 = fmaf(l0, p0, acc0);
+    acc1 = fmaf(l1, p1, acc1);
+    acc2 = fmaf(l2, p2, acc2);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scattering_density(
+    const Job* __restrict__ jobs, const int* __restrict__ first_block,
+    int njobs, int order) {
+  extern __shared__ float4 lds[];
+  __shared__ SdTheta th[kTheta];
+  __shared__ int s_y0[kTheta], s_y1[kTheta];
+  __shared__ float s_wy[kTheta], s_dw[kTheta];
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int lb = blockIdx.x - first_block[ji];
+  const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
+  const int plane = nmu * nnu;
+  const int tiles = (plane + kBlock - 1) / kBlock;
+  const int tile = lb % tiles;
+  const int pl = lb / tiles;
+  const int imus = pl % nmus, k = pl / nmus;
+  const int W = nnu * nmus;
+  const bool two = order == 2;
+  float4* tab_a = lds;
+  float4* tab_b = lds + 2 * kTheta * nnu;
+  float4* etab = lds + (two ? 4 : 2) * kTheta * nnu;
+  const int tid = threadIdx.x;
+
+  double r, rho;
+  s_layer_r(J, k, &r, &rho);
+  const double mu_s = s_col_mus(J, imus);
+
+  if (tid < kTheta) {
+    const double dtheta = kPi / kTheta;
+    const double theta = (tid + 0.5) * dtheta;
+    const double ct = cos(theta), st = sin(theta);
+    const bool g = hits_ground(J, r, ct);
+    const Taps ty = axis_taps(s_mu_texel(J, r, ct, g), nmu);
+    s_y0[tid] = ty.i0;
+    s_y1[tid] = ty.i1;
+    s_wy[tid] = ty.w;
+    const double dw = dtheta * (kPi / kTheta) * st;
+    s_dw[tid] = static_cast<float>(dw);
+    SdTheta& t = th[tid];
+    t.cos_t = static_cast<float>(ct);
+    t.sin_t = static_cast<float>(st);
+    t.ground = g ? 1 : 0;
+    if (g) {
+      const double dg = dist_bottom(J, r, ct);
+      const float3 tg = transmittance(J, J.T, r, ct, dg, true);
+      const double len = sqrt(r * r + 2.0 * r * dg * ct + dg * dg);
+      t.ga = static_cast<float>(r * mu_s / len);
+      t.gb = static_cast<float>(dg / len);
+      t.gf[0] = static_cast<float>(tg.x * J.albedo[0] / kPi * dw);
+      t.gf[1] = static_cast<float>(tg.y * J.albedo[1] / kPi * dw);
+      t.gf[2] = static_cast<float>(tg.z * J.albedo[2] / kPi * dw);
+    } else {
+      t.ga = t.gb = 0.0f;
+      t.gf[0] = t.gf[1] = t.gf[2] = 0.0f;
+    }
+  }
+  __syncthreads();
+
+  // Per-theta nu tables of the order-(n-1) scattering at (r, mu_s), mu rows
+  // blended, times d_omega (and the phase-function constants for order 1).
+  const float kR = static_cast<float>(3.0 / (16.0 * kPi));
+  const double gd = J.g;
+  const float kM = static_cast<float>(3.0 / (8.0 * kPi) * (1.0 - gd * gd) / (2.0 + gd * gd));
+  const int64_t layer = static_cast<int64_t>(k) * nmu;
+  for (int e = tid; e < kTheta * nnu; e += kBlock) {
+    const int l = e / nnu, n = e % nnu;
+    const int64_t r0 = (layer + s_y0[l]) * W, r1 = (layer + s_y1[l]) * W;
+    const int c0 = n * nmus + imus;
+    const int c1 = (n + 1 < nnu ? n + 1 : n) * nmus + imus;
+    const float wy = s_wy[l], wy0 = 1.0f - wy;
+    const float fa = s_dw[l] * (two ? kR : 1.0f);
+    const float4* src = J.dR;
+    float4 a0 = src[r0 + c0], a1 = src[r1 + c0];
+    float4 b0 = src[r0 + c1], b1 = src[r1 + c1];
+    float3 v = make_float3((wy0 * a0.x + wy * a1.x) * fa, (wy0 * a0.y + wy * a1.y) * fa,
+                           (wy0 * a0.z + wy * a1.z) * fa);
+    float3 vn = make_float3((wy0 * b0.x + wy * b1.x) * fa, (wy0 * b0.y + wy * b1.y) * fa,
+                            (wy0 * b0.z + wy * b1.z) * fa);
+    tab_a[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
+    tab_a[2 * e + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
+    if (two) {
+      const float fb = s_dw[l] * kM;
+      src = J.dM;
+      a0 = src[r0 + c0];
+      a1 = src[r1 + c0];
+      b0 = src[r0 + c1];
+      b1 = src[r1 + c1];
+      v = make_float3((wy0 * a0.x + wy * a1.x) * fb, (wy0 * a0.y + wy * a1.y) * fb,
+                      (wy0 * a0.z + wy * a1.z) * fb);
+      vn = make_float3((wy0 * b0.x + wy * b1.x) * fb, (wy0 * b0.y + wy * b1.y) * fb,
+                       (wy0 * b0.z + wy * b1.z) * fb);
+      tab_b[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
+      tab_b[2 * e + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
+    }
+  }
+  // Row 0 of delta irradiance (r = rb): the ground-irradiance lookup of the
+  // reference at bottom_radius reads exactly this row.
+  for (int e = tid; e < J.ew; e += kBlock) {
+    const float4 v = J.dE[e];
+    const float4 vn = J.dE[e + 1 < J.ew ? e + 1 : e];
+    etab[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
+    etab[2 * e + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
+  }
+  __syncthreads();
+
+  const int flat = tile * kBlock + tid;
+  if (flat >= plane) return;
+  const int nui = flat % nnu, y = flat / nnu;
+  double mu;
+  bool ground_unused;
+  s_row_mu(J, y, r, rho, &mu, &ground_unused);
+  const double nu = s_slice_nu(J, nui, mu, mu_s);
+  // functions.glsl:1181-1185
+  const double wx = sqrt(1.0 - mu * mu);
+  const double sx = wx == 0.0 ? 0.0 : (nu - mu * mu_s) / wx;
+  const double sy = sqrt(fmax(1.0 - sx * sx - mu_s * mu_s, 0.0));
+  const float fmus = static_cast<float>(mu_s), fsx = static_cast<float>(sx),
+              fsy = static_cast<float>(sy), fmu = static_cast<float>(mu),
+              fwx = static_cast<float>(wx);
+  const double h = r - J.rb;
+  const double dr = profile_density(J.prof_r, h);
+  const double dm = profile_density(J.prof_m, h);
+  float cR[3], cM[3];
+  for (int c = 0; c < 3; ++c) {
+    cR[c] = static_cast<float>(J.rayleigh[c] * dr * (3.0 / (16.0 * kPi)));
+    cM[c] = static_cast<float>(J.mie_sca[c] * dm * (3.0 / (8.0 * kPi)) *
+                               (1.0 - gd * gd) / (2.0 + gd * gd));
+  }
+  const float g1 = static_cast<float>(1.0 + gd * gd);
+  const float mg2 = static_cast<float>(-2.0 * gd);
+  const float nu_max = static_cast<float>(nnu - 1);
+  const float e_max = static_cast<float>(J.ew - 1);
+  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+  for (int l = 0; l < kTheta; ++l) {
+    const SdTheta t = th[l];
+    const float A = fmus * t.cos_t, B = fsx * t.sin_t, C = fsy * t.sin_t;
+    const float P = fmu * t.cos_t, Q = fwx * t.sin_t;
+    const float4* ta = tab_a + 2 * l * nnu;
+    const float4* tb = tab_b + 2 * l * nnu;
+    const bool g = __builtin_amdgcn_readfirstlane(t.ground) != 0;
+    if (two) {
+      if (g)
+        sd_theta<true, true>(ta, tb, etab, nu_max, e_max, A, B, C, P, Q, g1, mg2, cR,
+                             cM, t, acc0, acc1, acc2);
+      else
+        sd_theta<true, false>(ta, tb, etab, nu_max, e_max, A, B, C, P, Q, g1, mg2, cR,
+                              cM, t, acc0, acc1, acc2);
+    } else {
+      if (g)
+        sd_theta<false, true>(ta, tb, etab, nu_max, e_max, A, B, C, P, Q, g1, mg2, cR,
+                              cM, t, acc0, acc1, acc2);
+      else
+        sd_theta<false, false>(ta, tb, etab, nu_max, e_max, A, B, C, P, Q, g1, mg2, cR,
+                               cM, t, acc0, acc1, acc2);
+    }
+  }
+  J.dJ[(layer + y) * W + nui * nmus + imus] = make_float4(acc0, acc1, acc2, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Indirect irradiance (functions.glsl:1477-1511, :1573-1586) and the packing
+// of atmosphere/model.cc:176-190. One wave per irradiance texel: the wave
+// builds, for each of the 16 zenith angles, the nu table of the scattering at
+// (r, mu = cos theta, mu_s) (trilinear in r, mu, mu_s), then each lane takes
+// one of the 64 azimuths, sums over theta, and the wave reduces. One wave per
+// block, so the tables of a texel need at most 4 * 16 * NU float4 of LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_indirect_irradiance(
+    const Job* __restrict__ jobs, const int* __restrict__ first_block,
+    int njobs, int order) {
+  extern __shared__ float4 lds[];
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int lane = threadIdx.x;
+  const int texel = blockIdx.x - first_block[ji];
+  const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
+  const bool one = order == 1;
+  const int per_table = 2 * kTheta * nnu;
+  float4* tab_a = lds;
+  float4* tab_b = tab_a + per_table;
+  const bool active = texel < J.ew * J.eh;
+  double r = J.rb, mu_s = 1.0;
+  if (active) e_rmus_from_texel(J, texel % J.ew, texel / J.ew, &r, &mu_s);
+  const int W = nnu * nmus;
+  const double dtheta = kPi / 32.0, dphi = kPi / 32.0;
+  if (active) {
+    const Taps tz = axis_taps(s_r_texel(J, r), J.nr);
+    const Taps tx = axis_taps(s_mus_texel(J, mu_s), nmus);
+    const float kR = static_cast<float>(3.0 / (16.0 * kPi));
+    const double gd = J.g;
+    const float kM = static_cast<float>(3.0 / (8.0 * kPi) * (1.0 - gd * gd) / (2.0 + gd * gd));
+    for (int e = lane; e < kTheta * nnu; e += 64) {
+      const int jt = e / nnu, n = e % nnu;
+      const double theta = (jt + 0.5) * dtheta;
+      const double ct = cos(theta);
+      const Taps ty = axis_taps(s_mu_texel(J, r, ct, false), nmu);
+      const float f = static_cast<float>(ct * dtheta * dphi * sin(theta));
+      const int zs[2] = {tz.i0, tz.i1}, ys[2] = {ty.i0, ty.i1}, xs[2] = {tx.i0, tx.i1};
+      const float wz[2] = {1.f - tz.w, tz.w}, wy[2] = {1.f - ty.w, ty.w},
+                  wx[2] = {1.f - tx.w, tx.w};
+      const int n1 = n + 1 < nnu ? n + 1 : n;
+      for (int which = 0; which < (one ? 2 : 1); ++which) {
+        const float4* src = which == 0 ? J.dR : J.dM;
+        const float fk = f * (one ? (which == 0 ? kR : kM) : 1.0f);
+        float3 v = make_float3(0.f, 0.f, 0.f), vn = make_float3(0.f, 0.f, 0.f);
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+              const float w = wz[a] * wy[b] * wx[c];
+              const int64_t base = (static_cast<int64_t>(zs[a]) * nmu + ys[b]) * W + xs[c];
+              const float4 p = src[base + n * nmus], q = src[base + n1 * nmus];
+              v.x += w * p.x;
+              v.y += w * p.y;
+              v.z += w * p.z;
+              vn.x += w * q.x;
+              vn.y += w * q.y;
+              vn.z += w * q.z;
+            }
+        v.x *= fk;
+        v.y *= fk;
+        v.z *= fk;
+        vn.x *= fk;
+        vn.y *= fk;
+        vn.z *= fk;
+        float4* tab = which == 0 ? tab_a : tab_b;
+        tab[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
+        tab[2 * e + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
+      }
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const double phi = (lane + 0.5) * dphi;
+  const float wsx = static_cast<float>(sqrt(1.0 - mu_s * mu_s) * cos(phi));
+  const float fmus = static_cast<float>(mu_s);
+  const float nu_max = static_cast<float>(nnu - 1);
+  const float hn = 0.5f * nu_max;
+  const float g1 = static_cast<float>(1.0 + J.g * J.g);
+  const float mg2 = static_cast<float>(-2.0 * J.g);
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  for (int jt = 0; jt < kTheta; ++jt) {
+    const double theta = (jt + 0.5) * dtheta;
+    const float ct = static_cast<float>(cos(theta)), st = static_cast<float>(sin(theta));
+    const float nu = fmaf(st, wsx, ct * fmus);  // omega . omega_s
+    float3 v = table_lerp(tab_a + 2 * jt * nnu, nu_max, fmaf(nu, hn, hn));
+    if (one) {
+      const float3 vm = table_lerp(tab_b + 2 * jt * nnu, nu_max, fmaf(nu, hn, hn));
+      const float r1 = rsq_f(fmaf(mg2, nu, g1));
+      const float q = r1 * r1 * r1;
+      const float x1 = fmaf(nu, nu, 1.0f);
+      v.x = x1 * fmaf(vm.x, q, v.x);
+      v.y = x1 * fmaf(vm.y, q, v.y);
+      v.z = x1 * fmaf(vm.z, q, v.z);
+    }
+    a0 += v.x;
+    a1 += v.y;
+    a2 += v.z;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off);
+    a1 += __shfl_xor(a1, off);
+    a2 += __shfl_xor(a2, off);
+  }
+  if (lane == 0) {
+    const float3 de = make_float3(a0, a1, a2);
+    J.dE[texel] = make_float4(de.x, de.y, de.z, 0.0f);
+    const float3 le = apply_lum(J.lum, de);
+    float4 e = J.E[texel];
+    e.x += le.x;
+    e.y += le.y;
+    e.z += le.z;
+    J.E[texel] = e;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Multiple scattering (functions.glsl:1285-1330, :1369-1383) and the packing
+// of atmosphere/model.cc:192-208. Same row blocking as single scattering:
+// the LUT (r, mu) taps, their weights and the view transmittance of the 51
+// samples are per block; per texel and sample, the mu_s coordinate of
+// (r mu_s + d nu) / r_i and a 16-tap gather of the scattering density.
+// ---------------------------------------------------------------------------
+struct MsSample {
+  double d;
+  double inv_r;
+  int z0, z1, y0, y1;
+  float wz, wy;
+  float f[3];  // trapezoid weight * dx * view transmittance
+};
+
+__global__ __launch_bounds__(kBlock) void k_multiple_scattering(
+    const Job* __restrict__ jobs, const int* __restrict__ first_block,
+    int njobs) {
+  __shared__ MsSample smp[kMarch];
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int lb = blockIdx.x - first_block[ji];
+  const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
+  const int W = nnu * nmus;
+  const int tiles = (W + kBlock - 1) / kBlock;
+  const int tile = lb % tiles;
+  const int row = lb / tiles;
+  const int y = row % nmu, z = row / nmu;
+  const int tid = threadIdx.x;
+
+  double r, rho, mu;
+  bool ground;
+  s_layer_r(J, z, &r, &rho);
+  s_row_mu(J, y, r, rho, &mu, &ground);
+  const double dx = dist_nearest(J, r, mu, ground) / (kMarch - 1);
+  if (tid < kMarch) {
+    MsSample& S = smp[tid];
+    const double d = tid * dx;
+    const double r_i = clamp_radius(J, sqrt(d * d + 2.0 * r * mu * d + r * r));
+    const double mu_i = clamp_cos((r * mu + d) / r_i);
+    const Taps tz = axis_taps(s_r_texel(J, r_i), J.nr);
+    const Taps ty = axis_taps(s_mu_texel(J, r_i, mu_i, ground), nmu);
+    const float3 tv = transmittance(J, J.T, r, mu, d, ground);
+    const float w = static_cast<float>(((tid == 0 || tid == kMarch - 1) ? 0.5 : 1.0) * dx);
+    S.d = d;
+    S.inv_r = 1.0 / r_i;
+    S.z0 = tz.i0;
+    S.z1 = tz.i1;
+    S.wz = tz.w;
+    S.y0 = ty.i0;
+    S.y1 = ty.i1;
+    S.wy = ty.w;
+    S.f[0] = w * tv.x;
+    S.f[1] = w * tv.y;
+    S.f[2] = w * tv.z;
+  }
+  __syncthreads();
+
+  const int x = tile * kBlock + tid;
+  if (x >= W) return;
+  const int nui = x / nmus;
+  const double mu_s = s_col_mus(J, x % nmus);
+  const double nu = s_slice_nu(J, nui, mu, mu_s);
+  // nu slice taps (functions.glsl:967-973).
+  double tcx = (nu + 1.0) / 2.0 * (nnu - 1);
+  if (!(tcx >= -1.0)) tcx = -1.0;
+  if (tcx > nnu) tcx = nnu;
+  const double fl = floor(tcx);
+  const float ws = static_cast<float>(tcx - fl);
+  int s0 = static_cast<int>(fl);
+  int s1 = s0 + 1;
+  s0 = s0 < 0 ? 0 : (s0 > nnu - 1 ? nnu - 1 : s0);
+  s1 = s1 < 0 ? 0 : (s1 > nnu - 1 ? nnu - 1 : s1);
+  const int cs0 = s0 * nmus, cs1 = s1 * nmus;
+  const double rmus = r * mu_s;
+  // mu_s mapping constants (functions.glsl:814-827).
+  const float rbf = static_cast<float>(J.rb);
+  const float H2 = static_cast<float>(J.H * J.H);
+  const float dmin0 = static_cast<float>(J.rt - J.rb);
+  const float inv_range0 = static_cast<float>(1.0 / (J.H - (J.rt - J.rb)));
+  const float invA = static_cast<float>(1.0 / J.A);
+  const float xmax = static_cast<float>(nmus - 1);
+  const float4* __restrict__ D = J.dJ;
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  for (int s = 0; s < kMarch; ++s) {
+    const MsSample& S = smp[s];
+    const float mus = clampf(static_cast<float>(fma(S.d, nu, rmus) * S.inv_r), -1.0f, 1.0f);
+    const float tt = rbf * mus;
+    const float sq = sqrt_f(fmaf(tt, tt, H2));
+    const float dd = tt > 0.0f ? H2 * rcp_f(sq + tt) : sq - tt;
+    const float aa = (dd - dmin0) * inv_range0;
+    const float xm = fmaxf(fmaf(-aa, invA, 1.0f), 0.0f) * rcp_f(1.0f + aa);
+    const float xin = clamp_index_coord(xm * xmax, xmax);
+    const float fx = floorf(xin);
+    const float wx = xin - fx;
+    const int i0 = static_cast<int>(fx);
+    const int i1 = min(i0 + 1, nmus - 1);
+    const float wz1 = S.wz, wz0 = 1.0f - S.wz, wy1 = S.wy, wy0 = 1.0f - S.wy;
+    const float wrow[4] = {wz0 * wy0, wz0 * wy1, wz1 * wy0, wz1 * wy1};
+    const int64_t rows[4] = {(static_cast<int64_t>(S.z0) * nmu + S.y0) * W,
+                             (static_cast<int64_t>(S.z0) * nmu + S.y1) * W,
+                             (static_cast<int64_t>(S.z1) * nmu + S.y0) * W,
+                             (static_cast<int64_t>(S.z1) * nmu + S.y1) * W};
+    const float wc[4] = {(1.0f - ws) * (1.0f - wx), (1.0f - ws) * wx, ws * (1.0f - wx),
+                         ws * wx};
+    const int cols[4] = {cs0 + i0, cs0 + i1, cs1 + i0, cs1 + i1};
+    float j0 = 0.f, j1 = 0.f, j2 = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float4 v = D[rows[q] + cols[c]];
+        r0 = fmaf(wc[c], v.x, r0);
+        r1 = fmaf(wc[c], v.y, r1);
+        r2 = fmaf(wc[c], v.z, r2);
+      }
+      j0 = fmaf(wrow[q], r0, j0);
+      j1 = fmaf(wrow[q], r1, j1);
+      j2 = fmaf(wrow[q], r2, j2);
+    }
+    a0 = fmaf(j0, S.f[0], a0);
+    a1 = fmaf(j1, S.f[1], a1);
+    a2 = fmaf(j2, S.f[2], a2);
+  }
+  const int64_t t_idx = static_cast<int64_t>(row) * W + x;
+  J.dR[t_idx] = make_float4(a0, a1, a2, 0.0f);
+  const float3 l = apply_lum(J.lum, make_float3(a0, a1, a2));
+  const float inv_pr = static_cast<float>(1.0 / ((3.0 / (16.0 * kPi)) * (1.0 + nu * nu)));
+  float4 sv = J.S[t_idx];
+  sv.x += l.x * inv_pr;
+  sv.y += l.y * inv_pr;
+  sv.z += l.z * inv_pr;
+  J.S[t_idx] = sv;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launch geometry and launchers.
+// ---------------------------------------------------------------------------
+
+int blocks_transmittance(const Job& j) {
+  return (j.tw * j.th + kBlock / 64 - 1) / (kBlock / 64);
+}
+int blocks_direct_irradiance(const Job& j) {
+  return (j.ew * j.eh + kBlock - 1) / kBlock;
+}
+int blocks_single_scattering(const Job& j) {
+  const int w = j.nnu * j.nmus;
+  return ((w + kBlock - 1) / kBlock) * j.nmu * j.nr;
+}
+int blocks_scattering_density(const Job& j) {
+  const int plane = j.nmu * j.nnu;
+  return ((plane + kBlock - 1) / kBlock) * j.nmus * j.nr;
+}
+int blocks_indirect_irradiance(const Job& j) { return j.ew * j.eh; }
+int blocks_multiple_scattering(const Job& j) { return blocks_single_scattering(j); }
+
+hipError_t launch_transmittance(const PassLaunch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_transmittance, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                     p.jobs, p.first_block, p.njobs);
+  return hipGetLastError();
+}
+hipError_t launch_direct_irradiance(const PassLaunch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_direct_irradiance, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                     p.jobs, p.first_block, p.njobs);
+  return hipGetLastError();
+}
+hipError_t launch_single_scattering(const PassLaunch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_single_scattering, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                     p.jobs, p.first_block, p.njobs);
+  return hipGetLastError();
+}
+hipError_t launch_scattering_density(const PassLaunch& p, int order,
+                                     hipStream_t s) {
+  const size_t lds = sizeof(float4) *
+                     (static_cast<size_t>((order == 2 ? 4 : 2) * kTheta * p.max_nnu) +
+                      2 * static_cast<size_t>(p.max_ew));
+  hipLaunchKernelGGL(k_scattering_density, dim3(p.total_blocks), dim3(kBlock), lds, s,
+                     p.jobs, p.first_block, p.njobs, order);
+  return hipGetLastError();
+}
+hipError_t launch_indirect_irradiance(const PassLaunch& p, int order,
+                                      hipStream_t s) {
+  const size_t lds =
+      sizeof(float4) * static_cast<size_t>((order == 1 ? 4 : 2) * kTheta * p.max_nnu);
+  hipLaunchKernelGGL(k_indirect_irradiance, dim3(p.total_blocks), dim3(64), lds, s,
+                     p.jobs, p.first_block, p.njobs, order);
+  return hipGetLastError();
+}
+hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_multiple_scattering, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                     p.jobs, p.first_block, p.njobs);
+  return hipGetLastError();
+}
+
+}  // namespace atmo

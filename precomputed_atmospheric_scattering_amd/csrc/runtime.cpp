@@ -1,0 +1,960 @@
+// C ABI of libatmo (include/atmo.h): model objects, parameter lowering, the
+// Model::Init pass sequence, batched graph-captured Init, lookups.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/atmo.h"
+#include "atmo_device.h"
+#include "cie_1931.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+atmo_status fail(atmo_status s, const std::string& msg) {
+  g_last_error = msg;
+  return s;
+}
+
+#define ATMO_HIP(call)                                                       \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(e_ == hipErrorOutOfMemory ? ATMO_ERROR_OUT_OF_MEMORY       \
+                                            : ATMO_ERROR_DEVICE,             \
+                  std::string(#call) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+constexpr double kLambdaR = 680.0;
+constexpr double kLambdaG = 550.0;
+constexpr double kLambdaB = 440.0;
+constexpr int kLambdaMin = 360;
+constexpr int kLambdaMax = 830;
+
+// atmosphere/model.cc:535-552
+double interpolate(const std::vector<double>& wl, const std::vector<double>& f,
+                   double lambda) {
+  if (lambda < wl[0]) return f[0];
+  for (size_t i = 0; i + 1 < wl.size(); ++i) {
+    if (lambda < wl[i + 1]) {
+      double u = (lambda - wl[i]) / (wl[i + 1] - wl[i]);
+      return f[i] * (1.0 - u) + f[i + 1] * u;
+    }
+  }
+  return f.back();
+}
+
+// atmosphere/model.cc:521-533; column 1, 2, 3 = x_bar, y_bar, z_bar.
+double cie_value(double lambda, int column) {
+  if (lambda <= kLambdaMin || lambda >= kLambdaMax) return 0.0;
+  double u = (lambda - kLambdaMin) / atmo_cie::kCieLambdaStep;
+  int row = static_cast<int>(std::floor(u));
+  u -= row;
+  const double* t = column == 1 ? atmo_cie::kCieXBar
+                    : column == 2 ? atmo_cie::kCieYBar
+                                  : atmo_cie::kCieZBar;
+  return t[row] * (1.0 - u) + t[row + 1] * u;
+}
+
+struct Spec {
+  std::vector<double> wavelengths, solar, rayleigh, mie_sca, mie_ext, absorption,
+      albedo;
+  std::vector<atmo_density_layer> prof_r, prof_m, prof_a;
+  double sun_angular_radius, bottom_radius, top_radius, g, max_sun_zenith_angle,
+      length_unit;
+  unsigned num_precomputed_wavelengths;
+  bool combined, half_precision;
+};
+
+int num_iterations(const Spec& s) {
+  return s.num_precomputed_wavelengths <= 3
+             ? 1
+             : static_cast<int>((s.num_precomputed_wavelengths + 2) / 3);
+}
+
+// Wavelengths and luminance_from_radiance of iteration i
+// (atmosphere/model.cc:907-943).
+void iteration(const Spec& s, int i, double* lambdas, float* L) {
+  if (s.num_precomputed_wavelengths <= 3) {
+    lambdas[0] = kLambdaR;
+    lambdas[1] = kLambdaG;
+    lambdas[2] = kLambdaB;
+    for (int k = 0; k < 9; ++k) L[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+    return;
+  }
+  const int iters = num_iterations(s);
+  const double dlambda = (kLambdaMax - kLambdaMin) / (3.0 * iters);
+  for (int k = 0; k < 3; ++k) lambdas[k] = kLambdaMin + (3 * i + k + 0.5) * dlambda;
+  const double* m = atmo_cie::kXyzToSrgb;
+  for (int row = 0; row < 3; ++row)
+    for (int col = 0; col < 3; ++col) {
+      const double l = lambdas[col];
+      L[3 * row + col] = static_cast<float>(
+          (m[3 * row] * cie_value(l, 1) + m[3 * row + 1] * cie_value(l, 2) +
+           m[3 * row + 2] * cie_value(l, 3)) *
+          dlambda);
+    }
+}
+
+// atmosphere/model.cc:644-666 (layer scaling and front padding).
+void lower_profile(const std::vector<atmo_density_layer>& layers, double unit,
+                   atmo::LayerD* out) {
+  atmo_density_layer padded[2] = {};
+  const size_t n = std::min<size_t>(layers.size(), 2);
+  for (size_t i = 0; i < n; ++i) padded[2 - n + i] = layers[i];
+  for (int i = 0; i < 2; ++i) {
+    out[i].width = padded[i].width / unit;
+    out[i].exp_term = padded[i].exp_term;
+    out[i].exp_scale = padded[i].exp_scale * unit;
+    out[i].linear_term = padded[i].linear_term * unit;
+    out[i].constant_term = padded[i].constant_term;
+  }
+}
+
+// AtmosphereParameters at three wavelengths (atmosphere/model.cc:718-734),
+// in exact binary values (no text round trip).
+void lower(const Spec& s, const double* lambdas, atmo::Job* j) {
+  const double unit = s.length_unit;
+  for (int c = 0; c < 3; ++c) {
+    j->solar[c] = interpolate(s.wavelengths, s.solar, lambdas[c]);
+    j->rayleigh[c] = interpolate(s.wavelengths, s.rayleigh, lambdas[c]) * unit;
+    j->mie_sca[c] = interpolate(s.wavelengths, s.mie_sca, lambdas[c]) * unit;
+    j->mie_ext[c] = interpolate(s.wavelengths, s.mie_ext, lambdas[c]) * unit;
+    j->absorption[c] = interpolate(s.wavelengths, s.absorption, lambdas[c]) * unit;
+    j->albedo[c] = interpolate(s.wavelengths, s.albedo, lambdas[c]);
+  }
+  j->sun_angular_radius = s.sun_angular_radius;
+  j->rb = s.bottom_radius / unit;
+  j->rt = s.top_radius / unit;
+  j->mu_s_min = std::cos(s.max_sun_zenith_angle);
+  j->g = s.g;
+  lower_profile(s.prof_r, unit, j->prof_r);
+  lower_profile(s.prof_m, unit, j->prof_m);
+  lower_profile(s.prof_a, unit, j->prof_a);
+  j->H = std::sqrt(j->rt * j->rt - j->rb * j->rb);
+  // functions.glsl:814-821: A from the distance to the top at mu_s_min.
+  const double d_min = j->rt - j->rb, d_max = j->H;
+  const double D = atmo::dist_top(*j, j->rb, j->mu_s_min);
+  j->A = (D - d_min) / (d_max - d_min);
+}
+
+struct Sizes {
+  int tw, th, nr, nmu, nmus, nnu, ew, eh;
+  size_t t_texels() const { return static_cast<size_t>(tw) * th; }
+  size_t s_texels() const {
+    return static_cast<size_t>(nr) * nmu * nmus * nnu;
+  }
+  size_t e_texels() const { return static_cast<size_t>(ew) * eh; }
+};
+
+void set_sizes(atmo::Job* j, const Sizes& z) {
+  j->tw = z.tw;
+  j->th = z.th;
+  j->nr = z.nr;
+  j->nmu = z.nmu;
+  j->nmus = z.nmus;
+  j->nnu = z.nnu;
+  j->ew = z.ew;
+  j->eh = z.eh;
+}
+
+enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kPasses };
+
+int pass_blocks(Pass p, const atmo::Job& j) {
+  switch (p) {
+    case kT: return atmo::blocks_transmittance(j);
+    case kE0: return atmo::blocks_direct_irradiance(j);
+    case kSS: return atmo::blocks_single_scattering(j);
+    case kSD: return atmo::blocks_scattering_density(j);
+    case kII: return atmo::blocks_indirect_irradiance(j);
+    case kMS: return atmo::blocks_multiple_scattering(j);
+    default: return 0;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Model object.
+// ---------------------------------------------------------------------------
+
+struct atmo_model {
+  Spec spec;
+  Sizes sizes;
+  int device = 0;
+  bool retain = false;
+  bool initialized = false;
+  float4 *T = nullptr, *S = nullptr, *SM = nullptr, *E = nullptr;
+  float4 *dE = nullptr, *dR = nullptr, *dM = nullptr, *dJ = nullptr;
+  // Device job table: [iterations..., rgb transmittance job, query job].
+  std::vector<atmo::Job> jobs;
+  atmo::Job* d_jobs = nullptr;
+  // Per-pass block ranges of a single job: {0, blocks(pass)} for each pass.
+  int* d_first_block = nullptr;
+  int first_block_host[2 * kPasses] = {};
+  float solar_radiance[3] = {};
+};
+
+namespace {
+
+// Temporaries are released once the work that reads them has finished.
+void free_temporaries(atmo_model* m) {
+  float4** bufs[4] = {&m->dE, &m->dR, &m->dM, &m->dJ};
+  for (float4** b : bufs) {
+    if (*b != nullptr) {
+      (void)hipFree(*b);
+      *b = nullptr;
+    }
+  }
+}
+
+atmo_status alloc_temporaries(atmo_model* m) {
+  const Sizes& z = m->sizes;
+  if (m->dE == nullptr) ATMO_HIP(hipMalloc(&m->dE, z.e_texels() * sizeof(float4)));
+  if (m->dR == nullptr) ATMO_HIP(hipMalloc(&m->dR, z.s_texels() * sizeof(float4)));
+  if (m->dM == nullptr) ATMO_HIP(hipMalloc(&m->dM, z.s_texels() * sizeof(float4)));
+  if (m->dJ == nullptr) ATMO_HIP(hipMalloc(&m->dJ, z.s_texels() * sizeof(float4)));
+  return ATMO_OK;
+}
+
+void fill_job_luts(atmo_model* m, atmo::Job* j) {
+  j->T = m->T;
+  j->S = m->S;
+  j->SM = m->SM;
+  j->E = m->E;
+  j->dE = m->dE;
+  j->dR = m->dR;
+  j->dM = m->dM;
+  j->dJ = m->dJ;
+  set_sizes(j, m->sizes);
+  j->combined = m->spec.combined ? 1 : 0;
+}
+
+// Rebuild the host job table (LUT pointers may have changed) and upload it.
+atmo_status upload_jobs(atmo_model* m, unsigned orders, hipStream_t stream) {
+  const Spec& s = m->spec;
+  const int iters = num_iterations(s);
+  m->jobs.assign(iters + 2, atmo::Job{});
+  for (int i = 0; i < iters; ++i) {
+    double lambdas[3];
+    atmo::Job& j = m->jobs[i];
+    iteration(s, i, lambdas, j.lum);
+    lower(s, lambdas, &j);
+    fill_job_luts(m, &j);
+    j.blend = i > 0 ? 1 : 0;
+    j.orders = static_cast<int>(orders);
+  }
+  // Transmittance at 680/550/440 nm (atmosphere/model.cc:951-963) and the
+  // lookup parameters (the shader API header, model.cc:769-772).
+  const double rgb[3] = {kLambdaR, kLambdaG, kLambdaB};
+  for (int k = iters; k < iters + 2; ++k) {
+    atmo::Job& j = m->jobs[k];
+    lower(s, rgb, &j);
+    fill_job_luts(m, &j);
+    for (int q = 0; q < 9; ++q) j.lum[q] = (q % 4 == 0) ? 1.0f : 0.0f;
+  }
+  ATMO_HIP(hipMemcpyAsync(m->d_jobs, m->jobs.data(),
+                          m->jobs.size() * sizeof(atmo::Job),
+                          hipMemcpyHostToDevice, stream));
+  return ATMO_OK;
+}
+
+atmo::PassLaunch single_launch(const atmo_model* m, int job_index, Pass p) {
+  atmo::PassLaunch l;
+  l.jobs = m->d_jobs + job_index;
+  l.first_block = m->d_first_block + 2 * p;
+  l.njobs = 1;
+  l.total_blocks = m->first_block_host[2 * p + 1];
+  l.max_nnu = m->sizes.nnu;
+  l.max_ew = m->sizes.ew;
+  return l;
+}
+
+atmo_status check_launch(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    return fail(ATMO_ERROR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  return ATMO_OK;
+}
+
+#define ATMO_TRY(x)                      \
+  do {                                   \
+    atmo_status st_ = (x);               \
+    if (st_ != ATMO_OK) return st_;      \
+  } while (0)
+
+// Model::Precompute (atmosphere/model.cc:1048-1215) for job 'ji'.
+atmo_status enqueue_precompute(const atmo_model* m, int ji, unsigned orders,
+                               hipStream_t s) {
+  ATMO_TRY(check_launch(atmo::launch_transmittance(single_launch(m, ji, kT), s),
+                        "transmittance"));
+  ATMO_TRY(check_launch(
+      atmo::launch_direct_irradiance(single_launch(m, ji, kE0), s),
+      "direct irradiance"));
+  ATMO_TRY(check_launch(
+      atmo::launch_single_scattering(single_launch(m, ji, kSS), s),
+      "single scattering"));
+  for (unsigned order = 2; order <= orders; ++order) {
+    ATMO_TRY(check_launch(atmo::launch_scattering_density(
+                              single_launch(m, ji, kSD), static_cast<int>(order), s),
+                          "scattering density"));
+    ATMO_TRY(check_launch(
+        atmo::launch_indirect_irradiance(single_launch(m, ji, kII),
+                                         static_cast<int>(order) - 1, s),
+        "indirect irradiance"));
+    ATMO_TRY(check_launch(
+        atmo::launch_multiple_scattering(single_launch(m, ji, kMS), s),
+        "multiple scattering"));
+  }
+  return ATMO_OK;
+}
+
+atmo_status validate_layers(int n, const atmo_density_layer* l,
+                            const char* what) {
+  if (n < 0 || n > 2 || (n > 0 && l == nullptr))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT,
+                std::string(what) + ": at most 2 density layers");
+  return ATMO_OK;
+}
+
+atmo_status validate_sizes(const atmo_lut_sizes& z) {
+  const int v[8] = {z.transmittance_width, z.transmittance_height, z.scattering_r,
+                    z.scattering_mu,       z.scattering_mu_s,      z.scattering_nu,
+                    z.irradiance_width,    z.irradiance_height};
+  for (int x : v)
+    if (x < 2 || x > 4096)
+      return fail(ATMO_ERROR_INVALID_ARGUMENT, "LUT sizes must be in [2, 4096]");
+  if (z.scattering_mu % 2 != 0)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "scattering_mu must be even");
+  if (z.scattering_nu > 32)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "scattering_nu must be <= 32");
+  if (z.irradiance_width > 4096)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "irradiance_width too large");
+  const double texels = static_cast<double>(z.scattering_r) * z.scattering_mu *
+                        z.scattering_mu_s * z.scattering_nu;
+  if (texels > 1.0e9)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "scattering LUT too large");
+  return ATMO_OK;
+}
+
+bool finite_all(const double* v, int n) {
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+atmo_status validate_params(const atmo_model_params* p) {
+  if (p == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "params is NULL");
+  const int n = p->num_wavelengths;
+  if (n < 1 || p->wavelengths == nullptr || p->solar_irradiance == nullptr ||
+      p->rayleigh_scattering == nullptr || p->mie_scattering == nullptr ||
+      p->mie_extinction == nullptr || p->absorption_extinction == nullptr ||
+      p->ground_albedo == nullptr)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "missing spectrum");
+  for (int i = 0; i + 1 < n; ++i)
+    if (!(p->wavelengths[i] < p->wavelengths[i + 1]))
+      return fail(ATMO_ERROR_INVALID_ARGUMENT, "wavelengths must increase");
+  const double* spectra[7] = {p->wavelengths,     p->solar_irradiance,
+                              p->rayleigh_scattering, p->mie_scattering,
+                              p->mie_extinction,  p->absorption_extinction,
+                              p->ground_albedo};
+  for (const double* v : spectra)
+    if (!finite_all(v, n)) return fail(ATMO_ERROR_INVALID_ARGUMENT, "non-finite spectrum");
+  ATMO_TRY(validate_layers(p->num_rayleigh_layers, p->rayleigh_density, "rayleigh_density"));
+  ATMO_TRY(validate_layers(p->num_mie_layers, p->mie_density, "mie_density"));
+  ATMO_TRY(validate_layers(p->num_absorption_layers, p->absorption_density,
+                           "absorption_density"));
+  const double scalars[6] = {p->sun_angular_radius, p->bottom_radius, p->top_radius,
+                             p->mie_phase_function_g, p->max_sun_zenith_angle,
+                             p->length_unit_in_meters};
+  if (!finite_all(scalars, 6)) return fail(ATMO_ERROR_INVALID_ARGUMENT, "non-finite parameter");
+  if (!(p->bottom_radius > 0.0 && p->top_radius > p->bottom_radius))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "need 0 < bottom_radius < top_radius");
+  if (!(p->sun_angular_radius > 0.0))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "sun_angular_radius must be > 0");
+  if (!(p->length_unit_in_meters > 0.0))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "length_unit_in_meters must be > 0");
+  if (!(std::fabs(p->mie_phase_function_g) < 1.0))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "|mie_phase_function_g| must be < 1");
+  if (!(std::cos(p->max_sun_zenith_angle) < 1.0))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "max_sun_zenith_angle must be > 0");
+  return ATMO_OK;
+}
+
+void build_spec(const atmo_model_params* p, Spec* sp) {
+  Spec& s = *sp;
+  const int n = p->num_wavelengths;
+  auto vec = [n](const double* v) { return std::vector<double>(v, v + n); };
+  s.wavelengths = vec(p->wavelengths);
+  s.solar = vec(p->solar_irradiance);
+  s.rayleigh = vec(p->rayleigh_scattering);
+  s.mie_sca = vec(p->mie_scattering);
+  s.mie_ext = vec(p->mie_extinction);
+  s.absorption = vec(p->absorption_extinction);
+  s.albedo = vec(p->ground_albedo);
+  s.prof_r.assign(p->rayleigh_density, p->rayleigh_density + p->num_rayleigh_layers);
+  s.prof_m.assign(p->mie_density, p->mie_density + p->num_mie_layers);
+  s.prof_a.assign(p->absorption_density,
+                  p->absorption_density + p->num_absorption_layers);
+  s.sun_angular_radius = p->sun_angular_radius;
+  s.bottom_radius = p->bottom_radius;
+  s.top_radius = p->top_radius;
+  s.g = p->mie_phase_function_g;
+  s.max_sun_zenith_angle = p->max_sun_zenith_angle;
+  s.length_unit = p->length_unit_in_meters;
+  s.num_precomputed_wavelengths = p->num_precomputed_wavelengths;
+  s.combined = p->combine_scattering_textures != 0;
+  s.half_precision = p->half_precision != 0;
+}
+
+size_t lut_bytes(const atmo_model* m, bool with_single_mie) {
+  const Sizes& z = m->sizes;
+  size_t b = (z.t_texels() + z.e_texels() + z.s_texels()) * sizeof(float4);
+  if (with_single_mie) b += z.s_texels() * sizeof(float4);
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* atmo_last_error(void) { return g_last_error.c_str(); }
+
+atmo_status atmo_model_create(const atmo_model_params* p,
+                              const atmo_lut_sizes* sizes, atmo_model** out) {
+  if (out == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  ATMO_TRY(validate_params(p));
+  atmo_lut_sizes z = sizes != nullptr
+                         ? *sizes
+                         : atmo_lut_sizes{256, 64, 32, 128, 32, 8, 64, 16};
+  ATMO_TRY(validate_sizes(z));
+
+  // On any failure below, the partially built model is destroyed (which frees
+  // whatever was allocated).
+  std::unique_ptr<atmo_model, void (*)(atmo_model*)> m(new atmo_model(),
+                                                       atmo_model_destroy);
+  build_spec(p, &m->spec);
+  const Spec& s = m->spec;
+  m->sizes = Sizes{z.transmittance_width, z.transmittance_height, z.scattering_r,
+                   z.scattering_mu,       z.scattering_mu_s,      z.scattering_nu,
+                   z.irradiance_width,    z.irradiance_height};
+  // GetSolarRadiance of the GL API (atmosphere/model.cc:227-230), at the
+  // three RGB wavelengths.
+  const double rgb[3] = {kLambdaR, kLambdaG, kLambdaB};
+  for (int c = 0; c < 3; ++c)
+    m->solar_radiance[c] = static_cast<float>(
+        interpolate(s.wavelengths, s.solar, rgb[c]) /
+        (atmo::kPi * s.sun_angular_radius * s.sun_angular_radius));
+
+  ATMO_HIP(hipGetDevice(&m->device));
+  const Sizes& sz = m->sizes;
+  ATMO_HIP(hipMalloc(&m->T, sz.t_texels() * sizeof(float4)));
+  ATMO_HIP(hipMalloc(&m->S, sz.s_texels() * sizeof(float4)));
+  if (!s.combined) ATMO_HIP(hipMalloc(&m->SM, sz.s_texels() * sizeof(float4)));
+  ATMO_HIP(hipMalloc(&m->E, sz.e_texels() * sizeof(float4)));
+  const int iters = num_iterations(s);
+  ATMO_HIP(hipMalloc(&m->d_jobs, (iters + 2) * sizeof(atmo::Job)));
+  ATMO_HIP(hipMalloc(&m->d_first_block, 2 * kPasses * sizeof(int)));
+  atmo::Job probe{};
+  set_sizes(&probe, sz);
+  for (int p2 = 0; p2 < kPasses; ++p2) {
+    m->first_block_host[2 * p2] = 0;
+    m->first_block_host[2 * p2 + 1] = pass_blocks(static_cast<Pass>(p2), probe);
+  }
+  ATMO_HIP(hipMemcpy(m->d_first_block, m->first_block_host, sizeof(m->first_block_host),
+                     hipMemcpyHostToDevice));
+  ATMO_TRY(upload_jobs(m.get(), 4, nullptr));
+  ATMO_HIP(hipStreamSynchronize(nullptr));
+  *out = m.release();
+  return ATMO_OK;
+}
+
+void atmo_model_destroy(atmo_model* m) {
+  if (m == nullptr) return;
+  (void)hipDeviceSynchronize();
+  free_temporaries(m);
+  float4* bufs[4] = {m->T, m->S, m->SM, m->E};
+  for (float4* b : bufs)
+    if (b != nullptr) (void)hipFree(b);
+  if (m->d_jobs != nullptr) (void)hipFree(m->d_jobs);
+  if (m->d_first_block != nullptr) (void)hipFree(m->d_first_block);
+  delete m;
+}
+
+atmo_status atmo_model_set_option(atmo_model* m, atmo_option option, int value) {
+  if (m == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "model is NULL");
+  switch (option) {
+    case ATMO_OPTION_RETAIN_INTERMEDIATES:
+      m->retain = value != 0;
+      return ATMO_OK;
+  }
+  return fail(ATMO_ERROR_INVALID_ARGUMENT, "unknown option");
+}
+
+atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
+  if (m == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "model is NULL");
+  if (orders < 1) return fail(ATMO_ERROR_INVALID_ARGUMENT, "num_scattering_orders must be >= 1");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  ATMO_TRY(alloc_temporaries(m));
+  ATMO_TRY(upload_jobs(m, orders, s));
+  const int iters = num_iterations(m->spec);
+  for (int i = 0; i < iters; ++i) ATMO_TRY(enqueue_precompute(m, i, orders, s));
+  if (m->spec.num_precomputed_wavelengths > 3) {
+    ATMO_TRY(check_launch(atmo::launch_transmittance(single_launch(m, iters, kT), s),
+                          "transmittance (rgb)"));
+  }
+  m->initialized = true;
+  if (!m->retain) {
+    // Init then returns once the precomputation is complete, as the GL
+    // Init does once its draw calls have been issued and the temporaries
+    // deleted (atmosphere/model.cc:966-974).
+    ATMO_HIP(hipStreamSynchronize(s));
+    free_temporaries(m);
+  }
+  return ATMO_OK;
+}
+
+atmo_status atmo_model_get_lut(const atmo_model* m, atmo_lut_id id, atmo_lut_view* v) {
+  if (m == nullptr || v == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL argument");
+  const Sizes& z = m->sizes;
+  std::memset(v, 0, sizeof(*v));
+  v->channels = 4;
+  v->dtype = ATMO_DTYPE_F32;
+  for (int c = 0; c < 4; ++c) v->scale[c] = 1.0f;
+  const bool scat = id == ATMO_LUT_SCATTERING || id == ATMO_LUT_SINGLE_MIE_SCATTERING ||
+                    id == ATMO_LUT_DELTA_RAYLEIGH || id == ATMO_LUT_DELTA_MIE ||
+                    id == ATMO_LUT_DELTA_SCATTERING_DENSITY;
+  if (scat) {
+    v->width = z.nnu * z.nmus;
+    v->height = z.nmu;
+    v->depth = z.nr;
+  } else if (id == ATMO_LUT_TRANSMITTANCE) {
+    v->width = z.tw;
+    v->height = z.th;
+    v->depth = 1;
+  } else {
+    v->width = z.ew;
+    v->height = z.eh;
+    v->depth = 1;
+  }
+  switch (id) {
+    case ATMO_LUT_TRANSMITTANCE: v->data = m->T; break;
+    case ATMO_LUT_SCATTERING: v->data = m->S; break;
+    case ATMO_LUT_SINGLE_MIE_SCATTERING: v->data = m->SM; break;
+    case ATMO_LUT_IRRADIANCE: v->data = m->E; break;
+    case ATMO_LUT_DELTA_IRRADIANCE: v->data = m->dE; break;
+    case ATMO_LUT_DELTA_RAYLEIGH: v->data = m->dR; break;
+    case ATMO_LUT_DELTA_MIE: v->data = m->dM; break;
+    case ATMO_LUT_DELTA_SCATTERING_DENSITY: v->data = m->dJ; break;
+    default: return fail(ATMO_ERROR_INVALID_ARGUMENT, "unknown LUT id");
+  }
+  if (v->data == nullptr)
+    return fail(ATMO_ERROR_UNSUPPORTED, "this LUT is not held by the model");
+  return ATMO_OK;
+}
+
+atmo_status atmo_model_copy_lut(const atmo_model* m, atmo_lut_id id, float* host) {
+  atmo_lut_view v;
+  ATMO_TRY(atmo_model_get_lut(m, id, &v));
+  if (host == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "host is NULL");
+  ATMO_HIP(hipDeviceSynchronize());
+  const size_t texels = static_cast<size_t>(v.width) * v.height * v.depth;
+  ATMO_HIP(hipMemcpy(host, v.data, texels * 4 * sizeof(float), hipMemcpyDeviceToHost));
+  return ATMO_OK;
+}
+
+atmo_status atmo_model_footprint(const atmo_model* m, size_t* resident, size_t* scratch) {
+  if (m == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "model is NULL");
+  const Sizes& z = m->sizes;
+  if (resident != nullptr) *resident = lut_bytes(m, !m->spec.combined);
+  if (scratch != nullptr)
+    *scratch = (3 * z.s_texels() + z.e_texels()) * sizeof(float4);
+  return ATMO_OK;
+}
+
+atmo_status atmo_get_solar_radiance(const atmo_model* m, float* rgb) {
+  if (m == nullptr || rgb == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL argument");
+  for (int c = 0; c < 3; ++c) rgb[c] = m->solar_radiance[c];
+  return ATMO_OK;
+}
+
+static atmo_status check_query(const atmo_model* m, int64_t n) {
+  if (m == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "model is NULL");
+  if (n < 0) return fail(ATMO_ERROR_INVALID_ARGUMENT, "n < 0");
+  if (!m->initialized) return fail(ATMO_ERROR_NOT_INITIALIZED, "model not initialized");
+  return ATMO_OK;
+}
+
+static const atmo::Job* query_job(const atmo_model* m) {
+  return m->d_jobs + (m->jobs.size() - 1);
+}
+
+atmo_status atmo_get_sky_radiance(const atmo_model* m, int64_t n, const float* camera,
+                                  const float* view_ray, const float* shadow_length,
+                                  const float* sun_direction, float* radiance,
+                                  float* transmittance, void* stream) {
+  ATMO_TRY(check_query(m, n));
+  if (n > 0 && (!camera || !view_ray || !shadow_length || !sun_direction || !radiance ||
+                !transmittance))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL array");
+  return check_launch(atmo::launch_sky_radiance(query_job(m), n, camera, view_ray,
+                                                shadow_length, sun_direction, radiance,
+                                                transmittance,
+                                                static_cast<hipStream_t>(stream)),
+                      "sky radiance");
+}
+
+atmo_status atmo_get_sky_radiance_to_point(const atmo_model* m, int64_t n,
+                                           const float* camera, const float* point,
+                                           const float* shadow_length,
+                                           const float* sun_direction, float* radiance,
+                                           float* transmittance, void* stream) {
+  ATMO_TRY(check_query(m, n));
+  if (n > 0 && (!camera || !point || !shadow_length || !sun_direction || !radiance ||
+                !transmittance))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL array");
+  return check_launch(atmo::launch_sky_radiance_to_point(
+                          query_job(m), n, camera, point, shadow_length, sun_direction,
+                          radiance, transmittance, static_cast<hipStream_t>(stream)),
+                      "sky radiance to point");
+}
+
+atmo_status atmo_get_sun_and_sky_irradiance(const atmo_model* m, int64_t n,
+                                            const float* point, const float* normal,
+                                            const float* sun_direction,
+                                            float* sun_irradiance, float* sky_irradiance,
+                                            void* stream) {
+  ATMO_TRY(check_query(m, n));
+  if (n > 0 && (!point || !normal || !sun_direction || !sun_irradiance || !sky_irradiance))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL array");
+  return check_launch(atmo::launch_sun_and_sky_irradiance(
+                          query_job(m), n, point, normal, sun_direction, sun_irradiance,
+                          sky_irradiance, static_cast<hipStream_t>(stream)),
+                      "sun and sky irradiance");
+}
+
+atmo_status atmo_lower_parameters(const atmo_model_params* p, const double* lambdas3,
+                                  atmo_atmosphere* out) {
+  ATMO_TRY(validate_params(p));
+  if (lambdas3 == nullptr || out == nullptr)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL argument");
+  Spec s;
+  build_spec(p, &s);
+  atmo::Job j{};
+  lower(s, lambdas3, &j);
+  for (int c = 0; c < 3; ++c) {
+    out->solar_irradiance[c] = j.solar[c];
+    out->rayleigh_scattering[c] = j.rayleigh[c];
+    out->mie_scattering[c] = j.mie_sca[c];
+    out->mie_extinction[c] = j.mie_ext[c];
+    out->absorption_extinction[c] = j.absorption[c];
+    out->ground_albedo[c] = j.albedo[c];
+  }
+  out->sun_angular_radius = j.sun_angular_radius;
+  out->bottom_radius = j.rb;
+  out->top_radius = j.rt;
+  out->mie_phase_function_g = j.g;
+  out->mu_s_min = j.mu_s_min;
+  auto copy_layers = [](const atmo::LayerD* src, atmo_density_layer* dst) {
+    for (int i = 0; i < 2; ++i)
+      dst[i] = {src[i].width, src[i].exp_term, src[i].exp_scale, src[i].linear_term,
+                src[i].constant_term};
+  };
+  copy_layers(j.prof_r, out->rayleigh_density);
+  copy_layers(j.prof_m, out->mie_density);
+  copy_layers(j.prof_a, out->absorption_density);
+  return ATMO_OK;
+}
+
+int atmo_num_iterations(const atmo_model_params* p) {
+  if (validate_params(p) != ATMO_OK) return 0;
+  Spec s;
+  build_spec(p, &s);
+  return num_iterations(s);
+}
+
+atmo_status atmo_iteration(const atmo_model_params* p, int i, double* lambdas3,
+                           float* matrix9) {
+  ATMO_TRY(validate_params(p));
+  if (lambdas3 == nullptr || matrix9 == nullptr)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL argument");
+  Spec s;
+  build_spec(p, &s);
+  if (i < 0 || i >= num_iterations(s))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "iteration out of range");
+  iteration(s, i, lambdas3, matrix9);
+  return ATMO_OK;
+}
+
+atmo_status atmo_convert_spectrum_to_linear_srgb(int n, const double* wl,
+                                                 const double* spectrum, double* r,
+                                                 double* g, double* b) {
+  if (n < 1 || wl == nullptr || spectrum == nullptr || !r || !g || !b)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "bad arguments");
+  std::vector<double> w(wl, wl + n), f(spectrum, spectrum + n);
+  double x = 0.0, y = 0.0, z = 0.0;
+  for (int lambda = kLambdaMin; lambda < kLambdaMax; ++lambda) {
+    const double v = interpolate(w, f, lambda);
+    x += cie_value(lambda, 1) * v;
+    y += cie_value(lambda, 2) * v;
+    z += cie_value(lambda, 3) * v;
+  }
+  const double* m = atmo_cie::kXyzToSrgb;
+  const double k = atmo_cie::kMaxLuminousEfficacy;
+  *r = k * (m[0] * x + m[1] * y + m[2] * z);
+  *g = k * (m[3] * x + m[4] * y + m[5] * z);
+  *b = k * (m[6] * x + m[7] * y + m[8] * z);
+  return ATMO_OK;
+}
+
+atmo_status atmo_malloc(void** ptr, size_t bytes) {
+  if (ptr == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "ptr is NULL");
+  ATMO_HIP(hipMalloc(ptr, bytes));
+  return ATMO_OK;
+}
+atmo_status atmo_free(void* ptr) {
+  ATMO_HIP(hipFree(ptr));
+  return ATMO_OK;
+}
+atmo_status atmo_memcpy_to_device(void* dst, const void* src, size_t bytes, void* stream) {
+  ATMO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                          static_cast<hipStream_t>(stream)));
+  return ATMO_OK;
+}
+atmo_status atmo_memcpy_to_host(void* dst, const void* src, size_t bytes, void* stream) {
+  ATMO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost,
+                          static_cast<hipStream_t>(stream)));
+  ATMO_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return ATMO_OK;
+}
+atmo_status atmo_stream_create(void** stream) {
+  if (stream == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "stream is NULL");
+  hipStream_t s;
+  ATMO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return ATMO_OK;
+}
+atmo_status atmo_stream_destroy(void* stream) {
+  ATMO_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return ATMO_OK;
+}
+atmo_status atmo_stream_synchronize(void* stream) {
+  ATMO_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return ATMO_OK;
+}
+atmo_status atmo_device_synchronize(void) {
+  ATMO_HIP(hipDeviceSynchronize());
+  return ATMO_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched Init: one launch per pass level over every participating job of
+// every model, the level sequence captured once into a HIP graph.
+// ---------------------------------------------------------------------------
+
+struct atmo_batch {
+  std::vector<atmo_model*> models;
+  std::vector<unsigned> orders;
+  struct Level {
+    Pass pass;
+    int order;  // SD: order computed; II: order read; others unused
+    int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew;
+  };
+  std::vector<Level> levels;
+  atmo::Job* d_jobs = nullptr;
+  int* d_first_block = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = true;
+};
+
+namespace {
+
+atmo_status enqueue_levels(const atmo_batch* b, hipStream_t s) {
+  for (const atmo_batch::Level& l : b->levels) {
+    atmo::PassLaunch pl;
+    pl.jobs = b->d_jobs + l.job_offset;
+    pl.first_block = b->d_first_block + l.fb_offset;
+    pl.njobs = l.njobs;
+    pl.total_blocks = l.total_blocks;
+    pl.max_nnu = l.max_nnu;
+    pl.max_ew = l.max_ew;
+    hipError_t e = hipSuccess;
+    switch (l.pass) {
+      case kT: e = atmo::launch_transmittance(pl, s); break;
+      case kE0: e = atmo::launch_direct_irradiance(pl, s); break;
+      case kSS: e = atmo::launch_single_scattering(pl, s); break;
+      case kSD: e = atmo::launch_scattering_density(pl, l.order, s); break;
+      case kII: e = atmo::launch_indirect_irradiance(pl, l.order, s); break;
+      case kMS: e = atmo::launch_multiple_scattering(pl, s); break;
+      default: break;
+    }
+    ATMO_TRY(check_launch(e, "batched pass"));
+  }
+  return ATMO_OK;
+}
+
+void destroy_batch(atmo_batch* b) {
+  if (b == nullptr) return;
+  if (b->exec != nullptr) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph != nullptr) (void)hipGraphDestroy(b->graph);
+  if (b->d_jobs != nullptr) (void)hipFree(b->d_jobs);
+  if (b->d_first_block != nullptr) (void)hipFree(b->d_first_block);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+atmo_status atmo_batch_create(atmo_model* const* models, const unsigned* orders,
+                              int count, int use_graph, atmo_batch** out) {
+  if (out == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (count < 1 || models == nullptr || orders == nullptr)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "bad batch arguments");
+  std::unique_ptr<atmo_batch, void (*)(atmo_batch*)> b(new atmo_batch(), destroy_batch);
+  b->use_graph = use_graph != 0;
+  for (int i = 0; i < count; ++i) {
+    if (models[i] == nullptr || orders[i] < 1)
+      return fail(ATMO_ERROR_INVALID_ARGUMENT, "bad model or order in batch");
+    for (int k = 0; k < i; ++k)
+      if (models[k] == models[i])
+        return fail(ATMO_ERROR_INVALID_ARGUMENT, "a model appears twice in a batch");
+    b->models.push_back(models[i]);
+    b->orders.push_back(orders[i]);
+  }
+  // Temporaries and per-model job tables (the lookups use the latter).
+  for (int i = 0; i < count; ++i) {
+    ATMO_TRY(alloc_temporaries(models[i]));
+    ATMO_TRY(upload_jobs(models[i], orders[i], nullptr));
+  }
+  ATMO_HIP(hipStreamSynchronize(nullptr));
+
+  std::vector<atmo::Job> jobs;
+  std::vector<int> fb;
+  auto add_level = [&](Pass p, int order, const std::vector<std::pair<int, int>>& sel) {
+    if (sel.empty()) return;
+    atmo_batch::Level l;
+    l.pass = p;
+    l.order = order;
+    l.job_offset = static_cast<int>(jobs.size());
+    l.njobs = static_cast<int>(sel.size());
+    l.fb_offset = static_cast<int>(fb.size());
+    l.max_nnu = 0;
+    l.max_ew = 0;
+    int total = 0;
+    for (const auto& mj : sel) {
+      const atmo::Job& j = b->models[mj.first]->jobs[mj.second];
+      jobs.push_back(j);
+      fb.push_back(total);
+      total += pass_blocks(p, j);
+      l.max_nnu = std::max(l.max_nnu, j.nnu);
+      l.max_ew = std::max(l.max_ew, j.ew);
+    }
+    fb.push_back(total);
+    l.total_blocks = total;
+    b->levels.push_back(l);
+  };
+  int max_iters = 0;
+  unsigned max_orders = 0;
+  for (int i = 0; i < count; ++i) {
+    max_iters = std::max(max_iters, num_iterations(models[i]->spec));
+    max_orders = std::max(max_orders, orders[i]);
+  }
+  for (int it = 0; it < max_iters; ++it) {
+    std::vector<std::pair<int, int>> sel;
+    for (int i = 0; i < count; ++i)
+      if (it < num_iterations(models[i]->spec)) sel.push_back({i, it});
+    add_level(kT, 0, sel);
+    add_level(kE0, 0, sel);
+    add_level(kSS, 0, sel);
+    for (unsigned n = 2; n <= max_orders; ++n) {
+      std::vector<std::pair<int, int>> on;
+      for (const auto& mj : sel)
+        if (b->orders[mj.first] >= n) on.push_back(mj);
+      add_level(kSD, static_cast<int>(n), on);
+      add_level(kII, static_cast<int>(n) - 1, on);
+      add_level(kMS, 0, on);
+    }
+  }
+  std::vector<std::pair<int, int>> rgb;
+  for (int i = 0; i < count; ++i)
+    if (models[i]->spec.num_precomputed_wavelengths > 3)
+      rgb.push_back({i, num_iterations(models[i]->spec)});
+  add_level(kT, 0, rgb);
+
+  ATMO_HIP(hipMalloc(&b->d_jobs, jobs.size() * sizeof(atmo::Job)));
+  ATMO_HIP(hipMalloc(&b->d_first_block, fb.size() * sizeof(int)));
+  ATMO_HIP(hipMemcpy(b->d_jobs, jobs.data(), jobs.size() * sizeof(atmo::Job),
+                     hipMemcpyHostToDevice));
+  ATMO_HIP(hipMemcpy(b->d_first_block, fb.data(), fb.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  if (b->use_graph) {
+    hipStream_t cap;
+    ATMO_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    atmo_status st = ATMO_OK;
+    if (e == hipSuccess) st = enqueue_levels(b.get(), cap);
+    hipError_t e2 = hipStreamEndCapture(cap, &b->graph);
+    (void)hipStreamDestroy(cap);
+    if (e != hipSuccess || e2 != hipSuccess || st != ATMO_OK)
+      return fail(ATMO_ERROR_DEVICE, std::string("graph capture failed: ") +
+                                         hipGetErrorString(e != hipSuccess ? e : e2));
+    ATMO_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  *out = b.release();
+  return ATMO_OK;
+}
+
+atmo_status atmo_batch_launch(atmo_batch* b, void* stream) {
+  if (b == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "batch is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (b->use_graph) {
+    ATMO_HIP(hipGraphLaunch(b->exec, s));
+  } else {
+    ATMO_TRY(enqueue_levels(b, s));
+  }
+  for (atmo_model* m : b->models) m->initialized = true;
+  return ATMO_OK;
+}
+
+int atmo_batch_num_launches(const atmo_batch* b) {
+  return b == nullptr ? 0 : static_cast<int>(b->levels.size());
+}
+
+void atmo_batch_destroy(atmo_batch* b) {
+  if (b == nullptr) return;
+  (void)hipDeviceSynchronize();
+  for (atmo_model* m : b->models)
+    if (!m->retain) free_temporaries(m);
+  destroy_batch(b);
+}
+
+atmo_status atmo_model_init_batch(atmo_model* const* models, const unsigned* orders,
+                                  int count, void* stream) {
+  if (count == 0) return ATMO_OK;
+  atmo_batch* b = nullptr;
+  ATMO_TRY(atmo_batch_create(models, orders, count, 1, &b));
+  atmo_status st = atmo_batch_launch(b, stream);
+  if (st == ATMO_OK) {
+    hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) st = fail(ATMO_ERROR_DEVICE, hipGetErrorString(e));
+  }
+  atmo_batch_destroy(b);
+  return st;
+}
+
+}  // extern "C"
