@@ -123,7 +123,26 @@ typedef struct atmo_lut_view {
 
 typedef enum atmo_option {
   ATMO_OPTION_RETAIN_INTERMEDIATES = 0, /* keep the delta LUTs after Init */
+  ATMO_OPTION_PASS_TIMING = 1,          /* time every kernel launch of Init */
 } atmo_option;
+
+/* Kernel passes of the precomputation (atmosphere/model.cc:1085-1211). */
+typedef enum atmo_pass {
+  ATMO_PASS_TRANSMITTANCE = 0,
+  ATMO_PASS_DIRECT_IRRADIANCE = 1,
+  ATMO_PASS_SINGLE_SCATTERING = 2,
+  ATMO_PASS_SCATTERING_DENSITY = 3,
+  ATMO_PASS_INDIRECT_IRRADIANCE = 4,
+  ATMO_PASS_MULTIPLE_SCATTERING = 5,
+} atmo_pass;
+
+/* Duration of one kernel launch, measured with HIP events recorded on the
+ * Init stream around the launch. */
+typedef struct atmo_pass_timing {
+  int pass;   /* atmo_pass */
+  int order;  /* scattering order computed (SD, MS) or read (II); 0 else */
+  float ms;
+} atmo_pass_timing;
 
 typedef struct atmo_model atmo_model;
 
@@ -135,6 +154,11 @@ atmo_status atmo_model_create(const atmo_model_params* params,
 void atmo_model_destroy(atmo_model* model);
 atmo_status atmo_model_set_option(atmo_model* model, atmo_option option,
                                   int value);
+/* Launch timings of the last Init of a model created with
+ * ATMO_OPTION_PASS_TIMING (waits for that Init to finish). Writes at most
+ * capacity entries and the total count to *count. */
+atmo_status atmo_model_pass_timings(atmo_model* model, atmo_pass_timing* out,
+                                    int capacity, int* count);
 /* Model::Init (atmosphere/model.h:285, model.cc:866-975): precomputes the
  * LUTs with num_scattering_orders scattering orders (>= 1). */
 atmo_status atmo_model_init(atmo_model* model, unsigned num_scattering_orders,
@@ -227,6 +251,32 @@ int atmo_num_iterations(const atmo_model_params* params);
 atmo_status atmo_iteration(const atmo_model_params* params, int i,
                            double* lambdas3, float* matrix9);
 
+/* The luminance API of atmosphere/model.cc:252-281: GetSolarLuminance,
+ * GetSkyLuminance, GetSkyLuminanceToPoint and GetSunAndSkyIlluminance, i.e.
+ * the radiance lookups times SKY_SPECTRAL_RADIANCE_TO_LUMINANCE (sky terms)
+ * or SUN_SPECTRAL_RADIANCE_TO_LUMINANCE (sun terms). They are available in
+ * every mode; the radiance functions above only when at most 3 wavelengths
+ * are precomputed (as in the reference, model.cc:226 and 769-772), and
+ * return ATMO_ERROR_UNSUPPORTED otherwise. */
+atmo_status atmo_get_solar_luminance(const atmo_model* model, float* rgb);
+atmo_status atmo_get_sky_luminance(const atmo_model* model, int64_t n,
+                                   const float* camera, const float* view_ray,
+                                   const float* shadow_length,
+                                   const float* sun_direction,
+                                   float* luminance, float* transmittance,
+                                   void* stream);
+atmo_status atmo_get_sky_luminance_to_point(
+    const atmo_model* model, int64_t n, const float* camera,
+    const float* point, const float* shadow_length, const float* sun_direction,
+    float* luminance, float* transmittance, void* stream);
+atmo_status atmo_get_sun_and_sky_illuminance(
+    const atmo_model* model, int64_t n, const float* point,
+    const float* normal, const float* sun_direction, float* sun_illuminance,
+    float* sky_illuminance, void* stream);
+/* The two factor vectors (SKY_..., SUN_SPECTRAL_RADIANCE_TO_LUMINANCE). */
+atmo_status atmo_get_luminance_factors(const atmo_model* model, double* sky_k,
+                                       double* sun_k);
+
 /* Model::ConvertSpectrumToLinearSrgb (atmosphere/model.h:301-304,
  * model.cc:1020-1040). Host-only. */
 atmo_status atmo_convert_spectrum_to_linear_srgb(int n,
@@ -249,6 +299,11 @@ atmo_status atmo_stream_create(void** stream);
 atmo_status atmo_stream_destroy(void* stream);
 atmo_status atmo_stream_synchronize(void* stream);
 atmo_status atmo_device_synchronize(void);
+/* HIP events, for timing work on a stream (elapsed time in ms). */
+atmo_status atmo_event_create(void** event);
+atmo_status atmo_event_destroy(void* event);
+atmo_status atmo_event_record(void* event, void* stream);
+atmo_status atmo_event_elapsed_ms(void* start, void* stop, float* ms);
 
 #ifdef __cplusplus
 }

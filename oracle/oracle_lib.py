@@ -1,7 +1,8 @@
 """ctypes bindings of the FP64 oracle (oracle/liboracle.so).
 
 Test infrastructure only: the oracle is the checker. Only tests/,
-__graft_entry__.smoke() and the cpu_baseline leg of bench.py load it.
+__graft_entry__.smoke() and the cpu_baseline leg of bench.py load it; the
+product (precomputed_atmospheric_scattering_amd) never does.
 """
 import ctypes as C
 import os
@@ -9,8 +10,7 @@ import subprocess
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
 
 

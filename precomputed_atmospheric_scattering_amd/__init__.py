@@ -64,6 +64,37 @@ class Stream:
             pass
 
 
+class Event:
+    """A HIP event (timing work on a stream)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().atmo_event_create(C.byref(h)), "atmo_event_create")
+        self.handle = h
+
+    def record(self, stream=None):
+        check(lib().atmo_event_record(self.handle, _stream_handle(stream)),
+              "atmo_event_record")
+
+    def elapsed_ms(self, stop):
+        """Milliseconds from this event to 'stop' (waits for 'stop')."""
+        ms = C.c_float()
+        check(lib().atmo_event_elapsed_ms(self.handle, stop.handle, C.byref(ms)),
+              "atmo_event_elapsed_ms")
+        return ms.value
+
+    def close(self):
+        if self.handle:
+            lib().atmo_event_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _stream_handle(stream):
     if stream is None:
         return None
@@ -195,7 +226,7 @@ class Model:
                  absorption_extinction, ground_albedo, max_sun_zenith_angle,
                  length_unit_in_meters, num_precomputed_wavelengths,
                  combine_scattering_textures, half_precision, lut_sizes=None,
-                 retain_intermediates=False):
+                 retain_intermediates=False, pass_timing=False):
         self._handle = None
         self.params = ModelParameters(
             wavelengths, solar_irradiance, sun_angular_radius, bottom_radius, top_radius,
@@ -216,6 +247,20 @@ class Model:
         if retain_intermediates:
             check(lib().atmo_model_set_option(h, _capi.OPTION_RETAIN_INTERMEDIATES, 1),
                   "set_option")
+        if pass_timing:
+            check(lib().atmo_model_set_option(h, _capi.OPTION_PASS_TIMING, 1),
+                  "set_option")
+
+    def pass_timings(self):
+        """[(pass name, order, ms)] of the launches of the last Init (needs
+        pass_timing=True)."""
+        n = C.c_int()
+        check(lib().atmo_model_pass_timings(self._handle, None, 0, C.byref(n)),
+              "pass_timings")
+        buf = (_capi.PassTiming * max(1, n.value))()
+        check(lib().atmo_model_pass_timings(self._handle, buf, n.value, C.byref(n)),
+              "pass_timings")
+        return [(_capi.PASS_NAMES[t.pass_], t.order, t.ms) for t in buf[:n.value]]
 
     @property
     def handle(self):
@@ -288,6 +333,61 @@ class Model:
                                                     sky_irradiance.ptr,
                                                     _stream_handle(stream)),
               "GetSunAndSkyIrradiance")
+
+    # ---- luminance API (atmosphere/model.cc:252-281) on device arrays ----
+    def sky_luminance_device(self, n, camera, view_ray, shadow_length, sun_direction,
+                             luminance, transmittance, stream=None):
+        check(lib().atmo_get_sky_luminance(self._handle, n, camera.ptr, view_ray.ptr,
+                                           shadow_length.ptr, sun_direction.ptr,
+                                           luminance.ptr, transmittance.ptr,
+                                           _stream_handle(stream)), "GetSkyLuminance")
+
+    def sky_luminance_to_point_device(self, n, camera, point, shadow_length,
+                                      sun_direction, luminance, transmittance, stream=None):
+        check(lib().atmo_get_sky_luminance_to_point(self._handle, n, camera.ptr, point.ptr,
+                                                    shadow_length.ptr, sun_direction.ptr,
+                                                    luminance.ptr, transmittance.ptr,
+                                                    _stream_handle(stream)),
+              "GetSkyLuminanceToPoint")
+
+    def sun_and_sky_illuminance_device(self, n, point, normal, sun_direction,
+                                       sun_illuminance, sky_illuminance, stream=None):
+        check(lib().atmo_get_sun_and_sky_illuminance(self._handle, n, point.ptr,
+                                                     normal.ptr, sun_direction.ptr,
+                                                     sun_illuminance.ptr,
+                                                     sky_illuminance.ptr,
+                                                     _stream_handle(stream)),
+              "GetSunAndSkyIlluminance")
+
+    def solar_luminance(self):
+        out = (C.c_float * 3)()
+        check(lib().atmo_get_solar_luminance(self._handle, out), "GetSolarLuminance")
+        return np.array(list(out), dtype=np.float32)
+
+    def luminance_factors(self):
+        """(SKY_SPECTRAL_RADIANCE_TO_LUMINANCE, SUN_SPECTRAL_RADIANCE_TO_LUMINANCE)."""
+        sky, sun = (C.c_double * 3)(), (C.c_double * 3)()
+        check(lib().atmo_get_luminance_factors(self._handle, sky, sun), "luminance_factors")
+        return np.array(list(sky)), np.array(list(sun))
+
+    def _host_call(self, fn, arrays, n_out=2):
+        n = len(arrays[0])
+        d = [DeviceArray.from_host(np.asarray(a, np.float32)) for a in arrays]
+        outs = [DeviceArray((n, 3)) for _ in range(n_out)]
+        fn(n, *d, *outs)
+        return tuple(o.to_host() for o in outs)
+
+    def sky_luminance(self, camera, view_ray, shadow_length, sun_direction):
+        return self._host_call(self.sky_luminance_device,
+                               (camera, view_ray, shadow_length, sun_direction))
+
+    def sky_luminance_to_point(self, camera, point, shadow_length, sun_direction):
+        return self._host_call(self.sky_luminance_to_point_device,
+                               (camera, point, shadow_length, sun_direction))
+
+    def sun_and_sky_illuminance(self, point, normal, sun_direction):
+        return self._host_call(self.sun_and_sky_illuminance_device,
+                               (point, normal, sun_direction))
 
     # ---- host-array conveniences (copy in, run, copy out) ----
     def sky_radiance(self, camera, view_ray, shadow_length, sun_direction):

@@ -19,6 +19,9 @@ LUT_IDS = {
     "delta_scattering_density": 7,
 }
 OPTION_RETAIN_INTERMEDIATES = 0
+OPTION_PASS_TIMING = 1
+PASS_NAMES = ("transmittance", "direct_irradiance", "single_scattering",
+              "scattering_density", "indirect_irradiance", "multiple_scattering")
 
 
 class AtmoError(RuntimeError):
@@ -68,6 +71,10 @@ class LutView(C.Structure):
 D3 = C.c_double * 3
 
 
+class PassTiming(C.Structure):
+    _fields_ = [("pass_", C.c_int), ("order", C.c_int), ("ms", C.c_float)]
+
+
 class Atmosphere(C.Structure):
     """atmo_atmosphere: AtmosphereParameters at three wavelengths."""
     _fields_ = [
@@ -88,6 +95,8 @@ SIGNATURES = {
     "atmo_model_create": (ST, [C.POINTER(ModelParams), C.POINTER(LutSizes), C.POINTER(VP)]),
     "atmo_model_destroy": (None, [VP]),
     "atmo_model_set_option": (ST, [VP, C.c_int, C.c_int]),
+    "atmo_model_pass_timings": (ST, [VP, C.POINTER(PassTiming), C.c_int,
+                                     C.POINTER(C.c_int)]),
     "atmo_model_init": (ST, [VP, C.c_uint, VP]),
     "atmo_model_init_batch": (ST, [C.POINTER(VP), C.POINTER(C.c_uint), C.c_int, VP]),
     "atmo_batch_create": (ST, [C.POINTER(VP), C.POINTER(C.c_uint), C.c_int, C.c_int,
@@ -102,6 +111,11 @@ SIGNATURES = {
     "atmo_get_sky_radiance": (ST, [VP, I64, VP, VP, VP, VP, VP, VP, VP]),
     "atmo_get_sky_radiance_to_point": (ST, [VP, I64, VP, VP, VP, VP, VP, VP, VP]),
     "atmo_get_sun_and_sky_irradiance": (ST, [VP, I64, VP, VP, VP, VP, VP, VP]),
+    "atmo_get_solar_luminance": (ST, [VP, FP]),
+    "atmo_get_sky_luminance": (ST, [VP, I64, VP, VP, VP, VP, VP, VP, VP]),
+    "atmo_get_sky_luminance_to_point": (ST, [VP, I64, VP, VP, VP, VP, VP, VP, VP]),
+    "atmo_get_sun_and_sky_illuminance": (ST, [VP, I64, VP, VP, VP, VP, VP, VP]),
+    "atmo_get_luminance_factors": (ST, [VP, DP, DP]),
     "atmo_convert_spectrum_to_linear_srgb": (ST, [C.c_int, DP, DP, DP, DP, DP]),
     "atmo_lower_parameters": (ST, [C.POINTER(ModelParams), DP, C.POINTER(Atmosphere)]),
     "atmo_num_iterations": (C.c_int, [C.POINTER(ModelParams)]),
@@ -115,6 +129,10 @@ SIGNATURES = {
     "atmo_stream_destroy": (ST, [VP]),
     "atmo_stream_synchronize": (ST, [VP]),
     "atmo_device_synchronize": (ST, []),
+    "atmo_event_create": (ST, [C.POINTER(VP)]),
+    "atmo_event_destroy": (ST, [VP]),
+    "atmo_event_record": (ST, [VP, VP]),
+    "atmo_event_elapsed_ms": (ST, [VP, VP, C.POINTER(C.c_float)]),
 }
 
 _lib = None

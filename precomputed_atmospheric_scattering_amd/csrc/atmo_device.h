@@ -46,6 +46,10 @@ struct Job {
   double A;
   // luminance_from_radiance, row-major (atmosphere/model.cc:909, 939-943).
   float lum[9];
+  // SKY/SUN_SPECTRAL_RADIANCE_TO_LUMINANCE (atmosphere/model.cc:668-686,
+  // 735-742), used by the luminance lookups.
+  float sky_k[3];
+  float sun_k[3];
   int blend;     // add into S, S_Mie, E (illuminance iterations after the first)
   int combined;  // COMBINED_SCATTERING_TEXTURES: no separate S_Mie output
   int orders;    // number of scattering orders of this job
@@ -62,6 +66,15 @@ struct Job {
   float4* dR;  // delta Rayleigh; delta multiple scattering from order 2 on
   float4* dM;
   float4* dJ;
+  // FP16 storage (half_precision): the scattering LUTs are accumulated in
+  // FP32 in S / SM during Init, then stored as 4 x FP16 texels times a
+  // per-channel power-of-two scale: S16[t].c = S[t].c * scale[c],
+  // scales[0..3] for S16, scales[4..7] for SM16.
+  int half_luts;
+  uint2* S16;
+  uint2* SM16;
+  float* scales;    // 8 floats, written by the conversion kernel
+  unsigned* maxbits;  // 8 words: per-channel max |value| (float bits)
 };
 
 // ---------------------------------------------------------------------------
@@ -166,6 +179,21 @@ ATMO_HD void t_rmu_from_uv(const Job& j, double u, double v, double* r,
 }
 
 ATMO_HD float3 make3(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// A texel of 4 FP16 values stored in 8 bytes.
+ATMO_HD float4 unpack_half4(uint2 u) {
+  _Float16 h[4];
+  __builtin_memcpy(h, &u, sizeof(h));
+  return make_float4(static_cast<float>(h[0]), static_cast<float>(h[1]),
+                     static_cast<float>(h[2]), static_cast<float>(h[3]));
+}
+ATMO_HD uint2 pack_half4(float4 v) {
+  _Float16 h[4] = {static_cast<_Float16>(v.x), static_cast<_Float16>(v.y),
+                   static_cast<_Float16>(v.z), static_cast<_Float16>(v.w)};
+  uint2 u;
+  __builtin_memcpy(&u, h, sizeof(u));
+  return u;
+}
 
 // Bilinear read of a 2D RGBA LUT at GL texel-space (x, y).
 ATMO_HD float3 bilinear(const float4* lut, int w, int h, double x, double y) {
@@ -331,6 +359,125 @@ ATMO_HD float4 s_lookup(const Job& j, const float4* lut, const SCoords& c) {
   return acc;
 }
 
+// FP64-accumulating variants for the lookups: texels are FP32, weights and
+// sums FP64, so that differences of two nearby lookups (GetSkyRadianceToPoint)
+// keep the precision of the texel values.
+struct D4 {
+  double x, y, z, w;
+};
+
+// Texel sources of the lookups: FP32 texels, or FP16 texels with per-channel
+// inverse scales.
+struct F32Texels {
+  const float4* p;
+  ATMO_HD float4 operator[](int64_t i) const { return p[i]; }
+};
+struct F16Texels {
+  const uint2* p;
+  float4 inv;
+  ATMO_HD float4 operator[](int64_t i) const {
+    float4 v = unpack_half4(p[i]);
+    return make_float4(v.x * inv.x, v.y * inv.y, v.z * inv.z, v.w * inv.w);
+  }
+};
+
+template <class Texels>
+ATMO_HD D4 s_lookup_d(const Job& j, const Texels& lut, const SCoords& c) {
+  double cnu = c.nu;
+  if (!(cnu >= -1.0)) cnu = -1.0;
+  if (cnu > static_cast<double>(j.nnu)) cnu = static_cast<double>(j.nnu);
+  const double f = floor(cnu);
+  const double ws = cnu - f;
+  int s0 = static_cast<int>(f);
+  int s1 = s0 + 1;
+  s0 = s0 < 0 ? 0 : (s0 > j.nnu - 1 ? j.nnu - 1 : s0);
+  s1 = s1 < 0 ? 0 : (s1 > j.nnu - 1 ? j.nnu - 1 : s1);
+  double xz = c.z, xy = c.y, xx = c.x_in;
+  Taps tz = axis_taps(xz, j.nr), ty = axis_taps(xy, j.nmu), tx = axis_taps(xx, j.nmus);
+  // Exact FP64 weights (the Taps weights are rounded to float).
+  auto frac = [](double x, int n) {
+    if (!(x >= -1.0)) x = -1.0;
+    if (x > static_cast<double>(n)) x = static_cast<double>(n);
+    return x - floor(x);
+  };
+  const double fz = frac(xz, j.nr), fy = frac(xy, j.nmu), fx = frac(xx, j.nmus);
+  const int w = j.nnu * j.nmus;
+  const int zs[2] = {tz.i0, tz.i1};
+  const int ys[2] = {ty.i0, ty.i1};
+  const int xs[2] = {tx.i0, tx.i1};
+  const double wz[2] = {1.0 - fz, fz};
+  const double wy[2] = {1.0 - fy, fy};
+  const double wx[2] = {1.0 - fx, fx};
+  const int ss[2] = {s0, s1};
+  const double wsl[2] = {1.0 - ws, ws};
+  D4 acc{0.0, 0.0, 0.0, 0.0};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int e = 0; e < 2; ++e)
+        for (int q = 0; q < 2; ++q) {
+          const double wt = wz[a] * wy[b] * wx[e] * wsl[q];
+          const float4 v = lut[(static_cast<int64_t>(zs[a]) * j.nmu + ys[b]) * w +
+                               ss[q] * j.nmus + xs[e]];
+          acc.x += wt * v.x;
+          acc.y += wt * v.y;
+          acc.z += wt * v.z;
+          acc.w += wt * v.w;
+        }
+  return acc;
+}
+
+ATMO_HD D4 bilinear_d(const float4* lut, int w, int h, double x, double y) {
+  Taps tx = axis_taps(x, w), ty = axis_taps(y, h);
+  if (!(x >= -1.0)) x = -1.0;
+  if (x > static_cast<double>(w)) x = static_cast<double>(w);
+  if (!(y >= -1.0)) y = -1.0;
+  if (y > static_cast<double>(h)) y = static_cast<double>(h);
+  const double fx = x - floor(x), fy = y - floor(y);
+  const float4 a = lut[ty.i0 * w + tx.i0], b = lut[ty.i0 * w + tx.i1];
+  const float4 c = lut[ty.i1 * w + tx.i0], d = lut[ty.i1 * w + tx.i1];
+  const double wx1 = fx, wx0 = 1.0 - fx, wy1 = fy, wy0 = 1.0 - fy;
+  D4 o;
+  o.x = wy0 * (wx0 * a.x + wx1 * b.x) + wy1 * (wx0 * c.x + wx1 * d.x);
+  o.y = wy0 * (wx0 * a.y + wx1 * b.y) + wy1 * (wx0 * c.y + wx1 * d.y);
+  o.z = wy0 * (wx0 * a.z + wx1 * b.z) + wy1 * (wx0 * c.z + wx1 * d.z);
+  o.w = 0.0;
+  return o;
+}
+
+ATMO_HD D4 t_lookup_d(const Job& j, const float4* T, double r, double mu) {
+  double x, y;
+  t_texel_coords(j, r, mu, &x, &y);
+  return bilinear_d(T, j.tw, j.th, x, y);
+}
+
+// functions.glsl:493-519 in FP64 arithmetic.
+ATMO_HD D4 transmittance_d(const Job& j, const float4* T, double r, double mu,
+                           double d, bool ground) {
+  const double r_d = clamp_radius(j, sqrt(d * d + 2.0 * r * mu * d + r * r));
+  const double mu_d = clamp_cos((r * mu + d) / r_d);
+  D4 num, den;
+  if (ground) {
+    num = t_lookup_d(j, T, r_d, -mu_d);
+    den = t_lookup_d(j, T, r, -mu);
+  } else {
+    num = t_lookup_d(j, T, r, mu);
+    den = t_lookup_d(j, T, r_d, mu_d);
+  }
+  auto m1 = [](double v) { return v < 1.0 ? v : 1.0; };
+  return D4{m1(num.x / den.x), m1(num.y / den.y), m1(num.z / den.z), 0.0};
+}
+
+// functions.glsl:552-563 in FP64 arithmetic.
+ATMO_HD D4 transmittance_to_sun_d(const Job& j, const float4* T, double r,
+                                  double mu_s) {
+  const double sin_h = j.rb / r;
+  const double cos_h = -sqrt(1.0 - sin_h * sin_h > 0.0 ? 1.0 - sin_h * sin_h : 0.0);
+  const double f = smoothstep_d(-sin_h * j.sun_angular_radius,
+                                sin_h * j.sun_angular_radius, mu_s - cos_h);
+  D4 t = t_lookup_d(j, T, r, mu_s);
+  return D4{t.x * f, t.y * f, t.z * f, 0.0};
+}
+
 // Inverse mappings of a scattering texel (functions.glsl:837-890 and
 // :905-926, frag_coord at the texel centre), split per axis.
 
@@ -408,7 +555,22 @@ ATMO_HD float3 e_lookup(const Job& j, const float4* E, double r, double mu_s) {
                   unit_to_texel(x_r, j.eh));
 }
 
+ATMO_HD D4 e_lookup_d(const Job& j, const float4* E, double r, double mu_s) {
+  double x_r = (r - j.rb) / (j.rt - j.rb);
+  double x_mu_s = mu_s * 0.5 + 0.5;
+  return bilinear_d(E, j.ew, j.eh, unit_to_texel(x_mu_s, j.ew),
+                    unit_to_texel(x_r, j.eh));
+}
+
 // functions.glsl:739-747
+ATMO_HD double rayleigh_phase_d(double nu) {
+  return 3.0 / (16.0 * kPi) * (1.0 + nu * nu);
+}
+ATMO_HD double mie_phase_d(double g, double nu) {
+  const double k = 3.0 / (8.0 * kPi) * (1.0 - g * g) / (2.0 + g * g);
+  const double y = 1.0 + g * g - 2.0 * g * nu;
+  return k * (1.0 + nu * nu) / (y * sqrt(y));
+}
 ATMO_HD float rayleigh_phase(float nu) {
   return 3.0f / (16.0f * 3.14159265358979f) * (1.0f + nu * nu);
 }

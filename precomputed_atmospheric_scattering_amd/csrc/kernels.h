@@ -28,6 +28,7 @@ int blocks_single_scattering(const Job& j);
 int blocks_scattering_density(const Job& j);
 int blocks_indirect_irradiance(const Job& j);
 int blocks_multiple_scattering(const Job& j);
+int blocks_half_conversion(const Job& j);
 
 hipError_t launch_transmittance(const PassLaunch& p, hipStream_t s);
 hipError_t launch_direct_irradiance(const PassLaunch& p, hipStream_t s);
@@ -39,26 +40,35 @@ hipError_t launch_scattering_density(const PassLaunch& p, int order,
 hipError_t launch_indirect_irradiance(const PassLaunch& p, int order,
                                       hipStream_t s);
 hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s);
+// FP16 storage of the scattering LUTs of half-precision jobs:
+// launch_clear_max (one block per job) resets the per-channel maxima, then
+// launch_to_half (blocks_half_conversion blocks per job) computes the maxima
+// and writes the scaled FP16 texels.
+hipError_t launch_clear_max(const PassLaunch& p, hipStream_t s);
+hipError_t launch_to_half(const PassLaunch& p, hipStream_t s);
 
-// Lookups (one ray / point per thread). 'job' is a device pointer.
+// Lookups (one ray / point per thread). 'job' is a device pointer. With
+// 'luminance' the results are scaled by the job's radiance-to-luminance
+// factors (the Luminance API of atmosphere/model.cc:252-281).
 hipError_t launch_sky_radiance(const Job* job, int64_t n, const float* camera,
                                const float* view_ray,
                                const float* shadow_length,
                                const float* sun_direction, float* radiance,
-                               float* transmittance, hipStream_t s);
+                               float* transmittance, int luminance, hipStream_t s);
 hipError_t launch_sky_radiance_to_point(const Job* job, int64_t n,
                                         const float* camera,
                                         const float* point,
                                         const float* shadow_length,
                                         const float* sun_direction,
                                         float* radiance, float* transmittance,
-                                        hipStream_t s);
+                                        int luminance, hipStream_t s);
 hipError_t launch_sun_and_sky_irradiance(const Job* job, int64_t n,
                                          const float* point,
                                          const float* normal,
                                          const float* sun_direction,
                                          float* sun_irradiance,
-                                         float* sky_irradiance, hipStream_t s);
+                                         float* sky_irradiance, int luminance,
+                                         hipStream_t s);
 
 }  // namespace atmo
 

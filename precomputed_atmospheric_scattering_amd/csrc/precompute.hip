@@ -818,6 +818,77 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   J.S[t_idx] = sv;
 }
 
+// ---------------------------------------------------------------------------
+// FP16 storage of the scattering LUTs (half_precision): per-channel maximum
+// of the FP32 accumulators, then conversion with power-of-two scales that map
+// each channel's maximum into [2^13, 2^14), well inside the FP16 range, so
+// that FP16 keeps its 11-bit relative precision down to ~4e-9 of the maximum.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_clear_max(const Job* __restrict__ jobs,
+                                                  const int* __restrict__ first_block,
+                                                  int njobs) {
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  if (threadIdx.x < 8) jobs[ji].maxbits[threadIdx.x] = 0u;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lut_max(const Job* __restrict__ jobs,
+                                                    const int* __restrict__ first_block,
+                                                    int njobs) {
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int64_t n = static_cast<int64_t>(J.nr) * J.nmu * J.nmus * J.nnu;
+  const int64_t t = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f), m = s;
+  if (t < n) {
+    s = J.S[t];
+    if (!J.combined) m = J.SM[t];
+  }
+  // |value| as non-negative float bits compare like unsigned integers.
+  const float v[8] = {fabsf(s.x), fabsf(s.y), fabsf(s.z), fabsf(s.w),
+                      fabsf(m.x), fabsf(m.y), fabsf(m.z), 0.0f};
+  const bool lane0 = (threadIdx.x & 63) == 0;
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    const float w = wave_max(v[c]);
+    if (lane0 && w > 0.0f) atomicMax(&J.maxbits[c], __float_as_uint(w));
+  }
+}
+
+__device__ __forceinline__ float pow2_scale(unsigned bits) {
+  const float mx = __uint_as_float(bits);
+  if (!(mx > 0.0f) || isinf(mx)) return 1.0f;
+  int e;
+  frexpf(mx, &e);  // mx = f * 2^e, f in [0.5, 1)
+  return ldexpf(1.0f, 14 - e);
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_half(const Job* __restrict__ jobs,
+                                                    const int* __restrict__ first_block,
+                                                    int njobs) {
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int lb = blockIdx.x - first_block[ji];
+  float sc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) sc[c] = pow2_scale(J.maxbits[c]);
+  if (lb == 0 && threadIdx.x < 8) J.scales[threadIdx.x] = sc[threadIdx.x];
+  const int64_t n = static_cast<int64_t>(J.nr) * J.nmu * J.nmus * J.nnu;
+  const int64_t t = static_cast<int64_t>(lb) * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const float4 s = J.S[t];
+  J.S16[t] = pack_half4(make_float4(s.x * sc[0], s.y * sc[1], s.z * sc[2], s.w * sc[3]));
+  if (!J.combined) {
+    const float4 m = J.SM[t];
+    J.SM16[t] = pack_half4(make_float4(m.x * sc[4], m.y * sc[5], m.z * sc[6], 0.0f));
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -840,6 +911,10 @@ int blocks_scattering_density(const Job& j) {
 }
 int blocks_indirect_irradiance(const Job& j) { return j.ew * j.eh; }
 int blocks_multiple_scattering(const Job& j) { return blocks_single_scattering(j); }
+int blocks_half_conversion(const Job& j) {
+  const int64_t n = static_cast<int64_t>(j.nr) * j.nmu * j.nmus * j.nnu;
+  return static_cast<int>((n + kBlock - 1) / kBlock);
+}
 
 hipError_t launch_transmittance(const PassLaunch& p, hipStream_t s) {
   hipLaunchKernelGGL(k_transmittance, dim3(p.total_blocks), dim3(kBlock), 0, s,
@@ -873,6 +948,19 @@ hipError_t launch_indirect_irradiance(const PassLaunch& p, int order,
                      p.jobs, p.first_block, p.njobs, order);
   return hipGetLastError();
 }
+hipError_t launch_clear_max(const PassLaunch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_max, dim3(p.total_blocks), dim3(64), 0, s, p.jobs,
+                     p.first_block, p.njobs);
+  return hipGetLastError();
+}
+hipError_t launch_to_half(const PassLaunch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_lut_max, dim3(p.total_blocks), dim3(kBlock), 0, s, p.jobs,
+                     p.first_block, p.njobs);
+  hipLaunchKernelGGL(k_to_half, dim3(p.total_blocks), dim3(kBlock), 0, s, p.jobs,
+                     p.first_block, p.njobs);
+  return hipGetLastError();
+}
+
 hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s) {
   hipLaunchKernelGGL(k_multiple_scattering, dim3(p.total_blocks), dim3(kBlock), 0, s,
                      p.jobs, p.first_block, p.njobs);

@@ -1,9 +1,10 @@
 // Sky-shading lookups for gfx950: GetSkyRadiance, GetSkyRadianceToPoint and
 // GetSunAndSkyIrradiance (atmosphere/functions.glsl:1633-1896) over batches
-// of rays, one ray per thread. The scalar geometry (radius, ray/sphere
-// discriminants, ground flag, LUT coordinates) is FP64, because those are
-// where FP32 cancellation near the horizon flips the ground/sky choice or
-// shifts a lookup; texel fetches and blending are FP32.
+// of rays, one ray per thread. Texels are FP32; everything else is FP64: the
+// scalar geometry (radius, ray/sphere discriminants, ground flag, LUT
+// coordinates), where FP32 cancellation near the horizon flips the ground/sky
+// choice or shifts a lookup, and the texel blending, so that the difference
+// of two nearby lookups in GetSkyRadianceToPoint keeps the texels' precision.
 #include <hip/hip_runtime.h>
 
 #include "atmo_device.h"
@@ -18,52 +19,74 @@ struct D3 {
 __device__ __forceinline__ D3 load3(const float* p, int64_t i) {
   return D3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
 }
-__device__ __forceinline__ void store3(float* p, int64_t i, float3 v) {
-  p[3 * i] = v.x;
-  p[3 * i + 1] = v.y;
-  p[3 * i + 2] = v.z;
+__device__ __forceinline__ void store3(float* p, int64_t i, D4 v) {
+  p[3 * i] = static_cast<float>(v.x);
+  p[3 * i + 1] = static_cast<float>(v.y);
+  p[3 * i + 2] = static_cast<float>(v.z);
 }
 __device__ __forceinline__ double dot3(D3 a, D3 b) {
   return a.x * b.x + a.y * b.y + a.z * b.z;
 }
-__device__ __forceinline__ float3 mul3(float3 a, float3 b) {
-  return make_float3(a.x * b.x, a.y * b.y, a.z * b.z);
+__device__ __forceinline__ D4 mul(D4 a, D4 b) {
+  return D4{a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w};
 }
-__device__ __forceinline__ float3 scale3(float3 a, float s) {
-  return make_float3(a.x * s, a.y * s, a.z * s);
+__device__ __forceinline__ D4 scale(D4 a, double s) {
+  return D4{a.x * s, a.y * s, a.z * s, a.w * s};
 }
-__device__ __forceinline__ float3 sub3(float3 a, float3 b) {
-  return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
+__device__ __forceinline__ D4 sub(D4 a, D4 b) {
+  return D4{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w};
 }
+__device__ __forceinline__ D4 rgb(D4 a) { return D4{a.x, a.y, a.z, 0.0}; }
 
 // functions.glsl:1633-1645
-__device__ __forceinline__ float3 extrapolated_mie(const Job& J, float4 s) {
-  if (s.x <= 0.0f) return make_float3(0.f, 0.f, 0.f);
-  const float k = s.w / s.x * static_cast<float>(J.rayleigh[0] / J.mie_sca[0]);
-  return make_float3(s.x * k * static_cast<float>(J.mie_sca[0] / J.rayleigh[0]),
-                     s.y * k * static_cast<float>(J.mie_sca[1] / J.rayleigh[1]),
-                     s.z * k * static_cast<float>(J.mie_sca[2] / J.rayleigh[2]));
+__device__ __forceinline__ D4 extrapolated_mie(const Job& J, D4 s) {
+  if (s.x <= 0.0) return D4{0.0, 0.0, 0.0, 0.0};
+  const double k = s.w / s.x * (J.rayleigh[0] / J.mie_sca[0]);
+  return D4{s.x * k * (J.mie_sca[0] / J.rayleigh[0]),
+            s.y * k * (J.mie_sca[1] / J.rayleigh[1]),
+            s.z * k * (J.mie_sca[2] / J.rayleigh[2]), 0.0};
 }
 
 // functions.glsl:1658-1690
-__device__ float4 combined_scattering(const Job& J, double r, double mu,
-                                      double mu_s, double nu, bool ground,
-                                      float3* single_mie) {
+__device__ D4 combined_scattering(const Job& J, double r, double mu, double mu_s,
+                                  double nu, bool ground, D4* single_mie) {
   const SCoords c = s_coords(J, r, mu, mu_s, nu, ground);
-  const float4 s = s_lookup(J, J.S, c);
-  if (J.combined) {
-    *single_mie = extrapolated_mie(J, s);
+  D4 s;
+  if (J.half_luts) {
+    const float* sc = J.scales;
+    const F16Texels st{J.S16, make_float4(1.0f / sc[0], 1.0f / sc[1], 1.0f / sc[2],
+                                          1.0f / sc[3])};
+    s = s_lookup_d(J, st, c);
+    if (!J.combined) {
+      const F16Texels mt{J.SM16, make_float4(1.0f / sc[4], 1.0f / sc[5], 1.0f / sc[6],
+                                             0.0f)};
+      *single_mie = rgb(s_lookup_d(J, mt, c));
+    }
   } else {
-    *single_mie = make3(s_lookup(J, J.SM, c));
+    s = s_lookup_d(J, F32Texels{J.S}, c);
+    if (!J.combined) *single_mie = rgb(s_lookup_d(J, F32Texels{J.SM}, c));
   }
+  if (J.combined) *single_mie = extrapolated_mie(J, s);
   return s;
+}
+
+// scattering * P_R(nu) + single_mie * P_M(nu), optionally times the sky
+// radiance-to-luminance factors.
+__device__ __forceinline__ D4 shade(const Job& J, D4 scat, D4 mie, double nu,
+                                    int luminance) {
+  const double pr = rayleigh_phase_d(nu);
+  const double pm = mie_phase_d(J.g, nu);
+  D4 l{scat.x * pr + mie.x * pm, scat.y * pr + mie.y * pm, scat.z * pr + mie.z * pm,
+       0.0};
+  if (luminance) l = D4{l.x * J.sky_k[0], l.y * J.sky_k[1], l.z * J.sky_k[2], 0.0};
+  return l;
 }
 
 __global__ __launch_bounds__(kBlock) void k_sky_radiance(
     const Job* __restrict__ job, int64_t n, const float* __restrict__ camera,
     const float* __restrict__ view_ray, const float* __restrict__ shadow_length,
     const float* __restrict__ sun_direction, float* __restrict__ radiance,
-    float* __restrict__ transmittance_out) {
+    float* __restrict__ transmittance_out, int luminance) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const Job& J = *job;
@@ -80,36 +103,30 @@ __global__ __launch_bounds__(kBlock) void k_sky_radiance(
     r = J.rt;
     rmu += to_top;
   } else if (r > J.rt) {
-    store3(transmittance_out, i, make_float3(1.f, 1.f, 1.f));
-    store3(radiance, i, make_float3(0.f, 0.f, 0.f));
+    store3(transmittance_out, i, D4{1.0, 1.0, 1.0, 0.0});
+    store3(radiance, i, D4{0.0, 0.0, 0.0, 0.0});
     return;
   }
   const double mu = rmu / r;
   const double mu_s = dot3(cam, sd) / r;
   const double nu = dot3(v, sd);
   const bool ground = hits_ground(J, r, mu);
-  const float3 t = ground ? make_float3(0.f, 0.f, 0.f) : t_lookup(J, J.T, r, mu);
-  float3 mie;
-  float3 scat;
+  const D4 t = ground ? D4{0.0, 0.0, 0.0, 0.0} : t_lookup_d(J, J.T, r, mu);
+  D4 mie, scat;
   if (shadow == 0.0) {
-    scat = make3(combined_scattering(J, r, mu, mu_s, nu, ground, &mie));
+    scat = rgb(combined_scattering(J, r, mu, mu_s, nu, ground, &mie));
   } else {
     // Light shafts (functions.glsl:1746-1766).
     const double d = shadow;
     const double r_p = clamp_radius(J, sqrt(d * d + 2.0 * r * mu * d + r * r));
     const double mu_p = (r * mu + d) / r_p;
     const double mu_s_p = (r * mu_s + d * nu) / r_p;
-    scat = make3(combined_scattering(J, r_p, mu_p, mu_s_p, nu, ground, &mie));
-    const float3 st = transmittance(J, J.T, r, mu, shadow, ground);
-    scat = mul3(scat, st);
-    mie = mul3(mie, st);
+    scat = rgb(combined_scattering(J, r_p, mu_p, mu_s_p, nu, ground, &mie));
+    const D4 st = transmittance_d(J, J.T, r, mu, shadow, ground);
+    scat = mul(scat, st);
+    mie = mul(mie, st);
   }
-  const float fnu = static_cast<float>(nu);
-  const float pr = rayleigh_phase(fnu);
-  const float pm = mie_phase(static_cast<float>(J.g), fnu);
-  store3(radiance, i,
-         make_float3(scat.x * pr + mie.x * pm, scat.y * pr + mie.y * pm,
-                     scat.z * pr + mie.z * pm));
+  store3(radiance, i, shade(J, scat, mie, nu, luminance));
   store3(transmittance_out, i, t);
 }
 
@@ -117,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_sky_radiance_to_point(
     const Job* __restrict__ job, int64_t n, const float* __restrict__ camera,
     const float* __restrict__ point, const float* __restrict__ shadow_length,
     const float* __restrict__ sun_direction, float* __restrict__ radiance,
-    float* __restrict__ transmittance_out) {
+    float* __restrict__ transmittance_out, int luminance) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const Job& J = *job;
@@ -143,38 +160,31 @@ __global__ __launch_bounds__(kBlock) void k_sky_radiance_to_point(
   const D3 pc = D3{p.x - cam.x, p.y - cam.y, p.z - cam.z};
   double d = sqrt(dot3(pc, pc));
   const bool ground = hits_ground(J, r, mu);
-  const float3 t = transmittance(J, J.T, r, mu, d, ground);
-  float3 mie;
-  float3 scat = make3(combined_scattering(J, r, mu, mu_s, nu, ground, &mie));
+  const D4 t = transmittance_d(J, J.T, r, mu, d, ground);
+  D4 mie;
+  D4 scat = rgb(combined_scattering(J, r, mu, mu_s, nu, ground, &mie));
   // functions.glsl:1831-1840
   d = d - shadow > 0.0 ? d - shadow : 0.0;
   const double r_p = clamp_radius(J, sqrt(d * d + 2.0 * r * mu * d + r * r));
   const double mu_p = (r * mu + d) / r_p;
   const double mu_s_p = (r * mu_s + d * nu) / r_p;
-  float3 mie_p;
-  const float3 scat_p = make3(combined_scattering(J, r_p, mu_p, mu_s_p, nu, ground, &mie_p));
-  float3 st = t;
-  if (shadow > 0.0) st = transmittance(J, J.T, r, mu, d, ground);
-  scat = sub3(scat, mul3(st, scat_p));
-  mie = sub3(mie, mul3(st, mie_p));
-  if (J.combined) {
-    mie = extrapolated_mie(J, make_float4(scat.x, scat.y, scat.z, mie.x));
-  }
+  D4 mie_p;
+  const D4 scat_p = rgb(combined_scattering(J, r_p, mu_p, mu_s_p, nu, ground, &mie_p));
+  D4 st = t;
+  if (shadow > 0.0) st = transmittance_d(J, J.T, r, mu, d, ground);
+  scat = sub(scat, mul(st, scat_p));
+  mie = sub(mie, mul(st, mie_p));
+  if (J.combined) mie = extrapolated_mie(J, D4{scat.x, scat.y, scat.z, mie.x});
   // functions.glsl:1857-1859
-  mie = scale3(mie, static_cast<float>(smoothstep_d(0.0, 0.01, mu_s)));
-  const float fnu = static_cast<float>(nu);
-  const float pr = rayleigh_phase(fnu);
-  const float pm = mie_phase(static_cast<float>(J.g), fnu);
-  store3(radiance, i,
-         make_float3(scat.x * pr + mie.x * pm, scat.y * pr + mie.y * pm,
-                     scat.z * pr + mie.z * pm));
+  mie = scale(mie, smoothstep_d(0.0, 0.01, mu_s));
+  store3(radiance, i, shade(J, scat, mie, nu, luminance));
   store3(transmittance_out, i, t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sun_and_sky_irradiance(
     const Job* __restrict__ job, int64_t n, const float* __restrict__ point,
     const float* __restrict__ normal, const float* __restrict__ sun_direction,
-    float* __restrict__ sun_out, float* __restrict__ sky_out) {
+    float* __restrict__ sun_out, float* __restrict__ sky_out, int luminance) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const Job& J = *job;
@@ -183,15 +193,18 @@ __global__ __launch_bounds__(kBlock) void k_sun_and_sky_irradiance(
   const D3 sd = load3(sun_direction, i);
   const double r = sqrt(dot3(p, p));
   const double mu_s = dot3(p, sd) / r;
-  const float3 e = e_lookup(J, J.E, r, mu_s);
-  store3(sky_out, i, scale3(e, static_cast<float>((1.0 + dot3(nrm, p) / r) * 0.5)));
-  const float3 ts = transmittance_to_sun(J, J.T, r, mu_s);
+  D4 sky = scale(e_lookup_d(J, J.E, r, mu_s), (1.0 + dot3(nrm, p) / r) * 0.5);
+  const D4 ts = transmittance_to_sun_d(J, J.T, r, mu_s);
   const double c = dot3(nrm, sd);
-  const float cf = static_cast<float>(c > 0.0 ? c : 0.0);
-  store3(sun_out, i,
-         make_float3(static_cast<float>(J.solar[0]) * ts.x * cf,
-                     static_cast<float>(J.solar[1]) * ts.y * cf,
-                     static_cast<float>(J.solar[2]) * ts.z * cf));
+  const double cf = c > 0.0 ? c : 0.0;
+  D4 sun{J.solar[0] * ts.x * cf, J.solar[1] * ts.y * cf, J.solar[2] * ts.z * cf, 0.0};
+  if (luminance) {
+    // GetSunAndSkyIlluminance (atmosphere/model.cc:273-281).
+    sky = D4{sky.x * J.sky_k[0], sky.y * J.sky_k[1], sky.z * J.sky_k[2], 0.0};
+    sun = D4{sun.x * J.sun_k[0], sun.y * J.sun_k[1], sun.z * J.sun_k[2], 0.0};
+  }
+  store3(sky_out, i, sky);
+  store3(sun_out, i, sun);
 }
 
 inline unsigned grid_for(int64_t n) {
@@ -204,11 +217,11 @@ hipError_t launch_sky_radiance(const Job* job, int64_t n, const float* camera,
                                const float* view_ray,
                                const float* shadow_length,
                                const float* sun_direction, float* radiance,
-                               float* transmittance, hipStream_t s) {
+                               float* transmittance, int luminance, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sky_radiance, dim3(grid_for(n)), dim3(kBlock), 0, s, job, n,
                      camera, view_ray, shadow_length, sun_direction, radiance,
-                     transmittance);
+                     transmittance, luminance);
   return hipGetLastError();
 }
 
@@ -218,11 +231,11 @@ hipError_t launch_sky_radiance_to_point(const Job* job, int64_t n,
                                         const float* shadow_length,
                                         const float* sun_direction,
                                         float* radiance, float* transmittance,
-                                        hipStream_t s) {
+                                        int luminance, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sky_radiance_to_point, dim3(grid_for(n)), dim3(kBlock), 0, s,
                      job, n, camera, point, shadow_length, sun_direction, radiance,
-                     transmittance);
+                     transmittance, luminance);
   return hipGetLastError();
 }
 
@@ -231,11 +244,12 @@ hipError_t launch_sun_and_sky_irradiance(const Job* job, int64_t n,
                                          const float* normal,
                                          const float* sun_direction,
                                          float* sun_irradiance,
-                                         float* sky_irradiance, hipStream_t s) {
+                                         float* sky_irradiance, int luminance,
+                                         hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sun_and_sky_irradiance, dim3(grid_for(n)), dim3(kBlock), 0, s,
                      job, n, point, normal, sun_direction, sun_irradiance,
-                     sky_irradiance);
+                     sky_irradiance, luminance);
   return hipGetLastError();
 }
 

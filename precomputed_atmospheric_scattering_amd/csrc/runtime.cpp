@@ -66,6 +66,30 @@ double cie_value(double lambda, int column) {
   return t[row] * (1.0 - u) + t[row + 1] * u;
 }
 
+// ComputeSpectralRadianceToLuminanceFactors (atmosphere/model.cc:562-595):
+// 1 nm integration of the CIE functions against the solar spectrum relative
+// to its value at 680/550/440 nm, times (lambda / lambda_c)^power.
+void luminance_factors(const std::vector<double>& wl, const std::vector<double>& solar,
+                       double power, double* k) {
+  const double lambdas[3] = {kLambdaR, kLambdaG, kLambdaB};
+  double solar_c[3];
+  for (int c = 0; c < 3; ++c) {
+    solar_c[c] = interpolate(wl, solar, lambdas[c]);
+    k[c] = 0.0;
+  }
+  const double* m = atmo_cie::kXyzToSrgb;
+  for (int lambda = kLambdaMin; lambda < kLambdaMax; ++lambda) {
+    const double x = cie_value(lambda, 1), y = cie_value(lambda, 2),
+                 z = cie_value(lambda, 3);
+    const double irradiance = interpolate(wl, solar, lambda);
+    for (int c = 0; c < 3; ++c) {
+      const double bar = m[3 * c] * x + m[3 * c + 1] * y + m[3 * c + 2] * z;
+      k[c] += bar * irradiance / solar_c[c] * std::pow(lambda / lambdas[c], power);
+    }
+  }
+  for (int c = 0; c < 3; ++c) k[c] *= atmo_cie::kMaxLuminousEfficacy;
+}
+
 struct Spec {
   std::vector<double> wavelengths, solar, rayleigh, mie_sca, mie_ext, absorption,
       albedo;
@@ -168,7 +192,7 @@ void set_sizes(atmo::Job* j, const Sizes& z) {
   j->eh = z.eh;
 }
 
-enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kPasses };
+enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kHalf, kClear, kPasses };
 
 int pass_blocks(Pass p, const atmo::Job& j) {
   switch (p) {
@@ -178,6 +202,8 @@ int pass_blocks(Pass p, const atmo::Job& j) {
     case kSD: return atmo::blocks_scattering_density(j);
     case kII: return atmo::blocks_indirect_irradiance(j);
     case kMS: return atmo::blocks_multiple_scattering(j);
+    case kHalf: return atmo::blocks_half_conversion(j);
+    case kClear: return 1;
     default: return 0;
   }
 }
@@ -196,6 +222,11 @@ struct atmo_model {
   bool initialized = false;
   float4 *T = nullptr, *S = nullptr, *SM = nullptr, *E = nullptr;
   float4 *dE = nullptr, *dR = nullptr, *dM = nullptr, *dJ = nullptr;
+  // FP16 storage: resident half LUTs; S and SM above are then FP32
+  // accumulators that live only during Init, like the deltas.
+  uint2 *S16 = nullptr, *SM16 = nullptr;
+  float* d_scales = nullptr;
+  unsigned* d_maxbits = nullptr;
   // Device job table: [iterations..., rgb transmittance job, query job].
   std::vector<atmo::Job> jobs;
   atmo::Job* d_jobs = nullptr;
@@ -203,13 +234,26 @@ struct atmo_model {
   int* d_first_block = nullptr;
   int first_block_host[2 * kPasses] = {};
   float solar_radiance[3] = {};
+  double sky_k[3] = {}, sun_k[3] = {};
+  // Launch timing (ATMO_OPTION_PASS_TIMING).
+  bool timing = false;
+  struct Timed {
+    int pass, order;
+    hipEvent_t start, stop;
+  };
+  std::vector<Timed> timed;   // events of the last Init, in launch order
+  std::vector<hipEvent_t> event_pool;
 };
 
 namespace {
 
 // Temporaries are released once the work that reads them has finished.
 void free_temporaries(atmo_model* m) {
-  float4** bufs[4] = {&m->dE, &m->dR, &m->dM, &m->dJ};
+  std::vector<float4**> bufs = {&m->dE, &m->dR, &m->dM, &m->dJ};
+  if (m->spec.half_precision) {
+    bufs.push_back(&m->S);
+    bufs.push_back(&m->SM);
+  }
   for (float4** b : bufs) {
     if (*b != nullptr) {
       (void)hipFree(*b);
@@ -224,6 +268,11 @@ atmo_status alloc_temporaries(atmo_model* m) {
   if (m->dR == nullptr) ATMO_HIP(hipMalloc(&m->dR, z.s_texels() * sizeof(float4)));
   if (m->dM == nullptr) ATMO_HIP(hipMalloc(&m->dM, z.s_texels() * sizeof(float4)));
   if (m->dJ == nullptr) ATMO_HIP(hipMalloc(&m->dJ, z.s_texels() * sizeof(float4)));
+  if (m->spec.half_precision) {
+    if (m->S == nullptr) ATMO_HIP(hipMalloc(&m->S, z.s_texels() * sizeof(float4)));
+    if (m->SM == nullptr && !m->spec.combined)
+      ATMO_HIP(hipMalloc(&m->SM, z.s_texels() * sizeof(float4)));
+  }
   return ATMO_OK;
 }
 
@@ -238,6 +287,15 @@ void fill_job_luts(atmo_model* m, atmo::Job* j) {
   j->dJ = m->dJ;
   set_sizes(j, m->sizes);
   j->combined = m->spec.combined ? 1 : 0;
+  j->half_luts = m->spec.half_precision ? 1 : 0;
+  j->S16 = m->S16;
+  j->SM16 = m->SM16;
+  j->scales = m->d_scales;
+  j->maxbits = m->d_maxbits;
+  for (int c = 0; c < 3; ++c) {
+    j->sky_k[c] = static_cast<float>(m->sky_k[c]);
+    j->sun_k[c] = static_cast<float>(m->sun_k[c]);
+  }
 }
 
 // Rebuild the host job table (LUT pointers may have changed) and upload it.
@@ -292,28 +350,64 @@ atmo_status check_launch(hipError_t e, const char* what) {
     if (st_ != ATMO_OK) return st_;      \
   } while (0)
 
+// Records start/stop events around each launch when timing is enabled.
+struct LaunchTimer {
+  atmo_model* m;
+  hipStream_t s;
+  size_t next_event = 0;
+  hipEvent_t take() {
+    if (next_event == m->event_pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      m->event_pool.push_back(e);
+    }
+    return m->event_pool[next_event++];
+  }
+  template <class F>
+  atmo_status run(int pass, int order, const char* what, F&& launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (m->timing) {
+      a = take();
+      b = take();
+      if (a == nullptr || b == nullptr)
+        return fail(ATMO_ERROR_DEVICE, "hipEventCreate failed");
+      if (hipEventRecord(a, s) != hipSuccess)
+        return fail(ATMO_ERROR_DEVICE, "hipEventRecord failed");
+    }
+    ATMO_TRY(check_launch(launch(), what));
+    if (m->timing) {
+      if (hipEventRecord(b, s) != hipSuccess)
+        return fail(ATMO_ERROR_DEVICE, "hipEventRecord failed");
+      m->timed.push_back({pass, order, a, b});
+    }
+    return ATMO_OK;
+  }
+};
+
 // Model::Precompute (atmosphere/model.cc:1048-1215) for job 'ji'.
-atmo_status enqueue_precompute(const atmo_model* m, int ji, unsigned orders,
-                               hipStream_t s) {
-  ATMO_TRY(check_launch(atmo::launch_transmittance(single_launch(m, ji, kT), s),
-                        "transmittance"));
-  ATMO_TRY(check_launch(
-      atmo::launch_direct_irradiance(single_launch(m, ji, kE0), s),
-      "direct irradiance"));
-  ATMO_TRY(check_launch(
-      atmo::launch_single_scattering(single_launch(m, ji, kSS), s),
-      "single scattering"));
+atmo_status enqueue_precompute(atmo_model* m, int ji, unsigned orders,
+                               LaunchTimer& t) {
+  hipStream_t s = t.s;
+  ATMO_TRY(t.run(kT, 0, "transmittance", [&] {
+    return atmo::launch_transmittance(single_launch(m, ji, kT), s);
+  }));
+  ATMO_TRY(t.run(kE0, 0, "direct irradiance", [&] {
+    return atmo::launch_direct_irradiance(single_launch(m, ji, kE0), s);
+  }));
+  ATMO_TRY(t.run(kSS, 1, "single scattering", [&] {
+    return atmo::launch_single_scattering(single_launch(m, ji, kSS), s);
+  }));
   for (unsigned order = 2; order <= orders; ++order) {
-    ATMO_TRY(check_launch(atmo::launch_scattering_density(
-                              single_launch(m, ji, kSD), static_cast<int>(order), s),
-                          "scattering density"));
-    ATMO_TRY(check_launch(
-        atmo::launch_indirect_irradiance(single_launch(m, ji, kII),
-                                         static_cast<int>(order) - 1, s),
-        "indirect irradiance"));
-    ATMO_TRY(check_launch(
-        atmo::launch_multiple_scattering(single_launch(m, ji, kMS), s),
-        "multiple scattering"));
+    const int o = static_cast<int>(order);
+    ATMO_TRY(t.run(kSD, o, "scattering density", [&] {
+      return atmo::launch_scattering_density(single_launch(m, ji, kSD), o, s);
+    }));
+    ATMO_TRY(t.run(kII, o - 1, "indirect irradiance", [&] {
+      return atmo::launch_indirect_irradiance(single_launch(m, ji, kII), o - 1, s);
+    }));
+    ATMO_TRY(t.run(kMS, o, "multiple scattering", [&] {
+      return atmo::launch_multiple_scattering(single_launch(m, ji, kMS), s);
+    }));
   }
   return ATMO_OK;
 }
@@ -416,10 +510,21 @@ void build_spec(const atmo_model_params* p, Spec* sp) {
   s.half_precision = p->half_precision != 0;
 }
 
+// The radiance API exists only when at most 3 wavelengths are precomputed
+// (RADIANCE_API_ENABLED, atmosphere/model.cc:226, 769-772).
+atmo_status radiance_api(const atmo_model* m) {
+  if (m != nullptr && m->spec.num_precomputed_wavelengths > 3)
+    return fail(ATMO_ERROR_UNSUPPORTED,
+                "radiance lookups need num_precomputed_wavelengths <= 3; use the "
+                "luminance lookups");
+  return ATMO_OK;
+}
+
 size_t lut_bytes(const atmo_model* m, bool with_single_mie) {
   const Sizes& z = m->sizes;
-  size_t b = (z.t_texels() + z.e_texels() + z.s_texels()) * sizeof(float4);
-  if (with_single_mie) b += z.s_texels() * sizeof(float4);
+  const size_t texel = m->spec.half_precision ? sizeof(uint2) : sizeof(float4);
+  size_t b = (z.t_texels() + z.e_texels()) * sizeof(float4) + z.s_texels() * texel;
+  if (with_single_mie) b += z.s_texels() * texel;
   return b;
 }
 
@@ -455,12 +560,27 @@ atmo_status atmo_model_create(const atmo_model_params* p,
     m->solar_radiance[c] = static_cast<float>(
         interpolate(s.wavelengths, s.solar, rgb[c]) /
         (atmo::kPi * s.sun_angular_radius * s.sun_angular_radius));
+  // SKY/SUN_SPECTRAL_RADIANCE_TO_LUMINANCE (atmosphere/model.cc:668-686): in
+  // illuminance mode the LUTs hold luminance / MAX_LUMINOUS_EFFICACY.
+  if (s.num_precomputed_wavelengths > 3) {
+    for (int c = 0; c < 3; ++c) m->sky_k[c] = atmo_cie::kMaxLuminousEfficacy;
+  } else {
+    luminance_factors(s.wavelengths, s.solar, -3.0, m->sky_k);
+  }
+  luminance_factors(s.wavelengths, s.solar, 0.0, m->sun_k);
 
   ATMO_HIP(hipGetDevice(&m->device));
   const Sizes& sz = m->sizes;
   ATMO_HIP(hipMalloc(&m->T, sz.t_texels() * sizeof(float4)));
-  ATMO_HIP(hipMalloc(&m->S, sz.s_texels() * sizeof(float4)));
-  if (!s.combined) ATMO_HIP(hipMalloc(&m->SM, sz.s_texels() * sizeof(float4)));
+  if (s.half_precision) {
+    ATMO_HIP(hipMalloc(&m->S16, sz.s_texels() * sizeof(uint2)));
+    if (!s.combined) ATMO_HIP(hipMalloc(&m->SM16, sz.s_texels() * sizeof(uint2)));
+    ATMO_HIP(hipMalloc(&m->d_scales, 8 * sizeof(float)));
+    ATMO_HIP(hipMalloc(&m->d_maxbits, 8 * sizeof(unsigned)));
+  } else {
+    ATMO_HIP(hipMalloc(&m->S, sz.s_texels() * sizeof(float4)));
+    if (!s.combined) ATMO_HIP(hipMalloc(&m->SM, sz.s_texels() * sizeof(float4)));
+  }
   ATMO_HIP(hipMalloc(&m->E, sz.e_texels() * sizeof(float4)));
   const int iters = num_iterations(s);
   ATMO_HIP(hipMalloc(&m->d_jobs, (iters + 2) * sizeof(atmo::Job)));
@@ -483,11 +603,12 @@ void atmo_model_destroy(atmo_model* m) {
   if (m == nullptr) return;
   (void)hipDeviceSynchronize();
   free_temporaries(m);
-  float4* bufs[4] = {m->T, m->S, m->SM, m->E};
-  for (float4* b : bufs)
+  void* bufs[8] = {m->T, m->S, m->SM, m->E, m->S16, m->SM16, m->d_scales, m->d_maxbits};
+  for (void* b : bufs)
     if (b != nullptr) (void)hipFree(b);
   if (m->d_jobs != nullptr) (void)hipFree(m->d_jobs);
   if (m->d_first_block != nullptr) (void)hipFree(m->d_first_block);
+  for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
   delete m;
 }
 
@@ -496,6 +617,9 @@ atmo_status atmo_model_set_option(atmo_model* m, atmo_option option, int value) 
   switch (option) {
     case ATMO_OPTION_RETAIN_INTERMEDIATES:
       m->retain = value != 0;
+      return ATMO_OK;
+    case ATMO_OPTION_PASS_TIMING:
+      m->timing = value != 0;
       return ATMO_OK;
   }
   return fail(ATMO_ERROR_INVALID_ARGUMENT, "unknown option");
@@ -508,10 +632,19 @@ atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
   ATMO_TRY(alloc_temporaries(m));
   ATMO_TRY(upload_jobs(m, orders, s));
   const int iters = num_iterations(m->spec);
-  for (int i = 0; i < iters; ++i) ATMO_TRY(enqueue_precompute(m, i, orders, s));
+  m->timed.clear();
+  LaunchTimer timer{m, s};
+  for (int i = 0; i < iters; ++i) ATMO_TRY(enqueue_precompute(m, i, orders, timer));
   if (m->spec.num_precomputed_wavelengths > 3) {
-    ATMO_TRY(check_launch(atmo::launch_transmittance(single_launch(m, iters, kT), s),
-                          "transmittance (rgb)"));
+    ATMO_TRY(timer.run(kT, 0, "transmittance (rgb)", [&] {
+      return atmo::launch_transmittance(single_launch(m, iters, kT), s);
+    }));
+  }
+  if (m->spec.half_precision) {
+    ATMO_TRY(check_launch(atmo::launch_clear_max(single_launch(m, iters - 1, kClear), s),
+                          "fp16 maxima"));
+    ATMO_TRY(check_launch(atmo::launch_to_half(single_launch(m, iters - 1, kHalf), s),
+                          "fp16 conversion"));
   }
   m->initialized = true;
   if (!m->retain) {
@@ -520,6 +653,21 @@ atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
     // deleted (atmosphere/model.cc:966-974).
     ATMO_HIP(hipStreamSynchronize(s));
     free_temporaries(m);
+  }
+  return ATMO_OK;
+}
+
+atmo_status atmo_model_pass_timings(atmo_model* m, atmo_pass_timing* out, int capacity,
+                                    int* count) {
+  if (m == nullptr || count == nullptr || (capacity > 0 && out == nullptr))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL argument");
+  *count = static_cast<int>(m->timed.size());
+  for (int i = 0; i < *count && i < capacity; ++i) {
+    const atmo_model::Timed& t = m->timed[i];
+    ATMO_HIP(hipEventSynchronize(t.stop));
+    float ms = 0.0f;
+    ATMO_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+    out[i] = {t.pass, t.order, ms};
   }
   return ATMO_OK;
 }
@@ -547,6 +695,21 @@ atmo_status atmo_model_get_lut(const atmo_model* m, atmo_lut_id id, atmo_lut_vie
     v->height = z.eh;
     v->depth = 1;
   }
+  const bool half = m->spec.half_precision &&
+                    (id == ATMO_LUT_SCATTERING || id == ATMO_LUT_SINGLE_MIE_SCATTERING);
+  if (half) {
+    float sc[8];
+    ATMO_HIP(hipDeviceSynchronize());
+    ATMO_HIP(hipMemcpy(sc, m->d_scales, sizeof(sc), hipMemcpyDeviceToHost));
+    v->dtype = ATMO_DTYPE_F16;
+    const int off = id == ATMO_LUT_SCATTERING ? 0 : 4;
+    for (int c = 0; c < 4; ++c) v->scale[c] = sc[off + c];
+    v->data = id == ATMO_LUT_SCATTERING ? static_cast<void*>(m->S16)
+                                        : static_cast<void*>(m->SM16);
+    if (v->data == nullptr)
+      return fail(ATMO_ERROR_UNSUPPORTED, "this LUT is not held by the model");
+    return ATMO_OK;
+  }
   switch (id) {
     case ATMO_LUT_TRANSMITTANCE: v->data = m->T; break;
     case ATMO_LUT_SCATTERING: v->data = m->S; break;
@@ -569,7 +732,17 @@ atmo_status atmo_model_copy_lut(const atmo_model* m, atmo_lut_id id, float* host
   if (host == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "host is NULL");
   ATMO_HIP(hipDeviceSynchronize());
   const size_t texels = static_cast<size_t>(v.width) * v.height * v.depth;
-  ATMO_HIP(hipMemcpy(host, v.data, texels * 4 * sizeof(float), hipMemcpyDeviceToHost));
+  if (v.dtype == ATMO_DTYPE_F32) {
+    ATMO_HIP(hipMemcpy(host, v.data, texels * 4 * sizeof(float), hipMemcpyDeviceToHost));
+    return ATMO_OK;
+  }
+  std::vector<uint2> raw(texels);
+  ATMO_HIP(hipMemcpy(raw.data(), v.data, texels * sizeof(uint2), hipMemcpyDeviceToHost));
+  for (size_t t = 0; t < texels; ++t) {
+    const float4 f = atmo::unpack_half4(raw[t]);
+    const float c[4] = {f.x, f.y, f.z, f.w};
+    for (int k = 0; k < 4; ++k) host[4 * t + k] = v.scale[k] > 0.0f ? c[k] / v.scale[k] : 0.0f;
+  }
   return ATMO_OK;
 }
 
@@ -577,13 +750,16 @@ atmo_status atmo_model_footprint(const atmo_model* m, size_t* resident, size_t* 
   if (m == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "model is NULL");
   const Sizes& z = m->sizes;
   if (resident != nullptr) *resident = lut_bytes(m, !m->spec.combined);
-  if (scratch != nullptr)
-    *scratch = (3 * z.s_texels() + z.e_texels()) * sizeof(float4);
+  if (scratch != nullptr) {
+    size_t acc = m->spec.half_precision ? (m->spec.combined ? 1 : 2) : 0;
+    *scratch = ((3 + acc) * z.s_texels() + z.e_texels()) * sizeof(float4);
+  }
   return ATMO_OK;
 }
 
 atmo_status atmo_get_solar_radiance(const atmo_model* m, float* rgb) {
   if (m == nullptr || rgb == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL argument");
+  ATMO_TRY(radiance_api(m));
   for (int c = 0; c < 3; ++c) rgb[c] = m->solar_radiance[c];
   return ATMO_OK;
 }
@@ -599,19 +775,56 @@ static const atmo::Job* query_job(const atmo_model* m) {
   return m->d_jobs + (m->jobs.size() - 1);
 }
 
-atmo_status atmo_get_sky_radiance(const atmo_model* m, int64_t n, const float* camera,
-                                  const float* view_ray, const float* shadow_length,
-                                  const float* sun_direction, float* radiance,
-                                  float* transmittance, void* stream) {
+static atmo_status sky(const atmo_model* m, int64_t n, const float* camera,
+                       const float* view_ray, const float* shadow_length,
+                       const float* sun_direction, float* radiance, float* transmittance,
+                       void* stream, int luminance) {
   ATMO_TRY(check_query(m, n));
   if (n > 0 && (!camera || !view_ray || !shadow_length || !sun_direction || !radiance ||
                 !transmittance))
     return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL array");
   return check_launch(atmo::launch_sky_radiance(query_job(m), n, camera, view_ray,
                                                 shadow_length, sun_direction, radiance,
-                                                transmittance,
+                                                transmittance, luminance,
                                                 static_cast<hipStream_t>(stream)),
                       "sky radiance");
+}
+
+static atmo_status sky_to_point(const atmo_model* m, int64_t n, const float* camera,
+                                const float* point, const float* shadow_length,
+                                const float* sun_direction, float* radiance,
+                                float* transmittance, void* stream, int luminance) {
+  ATMO_TRY(check_query(m, n));
+  if (n > 0 && (!camera || !point || !shadow_length || !sun_direction || !radiance ||
+                !transmittance))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL array");
+  return check_launch(atmo::launch_sky_radiance_to_point(
+                          query_job(m), n, camera, point, shadow_length, sun_direction,
+                          radiance, transmittance, luminance,
+                          static_cast<hipStream_t>(stream)),
+                      "sky radiance to point");
+}
+
+static atmo_status sun_and_sky(const atmo_model* m, int64_t n, const float* point,
+                               const float* normal, const float* sun_direction,
+                               float* sun_out, float* sky_out, void* stream,
+                               int luminance) {
+  ATMO_TRY(check_query(m, n));
+  if (n > 0 && (!point || !normal || !sun_direction || !sun_out || !sky_out))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL array");
+  return check_launch(atmo::launch_sun_and_sky_irradiance(
+                          query_job(m), n, point, normal, sun_direction, sun_out, sky_out,
+                          luminance, static_cast<hipStream_t>(stream)),
+                      "sun and sky irradiance");
+}
+
+atmo_status atmo_get_sky_radiance(const atmo_model* m, int64_t n, const float* camera,
+                                  const float* view_ray, const float* shadow_length,
+                                  const float* sun_direction, float* radiance,
+                                  float* transmittance, void* stream) {
+  ATMO_TRY(radiance_api(m));
+  return sky(m, n, camera, view_ray, shadow_length, sun_direction, radiance,
+             transmittance, stream, 0);
 }
 
 atmo_status atmo_get_sky_radiance_to_point(const atmo_model* m, int64_t n,
@@ -619,14 +832,9 @@ atmo_status atmo_get_sky_radiance_to_point(const atmo_model* m, int64_t n,
                                            const float* shadow_length,
                                            const float* sun_direction, float* radiance,
                                            float* transmittance, void* stream) {
-  ATMO_TRY(check_query(m, n));
-  if (n > 0 && (!camera || !point || !shadow_length || !sun_direction || !radiance ||
-                !transmittance))
-    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL array");
-  return check_launch(atmo::launch_sky_radiance_to_point(
-                          query_job(m), n, camera, point, shadow_length, sun_direction,
-                          radiance, transmittance, static_cast<hipStream_t>(stream)),
-                      "sky radiance to point");
+  ATMO_TRY(radiance_api(m));
+  return sky_to_point(m, n, camera, point, shadow_length, sun_direction, radiance,
+                      transmittance, stream, 0);
 }
 
 atmo_status atmo_get_sun_and_sky_irradiance(const atmo_model* m, int64_t n,
@@ -634,13 +842,52 @@ atmo_status atmo_get_sun_and_sky_irradiance(const atmo_model* m, int64_t n,
                                             const float* sun_direction,
                                             float* sun_irradiance, float* sky_irradiance,
                                             void* stream) {
-  ATMO_TRY(check_query(m, n));
-  if (n > 0 && (!point || !normal || !sun_direction || !sun_irradiance || !sky_irradiance))
-    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL array");
-  return check_launch(atmo::launch_sun_and_sky_irradiance(
-                          query_job(m), n, point, normal, sun_direction, sun_irradiance,
-                          sky_irradiance, static_cast<hipStream_t>(stream)),
-                      "sun and sky irradiance");
+  ATMO_TRY(radiance_api(m));
+  return sun_and_sky(m, n, point, normal, sun_direction, sun_irradiance, sky_irradiance,
+                     stream, 0);
+}
+
+atmo_status atmo_get_sky_luminance(const atmo_model* m, int64_t n, const float* camera,
+                                   const float* view_ray, const float* shadow_length,
+                                   const float* sun_direction, float* luminance,
+                                   float* transmittance, void* stream) {
+  return sky(m, n, camera, view_ray, shadow_length, sun_direction, luminance,
+             transmittance, stream, 1);
+}
+
+atmo_status atmo_get_sky_luminance_to_point(const atmo_model* m, int64_t n,
+                                            const float* camera, const float* point,
+                                            const float* shadow_length,
+                                            const float* sun_direction, float* luminance,
+                                            float* transmittance, void* stream) {
+  return sky_to_point(m, n, camera, point, shadow_length, sun_direction, luminance,
+                      transmittance, stream, 1);
+}
+
+atmo_status atmo_get_sun_and_sky_illuminance(const atmo_model* m, int64_t n,
+                                             const float* point, const float* normal,
+                                             const float* sun_direction,
+                                             float* sun_illuminance,
+                                             float* sky_illuminance, void* stream) {
+  return sun_and_sky(m, n, point, normal, sun_direction, sun_illuminance, sky_illuminance,
+                     stream, 1);
+}
+
+atmo_status atmo_get_solar_luminance(const atmo_model* m, float* rgb) {
+  if (m == nullptr || rgb == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL argument");
+  for (int c = 0; c < 3; ++c)
+    rgb[c] = static_cast<float>(m->solar_radiance[c] * m->sun_k[c]);
+  return ATMO_OK;
+}
+
+atmo_status atmo_get_luminance_factors(const atmo_model* m, double* sky_k, double* sun_k) {
+  if (m == nullptr || sky_k == nullptr || sun_k == nullptr)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL argument");
+  for (int c = 0; c < 3; ++c) {
+    sky_k[c] = m->sky_k[c];
+    sun_k[c] = m->sun_k[c];
+  }
+  return ATMO_OK;
 }
 
 atmo_status atmo_lower_parameters(const atmo_model_params* p, const double* lambdas3,
@@ -756,6 +1003,28 @@ atmo_status atmo_device_synchronize(void) {
   ATMO_HIP(hipDeviceSynchronize());
   return ATMO_OK;
 }
+atmo_status atmo_event_create(void** event) {
+  if (event == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "event is NULL");
+  hipEvent_t e;
+  ATMO_HIP(hipEventCreate(&e));
+  *event = e;
+  return ATMO_OK;
+}
+atmo_status atmo_event_destroy(void* event) {
+  ATMO_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return ATMO_OK;
+}
+atmo_status atmo_event_record(void* event, void* stream) {
+  ATMO_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+  return ATMO_OK;
+}
+atmo_status atmo_event_elapsed_ms(void* start, void* stop, float* ms) {
+  if (ms == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "ms is NULL");
+  ATMO_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+  ATMO_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
+                               static_cast<hipEvent_t>(stop)));
+  return ATMO_OK;
+}
 
 }  // extern "C"
 
@@ -799,6 +1068,8 @@ atmo_status enqueue_levels(const atmo_batch* b, hipStream_t s) {
       case kSD: e = atmo::launch_scattering_density(pl, l.order, s); break;
       case kII: e = atmo::launch_indirect_irradiance(pl, l.order, s); break;
       case kMS: e = atmo::launch_multiple_scattering(pl, s); break;
+      case kClear: e = atmo::launch_clear_max(pl, s); break;
+      case kHalf: e = atmo::launch_to_half(pl, s); break;
       default: break;
     }
     ATMO_TRY(check_launch(e, "batched pass"));
@@ -895,6 +1166,12 @@ atmo_status atmo_batch_create(atmo_model* const* models, const unsigned* orders,
     if (models[i]->spec.num_precomputed_wavelengths > 3)
       rgb.push_back({i, num_iterations(models[i]->spec)});
   add_level(kT, 0, rgb);
+  std::vector<std::pair<int, int>> half;
+  for (int i = 0; i < count; ++i)
+    if (models[i]->spec.half_precision)
+      half.push_back({i, num_iterations(models[i]->spec) - 1});
+  add_level(kClear, 0, half);
+  add_level(kHalf, 0, half);
 
   ATMO_HIP(hipMalloc(&b->d_jobs, jobs.size() * sizeof(atmo::Job)));
   ATMO_HIP(hipMalloc(&b->d_first_block, fb.size() * sizeof(int)));
