@@ -195,6 +195,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU oracle work for the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device once (one blit kernel "
+                         "reading and writing MIB MiB), as the reference point of "
+                         "HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -204,6 +208,15 @@ def main():
     import numpy as np
     import precomputed_atmospheric_scattering_amd as atmo
     from precomputed_atmospheric_scattering_amd import configs
+
+    if args.copy_probe > 0:
+        nbytes = args.copy_probe << 20
+        src, dst = atmo.DeviceArray((nbytes,), np.uint8), atmo.DeviceArray((nbytes,), np.uint8)
+        atmo._capi.check(atmo.lib().atmo_memcpy_to_device(dst.ptr, src.ptr, nbytes, None),
+                         "copy probe")
+        atmo.synchronize()
+        src.free()
+        dst.free()
 
     cfg = configs.spectral(half_precision=False)
     kw, orders = configs.model_kwargs(cfg)

@@ -285,12 +285,104 @@ atmo_status atmo_convert_spectrum_to_linear_srgb(int n,
                                                  double* r, double* g,
                                                  double* b);
 
+/* ------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference). A worker thread takes
+ * precompute requests and lookup batches as they arrive. Precompute requests
+ * are admitted in arrival order by device-memory footprint (resident LUTs of
+ * all models held + Init temporaries of running batches <= budget) into up
+ * to max_inflight_batches concurrent batches, each on its own stream:
+ * a request that arrives while other batches run joins the next batch as soon
+ * as a slot and memory are free, and its work overlaps the running ones
+ * (continuous batching). Finished LUT sets stay resident until released: a
+ * request that does not fit waits for batches to finish and models to be
+ * released; one that cannot fit in an empty server completes with
+ * ATMO_ERROR_OUT_OF_MEMORY.
+ * Lookups run on resident models (or wait on the device for a model whose
+ * precompute is in flight). Every request completes with its submit, start
+ * and end times, for latency statistics.
+ * ---------------------------------------------------------------------- */
+typedef struct atmo_server atmo_server;
+typedef struct atmo_server_config {
+  size_t memory_budget;      /* bytes of device memory; 0 = 80% of free */
+  int max_inflight_batches;  /* concurrent precompute batches; 0 = 3 */
+  int max_batch_models;      /* models per batch; 0 = 16 */
+  int lookup_streams;        /* streams for lookups; 0 = 2 */
+} atmo_server_config;
+
+typedef enum atmo_request_kind {
+  ATMO_REQUEST_PRECOMPUTE = 0,
+  ATMO_REQUEST_LOOKUP = 1,
+} atmo_request_kind;
+
+typedef enum atmo_lookup {
+  ATMO_LOOKUP_SKY = 0,          /* a=camera b=view_ray c=shadow_length d=sun */
+  ATMO_LOOKUP_SKY_TO_POINT = 1, /* a=camera b=point c=shadow_length d=sun */
+  ATMO_LOOKUP_SUN_AND_SKY = 2,  /* a=point b=normal c=sun d unused */
+} atmo_lookup;
+
+typedef struct atmo_completion {
+  uint64_t request_id;
+  int kind;            /* atmo_request_kind */
+  int status;          /* atmo_status */
+  uint64_t model_id;   /* precompute: id of the new model (= request_id) */
+  int batch_models;    /* precompute: models in the batch that ran it */
+  double submit_ms;    /* host times since the server started */
+  double start_ms;     /* admitted and launched */
+  double end_ms;       /* device work observed complete */
+} atmo_completion;
+
+typedef struct atmo_server_stats {
+  size_t budget_bytes;
+  size_t resident_bytes;
+  size_t scratch_bytes;
+  int resident_models;
+  int pending_precomputes;
+  int pending_lookups;
+  int inflight_batches;
+  int inflight_lookups;
+  uint64_t completed_precomputes;
+  uint64_t completed_lookups;
+  uint64_t failed;
+} atmo_server_stats;
+
+atmo_status atmo_server_create(const atmo_server_config* config,
+                               atmo_server** out);
+/* Waits for in-flight work, then frees every model and buffer. */
+void atmo_server_destroy(atmo_server* server);
+/* Copies the parameters; the model id of the result is *request_id. */
+atmo_status atmo_server_submit_precompute(atmo_server* server,
+                                          const atmo_model_params* params,
+                                          const atmo_lut_sizes* sizes,
+                                          unsigned num_scattering_orders,
+                                          uint64_t* request_id);
+/* Device arrays as for the lookup functions; they must stay valid until the
+ * request completes. luminance selects the luminance API. */
+atmo_status atmo_server_submit_lookup(atmo_server* server, uint64_t model_id,
+                                      atmo_lookup lookup, int luminance,
+                                      int64_t n, const float* a,
+                                      const float* b, const float* c,
+                                      const float* d, float* out0,
+                                      float* out1, uint64_t* request_id);
+/* Moves up to capacity completions to out; waits up to timeout_ms for at
+ * least one when none is ready (0 = do not wait). */
+atmo_status atmo_server_poll(atmo_server* server, atmo_completion* out,
+                             int capacity, int* count, int timeout_ms);
+/* The LUTs of the model are freed once its launched lookups are done;
+ * lookups on it still queued complete with ATMO_ERROR_NOT_INITIALIZED. */
+atmo_status atmo_server_release(atmo_server* server, uint64_t model_id);
+/* The resident model (for LUT views); valid until released. */
+atmo_status atmo_server_model(atmo_server* server, uint64_t model_id,
+                              atmo_model** model);
+atmo_status atmo_server_get_stats(atmo_server* server,
+                                  atmo_server_stats* stats);
+
 const char* atmo_last_error(void);
 
 /* Minimal device-memory and stream helpers, so that callers without their
  * own HIP runtime (ctypes, cgo) can drive the lookups. */
 atmo_status atmo_malloc(void** ptr, size_t bytes);
 atmo_status atmo_free(void* ptr);
+/* dst is device memory; src is host or device memory (asynchronous). */
 atmo_status atmo_memcpy_to_device(void* dst, const void* src, size_t bytes,
                                   void* stream);
 atmo_status atmo_memcpy_to_host(void* dst, const void* src, size_t bytes,

@@ -268,7 +268,8 @@ class Model:
 
     def close(self):
         if self._handle:
-            lib().atmo_model_destroy(self._handle)
+            if not getattr(self, "_borrowed", False):
+                lib().atmo_model_destroy(self._handle)
             self._handle = None
 
     def __del__(self):
@@ -453,6 +454,109 @@ class Batch:
         if self.handle:
             lib().atmo_batch_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over one GPU (atmo_server_*): precompute requests are
+    admitted by device-memory footprint into concurrent batches (continuous
+    batching); lookups run on resident models. Requests complete
+    asynchronously; poll() returns their completion records."""
+
+    def __init__(self, memory_budget=0, max_inflight_batches=0, max_batch_models=0,
+                 lookup_streams=0):
+        cfg = _capi.ServerConfig(int(memory_budget), int(max_inflight_batches),
+                                 int(max_batch_models), int(lookup_streams))
+        h = C.c_void_p()
+        check(lib().atmo_server_create(C.byref(cfg), C.byref(h)), "atmo_server_create")
+        self.handle = h
+        self._keep = {}   # request id -> objects that must outlive the request
+
+    def submit_precompute(self, num_scattering_orders=4, lut_sizes=None, **params):
+        """Model constructor parameters as keywords; returns the request id,
+        which is also the id of the resulting model."""
+        p = ModelParameters(**params)
+        sizes = dict(DEFAULT_SIZES)
+        if lut_sizes:
+            sizes.update(lut_sizes)
+        z = _capi.LutSizes(**sizes)
+        rid = C.c_uint64()
+        check(lib().atmo_server_submit_precompute(self.handle, C.byref(p.c), C.byref(z),
+                                                  int(num_scattering_orders),
+                                                  C.byref(rid)), "submit_precompute")
+        return rid.value
+
+    def submit_lookup(self, model_id, lookup, n, inputs, outputs, luminance=False):
+        """lookup: 'sky', 'sky_to_point' or 'sun_and_sky'. inputs/outputs are
+        DeviceArrays in the order of the corresponding Model *_device method;
+        they are kept alive until the request completes."""
+        kind = {"sky": _capi.LOOKUP_SKY, "sky_to_point": _capi.LOOKUP_SKY_TO_POINT,
+                "sun_and_sky": _capi.LOOKUP_SUN_AND_SKY}[lookup]
+        ins = [a.ptr if a is not None else None for a in inputs] + [None] * (4 - len(inputs))
+        outs = [a.ptr for a in outputs]
+        rid = C.c_uint64()
+        check(lib().atmo_server_submit_lookup(self.handle, int(model_id), kind,
+                                              int(bool(luminance)), int(n), *ins, *outs,
+                                              C.byref(rid)), "submit_lookup")
+        self._keep[rid.value] = (inputs, outputs)
+        return rid.value
+
+    def poll(self, capacity=256, timeout_ms=0):
+        buf = (_capi.Completion * capacity)()
+        n = C.c_int()
+        check(lib().atmo_server_poll(self.handle, buf, capacity, C.byref(n),
+                                     int(timeout_ms)), "poll")
+        out = []
+        for c in buf[:n.value]:
+            self._keep.pop(c.request_id, None)
+            out.append(dict(request_id=c.request_id, kind=c.kind, status=c.status,
+                            model_id=c.model_id, batch_models=c.batch_models,
+                            submit_ms=c.submit_ms, start_ms=c.start_ms, end_ms=c.end_ms))
+        return out
+
+    def wait(self, request_ids, timeout_s=60.0):
+        """Poll until every id in request_ids has completed; returns their
+        completion records by id."""
+        import time
+        want = set(request_ids)
+        got = {}
+        deadline = time.monotonic() + timeout_s
+        while want - set(got):
+            if time.monotonic() > deadline:
+                raise TimeoutError("requests did not complete: %s" % sorted(want - set(got)))
+            for c in self.poll(timeout_ms=20):
+                got[c["request_id"]] = c
+        return got
+
+    def model(self, model_id):
+        """A borrowed Model view of a resident model (lut_view, lut, lookups);
+        valid until the model is released."""
+        h = C.c_void_p()
+        check(lib().atmo_server_model(self.handle, int(model_id), C.byref(h)),
+              "atmo_server_model")
+        m = Model.__new__(Model)
+        m._handle = h
+        m._borrowed = True
+        return m
+
+    def release(self, model_id):
+        check(lib().atmo_server_release(self.handle, int(model_id)), "release")
+
+    def stats(self):
+        st = _capi.ServerStats()
+        check(lib().atmo_server_get_stats(self.handle, C.byref(st)), "stats")
+        return {k: getattr(st, k) for k, _ in _capi.ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().atmo_server_destroy(self.handle)
+            self.handle = None
+            self._keep.clear()
 
     def __del__(self):
         try:

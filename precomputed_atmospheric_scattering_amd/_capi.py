@@ -88,6 +88,31 @@ class Atmosphere(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("memory_budget", C.c_size_t), ("max_inflight_batches", C.c_int),
+                ("max_batch_models", C.c_int), ("lookup_streams", C.c_int)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("kind", C.c_int), ("status", C.c_int),
+                ("model_id", C.c_uint64), ("batch_models", C.c_int),
+                ("submit_ms", C.c_double), ("start_ms", C.c_double),
+                ("end_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("budget_bytes", C.c_size_t), ("resident_bytes", C.c_size_t),
+                ("scratch_bytes", C.c_size_t), ("resident_models", C.c_int),
+                ("pending_precomputes", C.c_int), ("pending_lookups", C.c_int),
+                ("inflight_batches", C.c_int), ("inflight_lookups", C.c_int),
+                ("completed_precomputes", C.c_uint64), ("completed_lookups", C.c_uint64),
+                ("failed", C.c_uint64)]
+
+
+REQUEST_PRECOMPUTE, REQUEST_LOOKUP = 0, 1
+LOOKUP_SKY, LOOKUP_SKY_TO_POINT, LOOKUP_SUN_AND_SKY = 0, 1, 2
+
+
 # Every symbol declared in include/atmo.h, with its ctypes signature.
 VP, FP, I64, SZ = C.c_void_p, C.POINTER(C.c_float), C.c_int64, C.c_size_t
 ST = C.c_int
@@ -120,6 +145,17 @@ SIGNATURES = {
     "atmo_lower_parameters": (ST, [C.POINTER(ModelParams), DP, C.POINTER(Atmosphere)]),
     "atmo_num_iterations": (C.c_int, [C.POINTER(ModelParams)]),
     "atmo_iteration": (ST, [C.POINTER(ModelParams), C.c_int, DP, FP]),
+    "atmo_server_create": (ST, [C.POINTER(ServerConfig), C.POINTER(VP)]),
+    "atmo_server_destroy": (None, [VP]),
+    "atmo_server_submit_precompute": (ST, [VP, C.POINTER(ModelParams), C.POINTER(LutSizes),
+                                           C.c_uint, C.POINTER(C.c_uint64)]),
+    "atmo_server_submit_lookup": (ST, [VP, C.c_uint64, C.c_int, C.c_int, I64, VP, VP, VP,
+                                       VP, VP, VP, C.POINTER(C.c_uint64)]),
+    "atmo_server_poll": (ST, [VP, C.POINTER(Completion), C.c_int, C.POINTER(C.c_int),
+                              C.c_int]),
+    "atmo_server_release": (ST, [VP, C.c_uint64]),
+    "atmo_server_model": (ST, [VP, C.c_uint64, C.POINTER(VP)]),
+    "atmo_server_get_stats": (ST, [VP, C.POINTER(ServerStats)]),
     "atmo_last_error": (C.c_char_p, []),
     "atmo_malloc": (ST, [C.POINTER(VP), SZ]),
     "atmo_free": (ST, [VP]),

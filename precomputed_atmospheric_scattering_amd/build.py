@@ -27,6 +27,7 @@ SOURCES = [
     ("precompute.hip", True),
     ("query.hip", True),
     ("runtime.cpp", True),   # host code using the HIP runtime
+    ("server.cpp", True),    # serving loop (worker thread, HIP runtime)
     ("model.cc", False),     # C++ Model facade, host only
 ]
 

@@ -16,8 +16,9 @@
 #include "atmo_device.h"
 #include "cie_1931.h"
 #include "kernels.h"
+#include "runtime_internal.h"
 
-namespace {
+namespace atmo_rt {
 
 thread_local std::string g_last_error;
 
@@ -89,16 +90,6 @@ void luminance_factors(const std::vector<double>& wl, const std::vector<double>&
   }
   for (int c = 0; c < 3; ++c) k[c] *= atmo_cie::kMaxLuminousEfficacy;
 }
-
-struct Spec {
-  std::vector<double> wavelengths, solar, rayleigh, mie_sca, mie_ext, absorption,
-      albedo;
-  std::vector<atmo_density_layer> prof_r, prof_m, prof_a;
-  double sun_angular_radius, bottom_radius, top_radius, g, max_sun_zenith_angle,
-      length_unit;
-  unsigned num_precomputed_wavelengths;
-  bool combined, half_precision;
-};
 
 int num_iterations(const Spec& s) {
   return s.num_precomputed_wavelengths <= 3
@@ -172,15 +163,6 @@ void lower(const Spec& s, const double* lambdas, atmo::Job* j) {
   j->A = (D - d_min) / (d_max - d_min);
 }
 
-struct Sizes {
-  int tw, th, nr, nmu, nmus, nnu, ew, eh;
-  size_t t_texels() const { return static_cast<size_t>(tw) * th; }
-  size_t s_texels() const {
-    return static_cast<size_t>(nr) * nmu * nmus * nnu;
-  }
-  size_t e_texels() const { return static_cast<size_t>(ew) * eh; }
-};
-
 void set_sizes(atmo::Job* j, const Sizes& z) {
   j->tw = z.tw;
   j->th = z.th;
@@ -191,8 +173,6 @@ void set_sizes(atmo::Job* j, const Sizes& z) {
   j->ew = z.ew;
   j->eh = z.eh;
 }
-
-enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kHalf, kClear, kPasses };
 
 int pass_blocks(Pass p, const atmo::Job& j) {
   switch (p) {
@@ -208,47 +188,18 @@ int pass_blocks(Pass p, const atmo::Job& j) {
   }
 }
 
-}  // namespace
+}  // namespace atmo_rt
 
-// ---------------------------------------------------------------------------
-// Model object.
-// ---------------------------------------------------------------------------
+using namespace atmo_rt;
 
-struct atmo_model {
-  Spec spec;
-  Sizes sizes;
-  int device = 0;
-  bool retain = false;
-  bool initialized = false;
-  float4 *T = nullptr, *S = nullptr, *SM = nullptr, *E = nullptr;
-  float4 *dE = nullptr, *dR = nullptr, *dM = nullptr, *dJ = nullptr;
-  // FP16 storage: resident half LUTs; S and SM above are then FP32
-  // accumulators that live only during Init, like the deltas.
-  uint2 *S16 = nullptr, *SM16 = nullptr;
-  float* d_scales = nullptr;
-  unsigned* d_maxbits = nullptr;
-  // Device job table: [iterations..., rgb transmittance job, query job].
-  std::vector<atmo::Job> jobs;
-  atmo::Job* d_jobs = nullptr;
-  // Per-pass block ranges of a single job: {0, blocks(pass)} for each pass.
-  int* d_first_block = nullptr;
-  int first_block_host[2 * kPasses] = {};
-  float solar_radiance[3] = {};
-  double sky_k[3] = {}, sun_k[3] = {};
-  // Launch timing (ATMO_OPTION_PASS_TIMING).
-  bool timing = false;
-  struct Timed {
-    int pass, order;
-    hipEvent_t start, stop;
-  };
-  std::vector<Timed> timed;   // events of the last Init, in launch order
-  std::vector<hipEvent_t> event_pool;
-};
-
-namespace {
+namespace atmo_rt {
 
 // Temporaries are released once the work that reads them has finished.
 void free_temporaries(atmo_model* m) {
+  if (m->external_temporaries) {
+    drop_external_temporaries(m);
+    return;
+  }
   std::vector<float4**> bufs = {&m->dE, &m->dR, &m->dM, &m->dJ};
   if (m->spec.half_precision) {
     bufs.push_back(&m->S);
@@ -274,6 +225,39 @@ atmo_status alloc_temporaries(atmo_model* m) {
       ATMO_HIP(hipMalloc(&m->SM, z.s_texels() * sizeof(float4)));
   }
   return ATMO_OK;
+}
+
+// Sizes of the temporaries, each rounded up to 256 bytes.
+static size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+size_t temporaries_bytes(const atmo_model* m) {
+  const Sizes& z = m->sizes;
+  size_t b = round256(z.e_texels() * sizeof(float4)) +
+             3 * round256(z.s_texels() * sizeof(float4));
+  if (m->spec.half_precision)
+    b += (m->spec.combined ? 1 : 2) * round256(z.s_texels() * sizeof(float4));
+  return b;
+}
+
+void use_external_temporaries(atmo_model* m, char* base) {
+  const Sizes& z = m->sizes;
+  const size_t es = round256(z.e_texels() * sizeof(float4));
+  const size_t ss = round256(z.s_texels() * sizeof(float4));
+  m->dE = reinterpret_cast<float4*>(base);
+  m->dR = reinterpret_cast<float4*>(base + es);
+  m->dM = reinterpret_cast<float4*>(base + es + ss);
+  m->dJ = reinterpret_cast<float4*>(base + es + 2 * ss);
+  if (m->spec.half_precision) {
+    m->S = reinterpret_cast<float4*>(base + es + 3 * ss);
+    if (!m->spec.combined) m->SM = reinterpret_cast<float4*>(base + es + 4 * ss);
+  }
+  m->external_temporaries = true;
+}
+
+void drop_external_temporaries(atmo_model* m) {
+  m->dE = m->dR = m->dM = m->dJ = nullptr;
+  if (m->spec.half_precision) m->S = m->SM = nullptr;
+  m->external_temporaries = false;
 }
 
 void fill_job_luts(atmo_model* m, atmo::Job* j) {
@@ -343,12 +327,6 @@ atmo_status check_launch(hipError_t e, const char* what) {
     return fail(ATMO_ERROR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
   return ATMO_OK;
 }
-
-#define ATMO_TRY(x)                      \
-  do {                                   \
-    atmo_status st_ = (x);               \
-    if (st_ != ATMO_OK) return st_;      \
-  } while (0)
 
 // Records start/stop events around each launch when timing is enabled.
 struct LaunchTimer {
@@ -528,7 +506,9 @@ size_t lut_bytes(const atmo_model* m, bool with_single_mie) {
   return b;
 }
 
-}  // namespace
+}  // namespace atmo_rt
+
+using namespace atmo_rt;
 
 extern "C" {
 
@@ -974,7 +954,7 @@ atmo_status atmo_free(void* ptr) {
   return ATMO_OK;
 }
 atmo_status atmo_memcpy_to_device(void* dst, const void* src, size_t bytes, void* stream) {
-  ATMO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+  ATMO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault,
                           static_cast<hipStream_t>(stream)));
   return ATMO_OK;
 }
@@ -1036,12 +1016,7 @@ atmo_status atmo_event_elapsed_ms(void* start, void* stop, float* ms) {
 struct atmo_batch {
   std::vector<atmo_model*> models;
   std::vector<unsigned> orders;
-  struct Level {
-    Pass pass;
-    int order;  // SD: order computed; II: order read; others unused
-    int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew;
-  };
-  std::vector<Level> levels;
+  atmo_rt::LevelPlan plan;
   atmo::Job* d_jobs = nullptr;
   int* d_first_block = nullptr;
   hipGraph_t graph = nullptr;
@@ -1049,13 +1024,76 @@ struct atmo_batch {
   bool use_graph = true;
 };
 
-namespace {
+namespace atmo_rt {
 
-atmo_status enqueue_levels(const atmo_batch* b, hipStream_t s) {
-  for (const atmo_batch::Level& l : b->levels) {
+void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
+                 LevelPlan* plan) {
+  plan->jobs.clear();
+  plan->first_block.clear();
+  plan->levels.clear();
+  auto add_level = [&](Pass p, int order, const std::vector<std::pair<int, int>>& sel) {
+    if (sel.empty()) return;
+    Level l;
+    l.pass = p;
+    l.order = order;
+    l.job_offset = static_cast<int>(plan->jobs.size());
+    l.njobs = static_cast<int>(sel.size());
+    l.fb_offset = static_cast<int>(plan->first_block.size());
+    l.max_nnu = 0;
+    l.max_ew = 0;
+    int total = 0;
+    for (const auto& mj : sel) {
+      const atmo::Job& j = models[mj.first]->jobs[mj.second];
+      plan->jobs.push_back(j);
+      plan->first_block.push_back(total);
+      total += pass_blocks(p, j);
+      l.max_nnu = std::max(l.max_nnu, j.nnu);
+      l.max_ew = std::max(l.max_ew, j.ew);
+    }
+    plan->first_block.push_back(total);
+    l.total_blocks = total;
+    plan->levels.push_back(l);
+  };
+  int max_iters = 0;
+  unsigned max_orders = 0;
+  for (int i = 0; i < count; ++i) {
+    max_iters = std::max(max_iters, num_iterations(models[i]->spec));
+    max_orders = std::max(max_orders, orders[i]);
+  }
+  for (int it = 0; it < max_iters; ++it) {
+    std::vector<std::pair<int, int>> sel;
+    for (int i = 0; i < count; ++i)
+      if (it < num_iterations(models[i]->spec)) sel.push_back({i, it});
+    add_level(kT, 0, sel);
+    add_level(kE0, 0, sel);
+    add_level(kSS, 0, sel);
+    for (unsigned n = 2; n <= max_orders; ++n) {
+      std::vector<std::pair<int, int>> on;
+      for (const auto& mj : sel)
+        if (orders[mj.first] >= n) on.push_back(mj);
+      add_level(kSD, static_cast<int>(n), on);
+      add_level(kII, static_cast<int>(n) - 1, on);
+      add_level(kMS, 0, on);
+    }
+  }
+  std::vector<std::pair<int, int>> rgb, half;
+  for (int i = 0; i < count; ++i) {
+    if (models[i]->spec.num_precomputed_wavelengths > 3)
+      rgb.push_back({i, num_iterations(models[i]->spec)});
+    if (models[i]->spec.half_precision)
+      half.push_back({i, num_iterations(models[i]->spec) - 1});
+  }
+  add_level(kT, 0, rgb);
+  add_level(kClear, 0, half);
+  add_level(kHalf, 0, half);
+}
+
+atmo_status enqueue_levels(const LevelPlan& plan, const atmo::Job* d_jobs,
+                           const int* d_first_block, hipStream_t s) {
+  for (const Level& l : plan.levels) {
     atmo::PassLaunch pl;
-    pl.jobs = b->d_jobs + l.job_offset;
-    pl.first_block = b->d_first_block + l.fb_offset;
+    pl.jobs = d_jobs + l.job_offset;
+    pl.first_block = d_first_block + l.fb_offset;
     pl.njobs = l.njobs;
     pl.total_blocks = l.total_blocks;
     pl.max_nnu = l.max_nnu;
@@ -1076,6 +1114,10 @@ atmo_status enqueue_levels(const atmo_batch* b, hipStream_t s) {
   }
   return ATMO_OK;
 }
+
+}  // namespace atmo_rt
+
+namespace {
 
 void destroy_batch(atmo_batch* b) {
   if (b == nullptr) return;
@@ -1113,78 +1155,20 @@ atmo_status atmo_batch_create(atmo_model* const* models, const unsigned* orders,
     ATMO_TRY(upload_jobs(models[i], orders[i], nullptr));
   }
   ATMO_HIP(hipStreamSynchronize(nullptr));
-
-  std::vector<atmo::Job> jobs;
-  std::vector<int> fb;
-  auto add_level = [&](Pass p, int order, const std::vector<std::pair<int, int>>& sel) {
-    if (sel.empty()) return;
-    atmo_batch::Level l;
-    l.pass = p;
-    l.order = order;
-    l.job_offset = static_cast<int>(jobs.size());
-    l.njobs = static_cast<int>(sel.size());
-    l.fb_offset = static_cast<int>(fb.size());
-    l.max_nnu = 0;
-    l.max_ew = 0;
-    int total = 0;
-    for (const auto& mj : sel) {
-      const atmo::Job& j = b->models[mj.first]->jobs[mj.second];
-      jobs.push_back(j);
-      fb.push_back(total);
-      total += pass_blocks(p, j);
-      l.max_nnu = std::max(l.max_nnu, j.nnu);
-      l.max_ew = std::max(l.max_ew, j.ew);
-    }
-    fb.push_back(total);
-    l.total_blocks = total;
-    b->levels.push_back(l);
-  };
-  int max_iters = 0;
-  unsigned max_orders = 0;
-  for (int i = 0; i < count; ++i) {
-    max_iters = std::max(max_iters, num_iterations(models[i]->spec));
-    max_orders = std::max(max_orders, orders[i]);
-  }
-  for (int it = 0; it < max_iters; ++it) {
-    std::vector<std::pair<int, int>> sel;
-    for (int i = 0; i < count; ++i)
-      if (it < num_iterations(models[i]->spec)) sel.push_back({i, it});
-    add_level(kT, 0, sel);
-    add_level(kE0, 0, sel);
-    add_level(kSS, 0, sel);
-    for (unsigned n = 2; n <= max_orders; ++n) {
-      std::vector<std::pair<int, int>> on;
-      for (const auto& mj : sel)
-        if (b->orders[mj.first] >= n) on.push_back(mj);
-      add_level(kSD, static_cast<int>(n), on);
-      add_level(kII, static_cast<int>(n) - 1, on);
-      add_level(kMS, 0, on);
-    }
-  }
-  std::vector<std::pair<int, int>> rgb;
-  for (int i = 0; i < count; ++i)
-    if (models[i]->spec.num_precomputed_wavelengths > 3)
-      rgb.push_back({i, num_iterations(models[i]->spec)});
-  add_level(kT, 0, rgb);
-  std::vector<std::pair<int, int>> half;
-  for (int i = 0; i < count; ++i)
-    if (models[i]->spec.half_precision)
-      half.push_back({i, num_iterations(models[i]->spec) - 1});
-  add_level(kClear, 0, half);
-  add_level(kHalf, 0, half);
-
-  ATMO_HIP(hipMalloc(&b->d_jobs, jobs.size() * sizeof(atmo::Job)));
-  ATMO_HIP(hipMalloc(&b->d_first_block, fb.size() * sizeof(int)));
-  ATMO_HIP(hipMemcpy(b->d_jobs, jobs.data(), jobs.size() * sizeof(atmo::Job),
+  plan_levels(models, orders, count, &b->plan);
+  const LevelPlan& plan = b->plan;
+  ATMO_HIP(hipMalloc(&b->d_jobs, plan.jobs.size() * sizeof(atmo::Job)));
+  ATMO_HIP(hipMalloc(&b->d_first_block, plan.first_block.size() * sizeof(int)));
+  ATMO_HIP(hipMemcpy(b->d_jobs, plan.jobs.data(), plan.jobs.size() * sizeof(atmo::Job),
                      hipMemcpyHostToDevice));
-  ATMO_HIP(hipMemcpy(b->d_first_block, fb.data(), fb.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
+  ATMO_HIP(hipMemcpy(b->d_first_block, plan.first_block.data(),
+                     plan.first_block.size() * sizeof(int), hipMemcpyHostToDevice));
   if (b->use_graph) {
     hipStream_t cap;
     ATMO_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     atmo_status st = ATMO_OK;
-    if (e == hipSuccess) st = enqueue_levels(b.get(), cap);
+    if (e == hipSuccess) st = enqueue_levels(plan, b->d_jobs, b->d_first_block, cap);
     hipError_t e2 = hipStreamEndCapture(cap, &b->graph);
     (void)hipStreamDestroy(cap);
     if (e != hipSuccess || e2 != hipSuccess || st != ATMO_OK)
@@ -1202,14 +1186,14 @@ atmo_status atmo_batch_launch(atmo_batch* b, void* stream) {
   if (b->use_graph) {
     ATMO_HIP(hipGraphLaunch(b->exec, s));
   } else {
-    ATMO_TRY(enqueue_levels(b, s));
+    ATMO_TRY(enqueue_levels(b->plan, b->d_jobs, b->d_first_block, s));
   }
   for (atmo_model* m : b->models) m->initialized = true;
   return ATMO_OK;
 }
 
 int atmo_batch_num_launches(const atmo_batch* b) {
-  return b == nullptr ? 0 : static_cast<int>(b->levels.size());
+  return b == nullptr ? 0 : static_cast<int>(b->plan.levels.size());
 }
 
 void atmo_batch_destroy(atmo_batch* b) {

@@ -1,0 +1,129 @@
+// Internal interface of libatmo's runtime, shared by runtime.cpp (the C ABI
+// of models and batches) and server.cpp (the serving loop). Not installed.
+#ifndef ATMO_RUNTIME_INTERNAL_H_
+#define ATMO_RUNTIME_INTERNAL_H_
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/atmo.h"
+#include "atmo_device.h"
+#include "kernels.h"
+
+namespace atmo_rt {
+
+// The constructor parameters, copied.
+struct Spec {
+  std::vector<double> wavelengths, solar, rayleigh, mie_sca, mie_ext, absorption,
+      albedo;
+  std::vector<atmo_density_layer> prof_r, prof_m, prof_a;
+  double sun_angular_radius, bottom_radius, top_radius, g, max_sun_zenith_angle,
+      length_unit;
+  unsigned num_precomputed_wavelengths;
+  bool combined, half_precision;
+};
+
+
+struct Sizes {
+  int tw, th, nr, nmu, nmus, nnu, ew, eh;
+  size_t t_texels() const { return static_cast<size_t>(tw) * th; }
+  size_t s_texels() const {
+    return static_cast<size_t>(nr) * nmu * nmus * nnu;
+  }
+  size_t e_texels() const { return static_cast<size_t>(ew) * eh; }
+};
+
+
+enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kHalf, kClear, kPasses };
+
+
+// One launch of a batched pass sequence: pass kind, order argument, and the
+// segment of the plan's job and block-prefix arrays it uses.
+struct Level {
+  Pass pass;
+  int order;  // SD: order computed; II: order read; others unused
+  int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew;
+};
+
+// The launch sequence of several models' Init (see plan_levels).
+struct LevelPlan {
+  std::vector<atmo::Job> jobs;
+  std::vector<int> first_block;
+  std::vector<Level> levels;
+};
+
+}  // namespace atmo_rt
+
+// A model: parameters, LUTs, temporaries, job tables.
+
+struct atmo_model {
+  using Spec = atmo_rt::Spec;
+  using Sizes = atmo_rt::Sizes;
+  static constexpr int kPasses = atmo_rt::kPasses;
+  Spec spec;
+  Sizes sizes;
+  int device = 0;
+  bool retain = false;
+  bool initialized = false;
+  bool external_temporaries = false;  // temporaries carved from caller memory
+  float4 *T = nullptr, *S = nullptr, *SM = nullptr, *E = nullptr;
+  float4 *dE = nullptr, *dR = nullptr, *dM = nullptr, *dJ = nullptr;
+  // FP16 storage: resident half LUTs; S and SM above are then FP32
+  // accumulators that live only during Init, like the deltas.
+  uint2 *S16 = nullptr, *SM16 = nullptr;
+  float* d_scales = nullptr;
+  unsigned* d_maxbits = nullptr;
+  // Device job table: [iterations..., rgb transmittance job, query job].
+  std::vector<atmo::Job> jobs;
+  atmo::Job* d_jobs = nullptr;
+  // Per-pass block ranges of a single job: {0, blocks(pass)} for each pass.
+  int* d_first_block = nullptr;
+  int first_block_host[2 * kPasses] = {};
+  float solar_radiance[3] = {};
+  double sky_k[3] = {}, sun_k[3] = {};
+  // Launch timing (ATMO_OPTION_PASS_TIMING).
+  bool timing = false;
+  struct Timed {
+    int pass, order;
+    hipEvent_t start, stop;
+  };
+  std::vector<Timed> timed;   // events of the last Init, in launch order
+  std::vector<hipEvent_t> event_pool;
+};
+
+
+#define ATMO_TRY(x)                 \
+  do {                              \
+    atmo_status st_ = (x);          \
+    if (st_ != ATMO_OK) return st_; \
+  } while (0)
+
+namespace atmo_rt {
+
+atmo_status fail(atmo_status s, const std::string& msg);
+atmo_status check_launch(hipError_t e, const char* what);
+atmo_status validate_params(const atmo_model_params* p);
+atmo_status validate_sizes(const atmo_lut_sizes& z);
+int num_iterations(const Spec& s);
+int pass_blocks(Pass p, const atmo::Job& j);
+// Rebuild a model's job table for 'orders' and copy it to the device on
+// 'stream' (asynchronous with respect to other streams).
+atmo_status upload_jobs(atmo_model* m, unsigned orders, hipStream_t stream);
+// Bytes of Init temporaries of a model, and carving them out of caller-owned
+// memory (the model then does not free them).
+size_t temporaries_bytes(const atmo_model* m);
+void use_external_temporaries(atmo_model* m, char* base);
+void drop_external_temporaries(atmo_model* m);
+// Plan the levels of the models' Init (their job tables must be current).
+void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
+                 LevelPlan* plan);
+// Launch a plan whose jobs and block prefixes are at d_jobs / d_first_block.
+atmo_status enqueue_levels(const LevelPlan& plan, const atmo::Job* d_jobs,
+                           const int* d_first_block, hipStream_t stream);
+
+}  // namespace atmo_rt
+
+#endif  // ATMO_RUNTIME_INTERNAL_H_

@@ -87,6 +87,22 @@ def test_null_arguments():
     lib.atmo_model_destroy(None)  # no-op
 
 
+def test_server_null_arguments():
+    # Argument checks run before any device call: no GPU needed.
+    lib = _capi.lib()
+    rid = C.c_uint64()
+    n = C.c_int()
+    assert lib.atmo_server_create(None, None) == 1
+    assert lib.atmo_server_submit_precompute(None, None, None, 4, C.byref(rid)) == 1
+    assert lib.atmo_server_submit_lookup(None, 1, 0, 0, 1, None, None, None, None, None,
+                                         None, C.byref(rid)) == 1
+    assert lib.atmo_server_poll(None, None, 0, C.byref(n), 0) == 1
+    assert lib.atmo_server_release(None, 1) == 1
+    assert lib.atmo_server_model(None, 1, None) == 1
+    assert lib.atmo_server_get_stats(None, None) == 1
+    lib.atmo_server_destroy(None)  # no-op
+
+
 ATMO_FIELDS = ("solar_irradiance", "rayleigh_scattering", "mie_scattering",
                "mie_extinction", "absorption_extinction", "ground_albedo")
 
