@@ -187,8 +187,283 @@ def cpu_query_baseline(cfg, gpu_luts, queries, threads, n=20000):
     return 3.0 * n / dt
 
 
+def _json_line(result):
+    print(json.dumps(result), flush=True)
+
+
+def run_batched(args):
+    """64 seeded random atmospheres (radii, scale heights, coefficients,
+    albedo, per-dimension LUT size factors in {0.5, 1, 2}, 2..8 orders, 3
+    wavelengths, FP32): the plain one-at-a-time Init against one batched
+    submission (every pass level of all models in one launch, replayed from
+    a HIP graph)."""
+    import numpy as np
+    import precomputed_atmospheric_scattering_amd as atmo
+    from precomputed_atmospheric_scattering_amd import _capi, configs
+
+    cfgs = configs.batch(args.seed, 64)
+    kws = [configs.model_kwargs(c) for c in cfgs]
+    models = [atmo.Model(**kw) for kw, _ in kws]
+    orders = [o for _, o in kws]
+    n_s_total = sum(texel_counts(kw["lut_sizes"])[1] for kw, _ in kws)
+    stream = atmo.Stream()
+
+    # Plain: Model.init one model after the other, as a caller of the
+    # reference's API would (temporaries allocated and freed by each Init).
+    models[0].init(orders[0], stream)
+    t0 = time.perf_counter()
+    for m, o in zip(models, orders):
+        m.init(o, stream)
+    stream.synchronize()
+    plain_s = time.perf_counter() - t0
+    sample = (0, 21, 42, 63)
+    ref = {i: models[i].lut("scattering") for i in sample}
+    # Plain with the temporaries kept between Inits (no allocation in Init).
+    for m in models:
+        _capi.check(atmo.lib().atmo_model_set_option(
+            m.handle, _capi.OPTION_RETAIN_INTERMEDIATES, 1), "set_option")
+    for m, o in zip(models, orders):
+        m.init(o, stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for m, o in zip(models, orders):
+        m.init(o, stream)
+    stream.synchronize()
+    plain_kept_s = time.perf_counter() - t0
+
+    def timed_batch(use_graph, steps, warmup):
+        b = atmo.Batch(models, orders, use_graph=use_graph)
+        for _ in range(warmup):
+            b.launch(stream)
+        stream.synchronize()
+        ms = []
+        t_all = time.perf_counter()
+        for _ in range(steps):
+            t = time.perf_counter()
+            b.launch(stream)
+            stream.synchronize()
+            ms.append((time.perf_counter() - t) * 1e3)
+        total = time.perf_counter() - t_all
+        launches = b.num_launches
+        b.close()
+        return ms, total, launches
+
+    eager_ms, eager_s, _ = timed_batch(False, max(1, args.steps // 2), 1)
+    graph_ms, graph_s, launches = timed_batch(True, args.steps, max(1, args.warmup))
+    exact = all(np.array_equal(models[i].lut("scattering"), ref[i]) for i in sample)
+    value = 64 * args.steps / graph_s
+    plain = 64 / plain_s
+    _json_line({
+        "metric": "batched LUT precompute throughput (64 random atmospheres)",
+        "value": round(value, 2), "unit": "atmospheres/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(graph_s * 1e3 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 random atmospheres from seed %d: bottom radius "
+                               "3000-7000 km, thickness 40-200 km, Rayleigh scale height "
+                               "5-15 km and beta x0.3-3, Mie scale height 0.5-3 km, g "
+                               "0.6-0.9, albedo 0-0.5, LUT sizes x{0.5,1,2} per dimension, "
+                               "2-8 orders, 3 wavelengths, FP32" % args.seed,
+                   "atmospheres": 64, "scattering_texels_total": n_s_total,
+                   "orders": orders},
+        "batch_p50_ms": round(percentile(graph_ms, 0.5), 3),
+        "batch_p99_ms": round(percentile(graph_ms, 0.99), 3),
+        "launches_per_batch": launches,
+        "plain_atmospheres_per_s": round(plain, 2),
+        "plain_kept_temporaries_atmospheres_per_s": round(64 / plain_kept_s, 2),
+        "batched_eager_atmospheres_per_s": round(64 * len(eager_ms) / eager_s, 2),
+        "speedup_vs_plain": round(value / plain, 2),
+        "speedup_vs_plain_kept_temporaries": round(value * plain_kept_s / 64, 2),
+        "bit_exact_vs_plain": {"models_checked": list(sample), "equal": exact},
+    })
+    for m in models:
+        m.close()
+
+
+def run_serving(args):
+    """Serving loop under a Poisson stream of precompute requests (random
+    atmospheres of the batched workload) at --rate per second for --duration
+    seconds. Each finished model then serves one 1920x1080 query batch
+    (GetSkyRadiance, GetSkyRadianceToPoint, GetSunAndSkyIrradiance; inputs
+    made on the device from a seed: camera altitude 0-100 km, view direction
+    uniform, sun zenith 0-100 deg, hit distance 0-50 km) and is released."""
+    import precomputed_atmospheric_scattering_amd as atmo
+    from precomputed_atmospheric_scattering_amd import _capi, configs
+
+    rng = configs.SplitMix64(args.seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += -math.log(1.0 - rng.uniform()) / args.rate
+        if t > args.duration:
+            break
+        arrivals.append((t, configs.random_atmosphere(rng)))
+    n = WIDTH * HEIGHT
+    pool = []
+    for _ in range(4):
+        q = {k: atmo.DeviceArray((n,) if k == "shadow_length" else (n, 3))
+             for k in atmo.QUERY_FIELDS}
+        pool.append((q, [atmo.DeviceArray((n, 3)) for _ in range(6)]))
+    gen = atmo.Stream()
+    srv = atmo.Server(max_inflight_batches=3, max_batch_models=16, lookup_streams=2)
+
+    # Warm-up: one request and one query batch, not counted.
+    kw, o = configs.model_kwargs(configs.random_atmosphere(configs.SplitMix64(args.seed + 1)))
+    wid = srv.submit_precompute(num_scattering_orders=o, **kw)
+    srv.wait([wid])
+    srv.release(wid)
+
+    info, ready, batches = {}, [], {}
+    pre_lat, qry_lat, batch_sizes = [], [], []
+    failed = 0
+    i = 0
+    start = time.perf_counter()
+    outstanding = 0
+    peak_resident = 0
+    while i < len(arrivals) or outstanding > 0 or ready:
+        now = time.perf_counter() - start
+        while i < len(arrivals) and arrivals[i][0] <= now:
+            kw, o = configs.model_kwargs(arrivals[i][1])
+            rid = srv.submit_precompute(num_scattering_orders=o, **kw)
+            info[rid] = kw
+            outstanding += 1
+            i += 1
+        for c in srv.poll(timeout_ms=1):
+            if c["status"] != 0:
+                failed += 1
+            if c["kind"] == _capi.REQUEST_PRECOMPUTE:
+                outstanding -= 1
+                pre_lat.append(c["end_ms"] - c["submit_ms"])
+                batch_sizes.append(c["batch_models"])
+                if c["status"] == 0:
+                    ready.append(c["request_id"])
+            else:
+                b = batches[c["model_id"]]
+                b["left"] -= 1
+                b["end"] = max(b["end"], c["end_ms"])
+                b["submit"] = min(b["submit"], c["submit_ms"])
+                if b["left"] == 0:
+                    qry_lat.append(b["end"] - b["submit"])
+                    srv.release(c["model_id"])
+                    pool.append(b["buffers"])
+                    del batches[c["model_id"]]
+                    outstanding -= 1
+        while ready and pool:
+            mid = ready.pop(0)
+            kw = info[mid]
+            q, o = pool.pop()
+            rb, rt = kw["bottom_radius"] / 1000.0, kw["top_radius"] / 1000.0
+            atmo.generate_sky_queries(mid, n, rb, rt, out=q, stream=gen)
+            gen.synchronize()
+            ids = [srv.submit_lookup(mid, "sky", n, [q["camera"], q["view_ray"],
+                                                      q["shadow_length"], q["sun_direction"]],
+                                     o[0:2]),
+                   srv.submit_lookup(mid, "sky_to_point", n,
+                                     [q["camera"], q["point"], q["shadow_length"],
+                                      q["sun_direction"]], o[2:4]),
+                   srv.submit_lookup(mid, "sun_and_sky", n,
+                                     [q["point"], q["normal"], q["sun_direction"]], o[4:6])]
+            assert len(ids) == 3
+            batches[mid] = dict(left=3, end=0.0, submit=float("inf"), buffers=(q, o))
+            outstanding += 1
+        peak_resident = max(peak_resident, srv.stats()["resident_bytes"])
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    srv.close()
+    done = len(pre_lat)
+    _json_line({
+        "metric": "served precompute latency p50/p99 and throughput (Poisson arrivals)",
+        "value": round(done / wall, 3), "unit": "atmospheres/s", "n_gpus": 1,
+        "steps": done, "warmup": 1, "ms_per_step": round(wall * 1e3 / max(done, 1), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "Poisson %.1f req/s for %.1f s of random atmospheres "
+                               "(batched-workload distribution, seed %d), one 1920x1080 "
+                               "query batch of 3 lookups per finished model"
+                               % (args.rate, args.duration, args.seed),
+                   "requests": len(arrivals), "query_rays_per_batch": n},
+        "precompute_latency_p50_ms": round(percentile(pre_lat, 0.5), 3),
+        "precompute_latency_p99_ms": round(percentile(pre_lat, 0.99), 3),
+        "query_batch_latency_p50_ms": round(percentile(qry_lat, 0.5), 3),
+        "query_batch_latency_p99_ms": round(percentile(qry_lat, 0.99), 3),
+        "sky_rays_per_s": round(3.0 * n * len(qry_lat) / wall, 1),
+        "mean_batch_models": round(sum(batch_sizes) / max(1, len(batch_sizes)), 2),
+        "max_batch_models": max(batch_sizes) if batch_sizes else 0,
+        "failed": failed, "peak_resident_bytes": peak_resident,
+        "budget_bytes": st["budget_bytes"],
+    })
+
+
+def run_high_resolution(args):
+    """High-resolution stress: scattering LUT 64x256x64x16, transmittance
+    512x128, irradiance 128x32, the spectral atmosphere with ozone (15
+    wavelengths), 8 scattering orders, FP16 storage with FP32 accumulation."""
+    import precomputed_atmospheric_scattering_amd as atmo
+    from precomputed_atmospheric_scattering_amd import configs
+
+    cfg = configs.high_resolution(half_precision=True)
+    kw, orders = configs.model_kwargs(cfg)
+    n_t, n_s, n_e = texel_counts(kw["lut_sizes"])
+    m = atmo.Model(retain_intermediates=True, pass_timing=True, **kw)
+    stream = atmo.Stream()
+    steps = max(1, min(args.steps, 5))
+    for _ in range(max(1, min(args.warmup, 2))):
+        m.init(orders, stream)
+    stream.synchronize()
+    ms, pass_ms, pass_n = [], {}, {}
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        m.init(orders, stream)
+        stream.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+        for name, order, t in m.pass_timings():
+            key = name if name not in ("scattering_density",) else (
+                "scattering_density_2" if order == 2 else "scattering_density_n")
+            pass_ms[key] = pass_ms.get(key, 0.0) + t
+            pass_n[key] = pass_n.get(key, 0) + 1
+    # Compulsory bytes per launch (FP32 float4 texels of the delta LUTs):
+    # SD order 2 reads delta Rayleigh + Mie + irradiance and writes the
+    # density; SD order >= 3 reads the multiple-scattering delta; MS reads the
+    # density and transmittance, writes its delta and updates the FP32
+    # scattering accumulator (read + write).
+    tex = 16.0
+    bytes_per = {
+        "scattering_density_2": (3 * n_s + n_e) * tex,
+        "scattering_density_n": (2 * n_s + n_e) * tex,
+        "multiple_scattering": (4 * n_s + n_t) * tex,
+    }
+    gbs = {k: round(bytes_per[k] * pass_n[k] / (pass_ms[k] * 1e-3) / 1e9, 1)
+           for k in bytes_per if k in pass_ms}
+    _json_line({
+        "metric": "high-resolution LUT precompute ms/atmosphere",
+        "value": round(sum(ms) / steps, 3), "unit": "ms/atmosphere", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(sum(ms) / steps, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "spectral atmosphere with ozone, 15 precomputed "
+                               "wavelengths, 8 scattering orders, FP16 LUT storage with "
+                               "FP32 accumulation, LUTs 512x128 / 64x256x64x16 / 128x32",
+                   "scattering_texels": n_s, "storage": "f16"},
+        "p50_ms": round(percentile(ms, 0.5), 3), "p99_ms": round(percentile(ms, 0.99), 3),
+        "pass_ms_per_atmosphere": {k: round(v / steps, 3) for k, v in pass_ms.items()},
+        "compulsory_gb_per_s": gbs,
+        "footprint_bytes": dict(zip(("resident", "scratch"), m.footprint())),
+    })
+    m.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--workload", default="spectral",
+                    choices=("spectral", "batched", "serving", "high-resolution"),
+                    help="spectral (default): the headline line; batched: 64 seeded "
+                         "random atmospheres, batched vs one at a time; serving: Poisson "
+                         "precompute requests + query batches through the serving loop; "
+                         "high-resolution: 64x256x64x16 LUT, FP16, 8 orders")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--duration", type=float, default=10.0,
+                    help="serving: seconds of request arrivals")
+    ap.add_argument("--rate", type=float, default=20.0,
+                    help="serving: precompute requests per second (Poisson)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -217,6 +492,13 @@ def main():
         atmo.synchronize()
         src.free()
         dst.free()
+
+    if args.workload == "batched":
+        return run_batched(args)
+    if args.workload == "serving":
+        return run_serving(args)
+    if args.workload == "high-resolution":
+        return run_high_resolution(args)
 
     cfg = configs.spectral(half_precision=False)
     kw, orders = configs.model_kwargs(cfg)

@@ -378,6 +378,24 @@ atmo_status atmo_server_get_stats(atmo_server* server,
 
 const char* atmo_last_error(void);
 
+/* Seeded synthetic query batch on the device (no counterpart in the
+ * reference; the input generator of the lookup benchmarks and serving
+ * workload). Ray i uses outputs 6i+1 .. 6i+6 of the SplitMix64 sequence of
+ * 'seed': camera at altitude U[0, max_altitude] above (0, 0, bottom_radius),
+ * view ray uniform on the sphere, sun zenith U[0, max_sun_zenith_angle] and
+ * azimuth U[0, 2 pi), point = camera + U[0, max_distance] * view ray moved
+ * radially to within 1e-3 of the shell [bottom_radius, top_radius],
+ * normal = point / |point|, shadow length 0. Lengths in the model's unit;
+ * arrays n x 3 (shadow_length n), device memory. */
+atmo_status atmo_generate_sky_queries(uint64_t seed, int64_t n,
+                                      double bottom_radius, double top_radius,
+                                      double max_altitude, double max_distance,
+                                      double max_sun_zenith_angle,
+                                      float* camera, float* view_ray,
+                                      float* sun_direction, float* point,
+                                      float* normal, float* shadow_length,
+                                      void* stream);
+
 /* Minimal device-memory and stream helpers, so that callers without their
  * own HIP runtime (ctypes, cgo) can drive the lookups. */
 atmo_status atmo_malloc(void** ptr, size_t bytes);

@@ -565,6 +565,31 @@ class Server:
             pass
 
 
+QUERY_FIELDS = ("camera", "view_ray", "sun_direction", "point", "normal", "shadow_length")
+
+
+def generate_sky_queries(seed, n, bottom_radius, top_radius, max_altitude=100.0,
+                         max_distance=50.0, max_sun_zenith=None, out=None, stream=None):
+    """Seeded synthetic query batch made on the device
+    (atmo_generate_sky_queries), as {field: DeviceArray}; lengths in the
+    model's unit. Defaults: altitude U[0, 100], hit distance U[0, 50], sun
+    zenith U[0, 100 deg]. out: existing {field: DeviceArray} to fill."""
+    import math
+    if max_sun_zenith is None:
+        max_sun_zenith = math.radians(100.0)
+    if out is None:
+        out = {k: DeviceArray((n,) if k == "shadow_length" else (n, 3))
+               for k in QUERY_FIELDS}
+    for k in QUERY_FIELDS:
+        if out[k].nbytes < 4 * n * (1 if k == "shadow_length" else 3):
+            raise ValueError("output array %s too small" % k)
+    check(lib().atmo_generate_sky_queries(
+        int(seed), int(n), float(bottom_radius), float(top_radius), float(max_altitude),
+        float(max_distance), float(max_sun_zenith), *[out[k].ptr for k in QUERY_FIELDS],
+        _stream_handle(stream)), "generate_sky_queries")
+    return out
+
+
 def init_batch(models, orders, stream=None):
     n = len(models)
     hs = (C.c_void_p * n)(*[m.handle for m in models])

@@ -156,6 +156,8 @@ SIGNATURES = {
     "atmo_server_release": (ST, [VP, C.c_uint64]),
     "atmo_server_model": (ST, [VP, C.c_uint64, C.POINTER(VP)]),
     "atmo_server_get_stats": (ST, [VP, C.POINTER(ServerStats)]),
+    "atmo_generate_sky_queries": (ST, [C.c_uint64, I64, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, VP, VP, VP, VP, VP, VP, VP]),
     "atmo_last_error": (C.c_char_p, []),
     "atmo_malloc": (ST, [C.POINTER(VP), SZ]),
     "atmo_free": (ST, [VP]),

@@ -193,3 +193,43 @@ def sky_queries(seed, n, bottom_radius_units, top_radius_units):
     f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
     return dict(camera=f(cam), view_ray=f(v), sun_direction=f(sun), point=f(pt),
                 normal=f(normal), shadow_length=np.zeros(n, np.float32))
+
+
+def splitmix_uniforms(seed, first, count):
+    """Outputs first .. first+count-1 (1-based) of SplitMix64(seed) as uniform
+    doubles in [0, 1): the n-th output is the finalizer of seed + n * gamma."""
+    n = np.arange(first, first + count, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & MASK64) + n * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z ^= z >> np.uint64(31)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def hashed_sky_queries(seed, n, bottom_radius_units, top_radius_units, max_altitude=100.0,
+                       max_distance=50.0, max_sun_zenith=math.radians(100.0)):
+    """The query batch that atmo_generate_sky_queries makes on the device,
+    computed on the host in float64 (for checking the generator and for host
+    use): ray i takes SplitMix64 outputs 6i+1 .. 6i+6 as altitude, view z,
+    view azimuth, sun zenith, sun azimuth, hit distance."""
+    u = splitmix_uniforms(seed, 1, 6 * n).reshape(n, 6)
+    alt = max_altitude * u[:, 0]
+    vz = 2.0 * u[:, 1] - 1.0
+    vphi = 2.0 * math.pi * u[:, 2]
+    zen = max_sun_zenith * u[:, 3]
+    az = 2.0 * math.pi * u[:, 4]
+    dist = max_distance * u[:, 5]
+    vs = np.sqrt(np.maximum(0.0, 1.0 - vz * vz))
+    v = np.stack([vs * np.cos(vphi), vs * np.sin(vphi), vz], 1)
+    cam = np.zeros((n, 3))
+    cam[:, 2] = bottom_radius_units + alt
+    pt = cam + v * dist[:, None]
+    r = np.linalg.norm(pt, axis=1)
+    lo, hi = bottom_radius_units + 1e-3, top_radius_units - 1e-3
+    pt *= (np.clip(r, lo, hi) / r)[:, None]
+    normal = pt / np.linalg.norm(pt, axis=1, keepdims=True)
+    sun = np.stack([np.sin(zen) * np.cos(az), np.sin(zen) * np.sin(az), np.cos(zen)], 1)
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    return dict(camera=f(cam), view_ray=f(v), sun_direction=f(sun), point=f(pt),
+                normal=f(normal), shadow_length=np.zeros(n, np.float32))

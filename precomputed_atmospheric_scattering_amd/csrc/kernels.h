@@ -17,9 +17,21 @@ struct PassLaunch {
   int total_blocks;
   int max_nnu;  // largest nu size among the jobs (LDS sizing)
   int max_ew;   // largest irradiance width among the jobs (LDS sizing)
+  int max_w;    // largest packed row (nu x mu_s texels) among the jobs
 };
 
 constexpr int kBlock = 256;
+
+// Multiple scattering: packed rows of at most kMsMaxRow texels are processed
+// through LDS, ms_rows_per_block(W) rows per block (0: gather kernel path).
+constexpr int kMsRows = 4;
+constexpr int kMsMaxRow = 4 * kBlock;
+ATMO_HD int ms_rows_per_block(int w) {
+  if (w > kMsMaxRow) return 0;
+  if (w >= kBlock) return 1;
+  const int r = kBlock / w;
+  return r < kMsRows ? r : kMsRows;
+}
 
 // Blocks one job needs for each pass.
 int blocks_transmittance(const Job& j);
@@ -62,6 +74,14 @@ hipError_t launch_sky_radiance_to_point(const Job* job, int64_t n,
                                         const float* sun_direction,
                                         float* radiance, float* transmittance,
                                         int luminance, hipStream_t s);
+// Seeded synthetic query batch (camera, view ray, sun direction, point,
+// normal: n x 3; shadow length: n).
+hipError_t launch_generate_queries(uint64_t seed, int64_t n, double bottom_radius,
+                                  double top_radius, double max_altitude,
+                                  double max_distance, double max_sun_zenith,
+                                  float* camera, float* view_ray, float* sun_direction,
+                                  float* point, float* normal, float* shadow_length,
+                                  hipStream_t s);
 hipError_t launch_sun_and_sky_irradiance(const Job* job, int64_t n,
                                          const float* point,
                                          const float* normal,

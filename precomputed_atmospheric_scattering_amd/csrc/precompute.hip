@@ -684,11 +684,206 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
 
 // ---------------------------------------------------------------------------
 // Multiple scattering (functions.glsl:1285-1330, :1369-1383) and the packing
-// of atmosphere/model.cc:192-208. Same row blocking as single scattering:
-// the LUT (r, mu) taps, their weights and the view transmittance of the 51
-// samples are per block; per texel and sample, the mu_s coordinate of
-// (r mu_s + d nu) / r_i and a 16-tap gather of the scattering density.
+// of atmosphere/model.cc:192-208.
+//
+// Along one packed row (r layer, mu row) of the output only mu_s and nu
+// change, so the ray-march samples' radius r_i, cosine mu_i - hence the LUT
+// (r, mu) taps and weights - and the view transmittance are the same for the
+// whole row and are computed once per row (FP64). The 16-tap lookup of the
+// scattering density at sample i is then bilinear in (r, mu) with weights
+// shared by the row, followed by bilinear in (mu_s, nu) per texel. The
+// kernel below blends, for each sample, the four (r, mu) source rows into
+// one row in LDS with coalesced loads (every texel of a row reads the same
+// four rows), and each texel then reads its 4 (mu_s, nu) taps from LDS:
+// 4 coalesced global loads per texel and sample instead of 16 gathers.
+// Rows shorter than 256 texels are packed up to kMsRows per block; longer
+// rows (up to kMsMaxRow texels) give each thread up to kTpt texels. Rows
+// longer than that take the per-texel 16-tap gather of
+// k_multiple_scattering_gather. Both kernels are launched over the same
+// blocks and each returns at once on the other's jobs, so the path of a job
+// depends on its own row length only.
 // ---------------------------------------------------------------------------
+struct MsRowSample {
+  double d;      // distance along the view ray
+  double inv_r;  // 1 / r_i
+  int o00, o01, o10, o11;  // texel offsets of the (r, mu) source rows
+  float w00, w01, w10, w11;
+  float f[3];    // trapezoid weight * dx * view transmittance
+  float pad;
+};
+
+template <int kTpt>
+__global__ __launch_bounds__(kBlock) void k_multiple_scattering(
+    const Job* __restrict__ jobs, const int* __restrict__ first_block,
+    int njobs) {
+  __shared__ MsRowSample smp[kMsRows][kMarch];
+  __shared__ float4 buf[2][kBlock * kTpt];
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int lb = blockIdx.x - first_block[ji];
+  const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
+  const int W = nnu * nmus;
+  const int rpb = ms_rows_per_block(W);
+  // Gather-path job (uniform per block). The launcher picks kTpt so that
+  // every LDS-path row fits; the second test only keeps LDS indexing in
+  // bounds should it not.
+  if (rpb == 0 || W > kBlock * kTpt) return;
+  const int nrows = J.nr * nmu;
+  const int row0 = lb * rpb;
+  const int tid = threadIdx.x;
+
+  // Per-row sample geometry: thread t < rpb * kMarch sets up sample
+  // t % kMarch of row slot t / kMarch.
+  for (int e = tid; e < rpb * kMarch; e += kBlock) {
+    const int slot = e / kMarch, i = e % kMarch;
+    const int row = row0 + slot;
+    if (row >= nrows) continue;
+    const int y = row % nmu, z = row / nmu;
+    double r, rho, mu;
+    bool ground;
+    s_layer_r(J, z, &r, &rho);
+    s_row_mu(J, y, r, rho, &mu, &ground);
+    const double dx = dist_nearest(J, r, mu, ground) / (kMarch - 1);
+    const double d = i * dx;
+    const double r_i = clamp_radius(J, sqrt(d * d + 2.0 * r * mu * d + r * r));
+    const double mu_i = clamp_cos((r * mu + d) / r_i);
+    const Taps tz = axis_taps(s_r_texel(J, r_i), J.nr);
+    const Taps ty = axis_taps(s_mu_texel(J, r_i, mu_i, ground), nmu);
+    const float3 tv = transmittance(J, J.T, r, mu, d, ground);
+    const float w = static_cast<float>(((i == 0 || i == kMarch - 1) ? 0.5 : 1.0) * dx);
+    MsRowSample& S = smp[slot][i];
+    S.d = d;
+    S.inv_r = 1.0 / r_i;
+    S.o00 = (tz.i0 * nmu + ty.i0) * W;
+    S.o01 = (tz.i0 * nmu + ty.i1) * W;
+    S.o10 = (tz.i1 * nmu + ty.i0) * W;
+    S.o11 = (tz.i1 * nmu + ty.i1) * W;
+    const float wz1 = tz.w, wz0 = 1.0f - tz.w, wy1 = ty.w, wy0 = 1.0f - ty.w;
+    S.w00 = wz0 * wy0;
+    S.w01 = wz0 * wy1;
+    S.w10 = wz1 * wy0;
+    S.w11 = wz1 * wy1;
+    S.f[0] = w * tv.x;
+    S.f[1] = w * tv.y;
+    S.f[2] = w * tv.z;
+  }
+
+  // Per-texel constants: the texel's nu taps (fixed for all samples,
+  // functions.glsl:967-973) and r mu_s.
+  int slot_of[kTpt], x_of[kTpt], c0_of[kTpt], c1_of[kTpt];
+  float ws_of[kTpt];
+  double nu_of[kTpt], rmus_of[kTpt];
+  float acc[kTpt][3];
+#pragma unroll
+  for (int j = 0; j < kTpt; ++j) {
+    const int f = tid + j * kBlock;
+    const int slot = f / W;
+    slot_of[j] = (slot < rpb && row0 + slot < nrows) ? slot : -1;
+    x_of[j] = f - slot * W;
+    acc[j][0] = acc[j][1] = acc[j][2] = 0.0f;
+    c0_of[j] = c1_of[j] = 0;
+    ws_of[j] = 0.0f;
+    nu_of[j] = rmus_of[j] = 0.0;
+    if (slot_of[j] < 0) continue;
+    const int row = row0 + slot;
+    double r, rho, mu;
+    bool ground;
+    s_layer_r(J, row / nmu, &r, &rho);
+    s_row_mu(J, row % nmu, r, rho, &mu, &ground);
+    const int x = x_of[j];
+    const double mu_s = s_col_mus(J, x % nmus);
+    const double nu = s_slice_nu(J, x / nmus, mu, mu_s);
+    double tcx = (nu + 1.0) / 2.0 * (nnu - 1);
+    if (!(tcx >= -1.0)) tcx = -1.0;
+    if (tcx > nnu) tcx = nnu;
+    const double fl = floor(tcx);
+    int s0 = static_cast<int>(fl);
+    int s1 = s0 + 1;
+    s0 = s0 < 0 ? 0 : (s0 > nnu - 1 ? nnu - 1 : s0);
+    s1 = s1 < 0 ? 0 : (s1 > nnu - 1 ? nnu - 1 : s1);
+    ws_of[j] = static_cast<float>(tcx - fl);
+    c0_of[j] = slot * W + s0 * nmus;
+    c1_of[j] = slot * W + s1 * nmus;
+    nu_of[j] = nu;
+    rmus_of[j] = r * mu_s;
+  }
+  __syncthreads();
+
+  // mu_s mapping constants (functions.glsl:814-827).
+  const float rbf = static_cast<float>(J.rb);
+  const float H2 = static_cast<float>(J.H * J.H);
+  const float dmin0 = static_cast<float>(J.rt - J.rb);
+  const float inv_range0 = static_cast<float>(1.0 / (J.H - (J.rt - J.rb)));
+  const float invA = static_cast<float>(1.0 / J.A);
+  const float xmax = static_cast<float>(nmus - 1);
+  const float4* __restrict__ D = J.dJ;
+  for (int i = 0; i < kMarch; ++i) {
+    float4* B = buf[i & 1];
+    // Blend the four (r, mu) source rows of sample i into LDS.
+#pragma unroll
+    for (int j = 0; j < kTpt; ++j) {
+      if (slot_of[j] < 0) continue;
+      const MsRowSample& S = smp[slot_of[j]][i];
+      const int x = x_of[j];
+      const float4 a = D[S.o00 + x], b = D[S.o01 + x];
+      const float4 c = D[S.o10 + x], e = D[S.o11 + x];
+      float4 v;
+      v.x = fmaf(S.w11, e.x, fmaf(S.w10, c.x, fmaf(S.w01, b.x, S.w00 * a.x)));
+      v.y = fmaf(S.w11, e.y, fmaf(S.w10, c.y, fmaf(S.w01, b.y, S.w00 * a.y)));
+      v.z = fmaf(S.w11, e.z, fmaf(S.w10, c.z, fmaf(S.w01, b.z, S.w00 * a.z)));
+      v.w = 0.0f;
+      B[tid + j * kBlock] = v;
+    }
+    // One barrier per sample: buffer i & 1 is written again at sample i + 2,
+    // after every thread has passed the barrier of sample i + 1 and so has
+    // finished reading it.
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kTpt; ++j) {
+      if (slot_of[j] < 0) continue;
+      const MsRowSample& S = smp[slot_of[j]][i];
+      const float mus =
+          clampf(static_cast<float>(fma(S.d, nu_of[j], rmus_of[j]) * S.inv_r), -1.0f, 1.0f);
+      const float tt = rbf * mus;
+      const float sq = sqrt_f(fmaf(tt, tt, H2));
+      const float dd = tt > 0.0f ? H2 * rcp_f(sq + tt) : sq - tt;
+      const float aa = (dd - dmin0) * inv_range0;
+      const float xm = fmaxf(fmaf(-aa, invA, 1.0f), 0.0f) * rcp_f(1.0f + aa);
+      const float xin = clamp_index_coord(xm * xmax, xmax);
+      const float fx = floorf(xin);
+      const float wx = xin - fx;
+      const int i0 = static_cast<int>(fx);
+      const int i1 = min(i0 + 1, nmus - 1);
+      const float4 p = B[c0_of[j] + i0], q = B[c0_of[j] + i1];
+      const float4 u = B[c1_of[j] + i0], v = B[c1_of[j] + i1];
+      const float ws = ws_of[j];
+      const float lo0 = fmaf(wx, q.x - p.x, p.x), hi0 = fmaf(wx, v.x - u.x, u.x);
+      const float lo1 = fmaf(wx, q.y - p.y, p.y), hi1 = fmaf(wx, v.y - u.y, u.y);
+      const float lo2 = fmaf(wx, q.z - p.z, p.z), hi2 = fmaf(wx, v.z - u.z, u.z);
+      acc[j][0] = fmaf(fmaf(ws, hi0 - lo0, lo0), S.f[0], acc[j][0]);
+      acc[j][1] = fmaf(fmaf(ws, hi1 - lo1, lo1), S.f[1], acc[j][1]);
+      acc[j][2] = fmaf(fmaf(ws, hi2 - lo2, lo2), S.f[2], acc[j][2]);
+    }
+  }
+
+#pragma unroll
+  for (int j = 0; j < kTpt; ++j) {
+    if (slot_of[j] < 0) continue;
+    const int64_t t_idx = static_cast<int64_t>(row0 + slot_of[j]) * W + x_of[j];
+    J.dR[t_idx] = make_float4(acc[j][0], acc[j][1], acc[j][2], 0.0f);
+    const float3 l = apply_lum(J.lum, make_float3(acc[j][0], acc[j][1], acc[j][2]));
+    const double nu = nu_of[j];
+    const float inv_pr = static_cast<float>(1.0 / ((3.0 / (16.0 * kPi)) * (1.0 + nu * nu)));
+    float4 sv = J.S[t_idx];
+    sv.x += l.x * inv_pr;
+    sv.y += l.y * inv_pr;
+    sv.z += l.z * inv_pr;
+    J.S[t_idx] = sv;
+  }
+}
+
+// Rows longer than kMsMaxRow: blocks of 256 texels of one row; per texel
+// and sample a 16-tap gather of the scattering density.
 struct MsSample {
   double d;
   double inv_r;
@@ -697,7 +892,7 @@ struct MsSample {
   float f[3];  // trapezoid weight * dx * view transmittance
 };
 
-__global__ __launch_bounds__(kBlock) void k_multiple_scattering(
+__global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
     int njobs) {
   __shared__ MsSample smp[kMarch];
@@ -706,6 +901,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   const int lb = blockIdx.x - first_block[ji];
   const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
   const int W = nnu * nmus;
+  if (ms_rows_per_block(W) != 0) return;  // LDS-path job
   const int tiles = (W + kBlock - 1) / kBlock;
   const int tile = lb % tiles;
   const int row = lb / tiles;
@@ -910,7 +1106,13 @@ int blocks_scattering_density(const Job& j) {
   return ((plane + kBlock - 1) / kBlock) * j.nmus * j.nr;
 }
 int blocks_indirect_irradiance(const Job& j) { return j.ew * j.eh; }
-int blocks_multiple_scattering(const Job& j) { return blocks_single_scattering(j); }
+int blocks_multiple_scattering(const Job& j) {
+  const int w = j.nnu * j.nmus;
+  const int rpb = ms_rows_per_block(w);
+  const int rows = j.nmu * j.nr;
+  if (rpb == 0) return ((w + kBlock - 1) / kBlock) * rows;
+  return (rows + rpb - 1) / rpb;
+}
 int blocks_half_conversion(const Job& j) {
   const int64_t n = static_cast<int64_t>(j.nr) * j.nmu * j.nmus * j.nnu;
   return static_cast<int>((n + kBlock - 1) / kBlock);
@@ -962,8 +1164,20 @@ hipError_t launch_to_half(const PassLaunch& p, hipStream_t s) {
 }
 
 hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s) {
-  hipLaunchKernelGGL(k_multiple_scattering, dim3(p.total_blocks), dim3(kBlock), 0, s,
-                     p.jobs, p.first_block, p.njobs);
+  // Texels per thread: enough for the longest row that takes the LDS path.
+  const int w = p.max_w < kMsMaxRow ? p.max_w : kMsMaxRow;
+  if (w <= kBlock)
+    hipLaunchKernelGGL(k_multiple_scattering<1>, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                       p.jobs, p.first_block, p.njobs);
+  else if (w <= 2 * kBlock)
+    hipLaunchKernelGGL(k_multiple_scattering<2>, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                       p.jobs, p.first_block, p.njobs);
+  else
+    hipLaunchKernelGGL(k_multiple_scattering<4>, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                       p.jobs, p.first_block, p.njobs);
+  if (p.max_w > kMsMaxRow)
+    hipLaunchKernelGGL(k_multiple_scattering_gather, dim3(p.total_blocks), dim3(kBlock), 0,
+                       s, p.jobs, p.first_block, p.njobs);
   return hipGetLastError();
 }
 

@@ -207,6 +207,56 @@ __global__ __launch_bounds__(kBlock) void k_sun_and_sky_irradiance(
   store3(sun_out, i, sun);
 }
 
+// Seeded synthetic query batches. Variate k of ray i is output number
+// 6 i + k + 1 of the SplitMix64 sequence started at 'seed' (Steele, Lea and
+// Flood, "Fast splittable pseudorandom number generators", OOPSLA 2014): the
+// sequence's n-th output is the finalizer applied to seed + n * golden gamma.
+__device__ __forceinline__ double splitmix_uniform(uint64_t seed, uint64_t n) {
+  uint64_t z = seed + n * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return static_cast<double>(z >> 11) * 0x1.0p-53;
+}
+
+__global__ __launch_bounds__(kBlock) void k_generate_queries(
+    uint64_t seed, int64_t n, double rb, double rt, double max_altitude,
+    double max_distance, double max_sun_zenith, float* __restrict__ camera,
+    float* __restrict__ view_ray, float* __restrict__ sun_direction,
+    float* __restrict__ point, float* __restrict__ normal,
+    float* __restrict__ shadow_length) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t base = 6ull * static_cast<uint64_t>(i) + 1ull;
+  const double alt = max_altitude * splitmix_uniform(seed, base);
+  const double vz = 2.0 * splitmix_uniform(seed, base + 1) - 1.0;
+  const double vphi = 2.0 * kPi * splitmix_uniform(seed, base + 2);
+  const double zen = max_sun_zenith * splitmix_uniform(seed, base + 3);
+  const double az = 2.0 * kPi * splitmix_uniform(seed, base + 4);
+  const double dist = max_distance * splitmix_uniform(seed, base + 5);
+  const double vs = sqrt(fmax(0.0, 1.0 - vz * vz));
+  const double v[3] = {vs * cos(vphi), vs * sin(vphi), vz};
+  const double cz = rb + alt;
+  double p[3] = {v[0] * dist, v[1] * dist, cz + v[2] * dist};
+  const double pr = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  // Points are kept 1e-3 length units inside the atmosphere's shell.
+  const double lo = rb + 1e-3, hi = rt - 1e-3;
+  const double k = (pr < lo ? lo : (pr > hi ? hi : pr)) / pr;
+  p[0] *= k;
+  p[1] *= k;
+  p[2] *= k;
+  const double pn = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  const double sun[3] = {sin(zen) * cos(az), sin(zen) * sin(az), cos(zen)};
+  for (int c = 0; c < 3; ++c) {
+    camera[3 * i + c] = c == 2 ? static_cast<float>(cz) : 0.0f;
+    view_ray[3 * i + c] = static_cast<float>(v[c]);
+    sun_direction[3 * i + c] = static_cast<float>(sun[c]);
+    point[3 * i + c] = static_cast<float>(p[c]);
+    normal[3 * i + c] = static_cast<float>(p[c] / pn);
+  }
+  shadow_length[i] = 0.0f;
+}
+
 inline unsigned grid_for(int64_t n) {
   return static_cast<unsigned>((n + kBlock - 1) / kBlock);
 }
@@ -250,6 +300,20 @@ hipError_t launch_sun_and_sky_irradiance(const Job* job, int64_t n,
   hipLaunchKernelGGL(k_sun_and_sky_irradiance, dim3(grid_for(n)), dim3(kBlock), 0, s,
                      job, n, point, normal, sun_direction, sun_irradiance,
                      sky_irradiance, luminance);
+  return hipGetLastError();
+}
+
+hipError_t launch_generate_queries(uint64_t seed, int64_t n, double bottom_radius,
+                                  double top_radius, double max_altitude,
+                                  double max_distance, double max_sun_zenith,
+                                  float* camera, float* view_ray, float* sun_direction,
+                                  float* point, float* normal, float* shadow_length,
+                                  hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_generate_queries, dim3(grid_for(n)), dim3(kBlock), 0, s, seed, n,
+                     bottom_radius, top_radius, max_altitude, max_distance,
+                     max_sun_zenith, camera, view_ray, sun_direction, point, normal,
+                     shadow_length);
   return hipGetLastError();
 }
 

@@ -319,6 +319,7 @@ atmo::PassLaunch single_launch(const atmo_model* m, int job_index, Pass p) {
   l.total_blocks = m->first_block_host[2 * p + 1];
   l.max_nnu = m->sizes.nnu;
   l.max_ew = m->sizes.ew;
+  l.max_w = m->sizes.nnu * m->sizes.nmus;
   return l;
 }
 
@@ -944,6 +945,28 @@ atmo_status atmo_convert_spectrum_to_linear_srgb(int n, const double* wl,
   return ATMO_OK;
 }
 
+atmo_status atmo_generate_sky_queries(uint64_t seed, int64_t n, double bottom_radius,
+                                      double top_radius, double max_altitude,
+                                      double max_distance, double max_sun_zenith_angle,
+                                      float* camera, float* view_ray, float* sun_direction,
+                                      float* point, float* normal, float* shadow_length,
+                                      void* stream) {
+  if (n < 0) return fail(ATMO_ERROR_INVALID_ARGUMENT, "n < 0");
+  if (n > 0 && (camera == nullptr || view_ray == nullptr || sun_direction == nullptr ||
+                point == nullptr || normal == nullptr || shadow_length == nullptr))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "NULL output array");
+  if (!(bottom_radius > 0.0 && top_radius > bottom_radius + 2e-3))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "need 0 < bottom_radius < top_radius");
+  if (!(max_altitude >= 0.0 && max_distance >= 0.0 && max_sun_zenith_angle >= 0.0))
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "negative range");
+  return check_launch(
+      atmo::launch_generate_queries(seed, n, bottom_radius, top_radius, max_altitude,
+                                    max_distance, max_sun_zenith_angle, camera, view_ray,
+                                    sun_direction, point, normal, shadow_length,
+                                    static_cast<hipStream_t>(stream)),
+      "generate sky queries");
+}
+
 atmo_status atmo_malloc(void** ptr, size_t bytes) {
   if (ptr == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "ptr is NULL");
   ATMO_HIP(hipMalloc(ptr, bytes));
@@ -1041,6 +1064,7 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
     l.fb_offset = static_cast<int>(plan->first_block.size());
     l.max_nnu = 0;
     l.max_ew = 0;
+    l.max_w = 0;
     int total = 0;
     for (const auto& mj : sel) {
       const atmo::Job& j = models[mj.first]->jobs[mj.second];
@@ -1049,6 +1073,7 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
       total += pass_blocks(p, j);
       l.max_nnu = std::max(l.max_nnu, j.nnu);
       l.max_ew = std::max(l.max_ew, j.ew);
+      l.max_w = std::max(l.max_w, j.nnu * j.nmus);
     }
     plan->first_block.push_back(total);
     l.total_blocks = total;
@@ -1098,6 +1123,7 @@ atmo_status enqueue_levels(const LevelPlan& plan, const atmo::Job* d_jobs,
     pl.total_blocks = l.total_blocks;
     pl.max_nnu = l.max_nnu;
     pl.max_ew = l.max_ew;
+    pl.max_w = l.max_w;
     hipError_t e = hipSuccess;
     switch (l.pass) {
       case kT: e = atmo::launch_transmittance(pl, s); break;

@@ -45,7 +45,7 @@ enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kHalf, kClear, kPasses };
 struct Level {
   Pass pass;
   int order;  // SD: order computed; II: order read; others unused
-  int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew;
+  int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew, max_w;
 };
 
 // The launch sequence of several models' Init (see plan_levels).

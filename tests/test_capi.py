@@ -165,3 +165,18 @@ def test_product_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_capi, "_lib", None)
     with pytest.raises(ImportError):
         _capi.lib()
+
+
+def test_counter_based_query_stream_matches_splitmix_sequence():
+    # Ray i of the query generator takes outputs 6i+1 .. 6i+6 of the
+    # sequential SplitMix64 stream: the counter form must agree with it.
+    rng = configs.SplitMix64(77)
+    seq = [rng.uniform() for _ in range(60)]
+    assert list(configs.splitmix_uniforms(77, 1, 60)) == seq
+    q = configs.hashed_sky_queries(77, 10, 6360.0, 6420.0)
+    u = np.array(seq).reshape(10, 6)
+    assert np.allclose(q["camera"][:, 2], 6360.0 + 100.0 * u[:, 0], rtol=1e-7)
+    assert np.allclose(q["view_ray"][:, 2], 2.0 * u[:, 1] - 1.0, atol=1e-7)
+    assert np.allclose(np.linalg.norm(q["view_ray"], axis=1), 1.0, atol=1e-6)
+    r = np.linalg.norm(q["point"].astype(np.float64), axis=1)
+    assert (r >= 6360.0 - 1e-3).all() and (r <= 6420.0).all()

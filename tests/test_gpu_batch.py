@@ -62,3 +62,23 @@ def test_batch_matches_oracle():
             e = parity.rel_err(g, r, channels=4 if n == "scattering" else 3)[0]
             assert e <= parity.TOL_FP32, (n, e)
         m.close()
+
+
+def test_full_size_batch_equals_alone():
+    # The first 6 atmospheres of the 64-atmosphere batched workload (seed
+    # 2024) at their drawn LUT sizes (0.5x..2x of the reference per
+    # dimension) and orders: batched graph replay == each alone, bit for bit.
+    cfgs = configs.batch(2024, 6)
+    alone = []
+    for cfg in cfgs:
+        kw, orders = configs.model_kwargs(cfg)
+        m = atmo.Model(**kw)
+        m.init(orders)
+        alone.append({n: m.lut(n) for n in ("scattering", "irradiance")})
+        m.close()
+    models = [atmo.Model(**configs.model_kwargs(c)[0]) for c in cfgs]
+    atmo.init_batch(models, [c["orders"] for c in cfgs])
+    for m, ref in zip(models, alone):
+        for n, r in ref.items():
+            assert np.array_equal(m.lut(n), r), n
+        m.close()

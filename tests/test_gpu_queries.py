@@ -267,3 +267,14 @@ def test_combined_textures_lookups():
                                       ol.ptr(f64(q["sun_direction"])), ol.ptr(r), ol.ptr(t))
     assert err(rad, r) <= TOL and err(tr, t) <= TOL
     m.close()
+
+
+def test_query_generator_matches_host_twin():
+    n = 50000
+    dev = atmo.generate_sky_queries(31, n, 6360.0, 6420.0)
+    host = configs.hashed_sky_queries(31, n, 6360.0, 6420.0)
+    for k, v in dev.items():
+        g = v.to_host().reshape(host[k].shape)
+        # FP64 on both sides, rounded to FP32; sin/cos/sqrt may differ in the
+        # last bit of FP64 before that rounding.
+        assert np.allclose(g, host[k], rtol=2e-7, atol=2e-7), k
