@@ -16,6 +16,26 @@
 
 #define ATMO_HD __host__ __device__ inline
 
+// LUT loads and stores through the global address space. The LUT pointers
+// are read from the job table, so without this the compiler cannot tell
+// they are global and emits flat accesses, which also wait on LDS traffic.
+template <class T>
+ATMO_HD T gload(const T* p, int64_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+#else
+  return p[i];
+#endif
+}
+template <class T>
+ATMO_HD void gstore(T* p, int64_t i, const T& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  ((__attribute__((address_space(1))) T*)p)[i] = v;
+#else
+  p[i] = v;
+#endif
+}
+
 namespace atmo {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -44,6 +64,12 @@ struct Job {
   // mu_s mapping (functions.glsl:819-821).
   double H;
   double A;
+  // Lookup constants derived from the above on the host: squared radii and
+  // FP32 copies for the lookups' FP32 stages (1/H; mu_s mapping rb, H^2,
+  // rt - rb, 1 / (H - (rt - rb)), 1/A; phase-function constants).
+  double rb2, rt2, H2;
+  float q_inv_H, q_rb, q_H2, q_dmin, q_inv_range, q_inv_A;
+  float q_kR, q_kM, q_g1, q_mg2;
   // luminance_from_radiance, row-major (atmosphere/model.cc:909, 939-943).
   float lum[9];
   // SKY/SUN_SPECTRAL_RADIANCE_TO_LUMINANCE (atmosphere/model.cc:668-686,
@@ -198,8 +224,8 @@ ATMO_HD uint2 pack_half4(float4 v) {
 // Bilinear read of a 2D RGBA LUT at GL texel-space (x, y).
 ATMO_HD float3 bilinear(const float4* lut, int w, int h, double x, double y) {
   Taps tx = axis_taps(x, w), ty = axis_taps(y, h);
-  float4 a = lut[ty.i0 * w + tx.i0], b = lut[ty.i0 * w + tx.i1];
-  float4 c = lut[ty.i1 * w + tx.i0], d = lut[ty.i1 * w + tx.i1];
+  float4 a = gload(lut, ty.i0 * w + tx.i0), b = gload(lut, ty.i0 * w + tx.i1);
+  float4 c = gload(lut, ty.i1 * w + tx.i0), d = gload(lut, ty.i1 * w + tx.i1);
   float wx1 = tx.w, wx0 = 1.0f - tx.w, wy1 = ty.w, wy0 = 1.0f - ty.w;
   return make_float3(wy0 * (wx0 * a.x + wx1 * b.x) + wy1 * (wx0 * c.x + wx1 * d.x),
                      wy0 * (wx0 * a.y + wx1 * b.y) + wy1 * (wx0 * c.y + wx1 * d.y),
@@ -218,8 +244,8 @@ ATMO_HD float min1(float v) { return v < 1.0f ? v : 1.0f; }
 // Clamp to [0, hi]; NaN maps to 0. Used before every float -> index
 // conversion so that indices stay in bounds whatever the input.
 ATMO_HD float clamp_index_coord(float t, float hi) {
-  t = t >= 0.0f ? t : 0.0f;
-  return t <= hi ? t : hi;
+  // fmaxf returns the non-NaN operand.
+  return fminf(fmaxf(t, 0.0f), hi);
 }
 
 // functions.glsl:493-519
@@ -349,8 +375,8 @@ ATMO_HD float4 s_lookup(const Job& j, const float4* lut, const SCoords& c) {
       for (int e = 0; e < 2; ++e)
         for (int q = 0; q < 2; ++q) {
           float wt = wz[a] * wy[b] * wx[e] * wsl[q];
-          float4 v = lut[(static_cast<int64_t>(zs[a]) * j.nmu + ys[b]) * w +
-                         ss[q] * j.nmus + xs[e]];
+          float4 v = gload(lut, (static_cast<int64_t>(zs[a]) * j.nmu + ys[b]) * w +
+                                    ss[q] * j.nmus + xs[e]);
           acc.x += wt * v.x;
           acc.y += wt * v.y;
           acc.z += wt * v.z;
@@ -370,13 +396,13 @@ struct D4 {
 // inverse scales.
 struct F32Texels {
   const float4* p;
-  ATMO_HD float4 operator[](int64_t i) const { return p[i]; }
+  ATMO_HD float4 operator[](int64_t i) const { return gload(p, i); }
 };
 struct F16Texels {
   const uint2* p;
   float4 inv;
   ATMO_HD float4 operator[](int64_t i) const {
-    float4 v = unpack_half4(p[i]);
+    float4 v = unpack_half4(gload(p, i));
     return make_float4(v.x * inv.x, v.y * inv.y, v.z * inv.z, v.w * inv.w);
   }
 };
@@ -433,8 +459,8 @@ ATMO_HD D4 bilinear_d(const float4* lut, int w, int h, double x, double y) {
   if (!(y >= -1.0)) y = -1.0;
   if (y > static_cast<double>(h)) y = static_cast<double>(h);
   const double fx = x - floor(x), fy = y - floor(y);
-  const float4 a = lut[ty.i0 * w + tx.i0], b = lut[ty.i0 * w + tx.i1];
-  const float4 c = lut[ty.i1 * w + tx.i0], d = lut[ty.i1 * w + tx.i1];
+  const float4 a = gload(lut, ty.i0 * w + tx.i0), b = gload(lut, ty.i0 * w + tx.i1);
+  const float4 c = gload(lut, ty.i1 * w + tx.i0), d = gload(lut, ty.i1 * w + tx.i1);
   const double wx1 = fx, wx0 = 1.0 - fx, wy1 = fy, wy0 = 1.0 - fy;
   D4 o;
   o.x = wy0 * (wx0 * a.x + wx1 * b.x) + wy1 * (wx0 * c.x + wx1 * d.x);

@@ -13,6 +13,18 @@
 namespace atmo {
 namespace {
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin
+// by workgroup id. This permutation of a job's nb blocks gives the blocks of
+// one XCD (same id mod 8) a contiguous range of the job's work, so that the
+// blocks resident on an XCD at a time work on neighbouring rows and share
+// their L2 lines.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_contiguous(int lb, int nb) {
+  const int q = nb / kXcds, rem = nb % kXcds;
+  const int g = lb % kXcds;
+  return g * q + (g < rem ? g : rem) + lb / kXcds;
+}
+
 // Block b's job: the i with first_block[i] <= b < first_block[i + 1]. The
 // search is uniform across the block.
 __device__ __forceinline__ int find_job(const int* __restrict__ first_block,
@@ -124,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_transmittance(
     t.z = static_cast<float>(exp(-(J.rayleigh[2] * lr + J.mie_ext[2] * lm +
                                    J.absorption[2] * la)));
     t.w = 0.0f;
-    J.T[texel] = t;
+    gstore(J.T, texel, t);
   }
 }
 
@@ -147,10 +159,11 @@ __global__ __launch_bounds__(kBlock) void k_direct_irradiance(
                     : (mu_s > alpha ? mu_s
                                     : (mu_s + alpha) * (mu_s + alpha) / (4.0 * alpha));
   const float3 t = t_lookup(J, J.T, r, mu_s);
-  J.dE[texel] = make_float4(static_cast<float>(J.solar[0] * avg_cos) * t.x,
-                            static_cast<float>(J.solar[1] * avg_cos) * t.y,
-                            static_cast<float>(J.solar[2] * avg_cos) * t.z, 0.0f);
-  if (!J.blend) J.E[texel] = make_float4(0.f, 0.f, 0.f, 0.f);
+  gstore(J.dE, texel,
+         make_float4(static_cast<float>(J.solar[0] * avg_cos) * t.x,
+                     static_cast<float>(J.solar[1] * avg_cos) * t.y,
+                     static_cast<float>(J.solar[2] * avg_cos) * t.z, 0.0f));
+  if (!J.blend) gstore(J.E, texel, make_float4(0.f, 0.f, 0.f, 0.f));
 }
 
 // ---------------------------------------------------------------------------
@@ -198,8 +211,8 @@ __device__ __forceinline__ float3 tsun_lookup(const Job& J, const SsSample& S,
   const int i0 = static_cast<int>(fl);
   const int i1 = min(i0 + 1, J.tw - 1);
   const float4* __restrict__ T = J.T;
-  const float4 a = T[S.y0 * J.tw + i0], b = T[S.y0 * J.tw + i1];
-  const float4 c = T[S.y1 * J.tw + i0], d = T[S.y1 * J.tw + i1];
+  const float4 a = gload(T, S.y0 * J.tw + i0), b = gload(T, S.y0 * J.tw + i1);
+  const float4 c = gload(T, S.y1 * J.tw + i0), d = gload(T, S.y1 * J.tw + i1);
   const float wy1 = S.wy, wy0 = 1.0f - S.wy;
   float3 o;
   o.x = wy0 * fmaf(w, b.x - a.x, a.x) + wy1 * fmaf(w, d.x - c.x, c.x);
@@ -289,28 +302,28 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
                                  am1 * static_cast<float>(dx * J.solar[1] * J.mie_sca[1]),
                                  am2 * static_cast<float>(dx * J.solar[2] * J.mie_sca[2]));
   const int64_t t_idx = static_cast<int64_t>(row) * W + x;
-  J.dR[t_idx] = make_float4(ray.x, ray.y, ray.z, 0.0f);
-  J.dM[t_idx] = make_float4(mie.x, mie.y, mie.z, 0.0f);
+  gstore(J.dR, t_idx, make_float4(ray.x, ray.y, ray.z, 0.0f));
+  gstore(J.dM, t_idx, make_float4(mie.x, mie.y, mie.z, 0.0f));
   const float3 lr = apply_lum(J.lum, ray);
   const float3 lm = apply_lum(J.lum, mie);
   float4 s = make_float4(lr.x, lr.y, lr.z, lm.x);
   if (J.blend) {
-    const float4 o = J.S[t_idx];
+    const float4 o = gload(J.S, t_idx);
     s.x += o.x;
     s.y += o.y;
     s.z += o.z;
     s.w += o.w;
   }
-  J.S[t_idx] = s;
+  gstore(J.S, t_idx, s);
   if (!J.combined) {
     float4 m = make_float4(lm.x, lm.y, lm.z, 0.0f);
     if (J.blend) {
-      const float4 o = J.SM[t_idx];
+      const float4 o = gload(J.SM, t_idx);
       m.x += o.x;
       m.y += o.y;
       m.z += o.z;
     }
-    J.SM[t_idx] = m;
+    gstore(J.SM, t_idx, m);
   }
 }
 
@@ -334,7 +347,10 @@ constexpr int kTheta = 16;
 // dv = v[i + 1] - v[i] (0 for the last entry).
 __device__ __forceinline__ float3 table_lerp(const float4* __restrict__ tab,
                                              float t_max, float t) {
-  t = clamp_index_coord(t, t_max);
+  // Median of (t, 0, t_max): one instruction. The coordinates come from
+  // finite dot products of unit vectors; were t NaN, the float -> int
+  // conversion below gives 0, still in bounds.
+  t = __builtin_amdgcn_fmed3f(t, 0.0f, t_max);
   const float fl = floorf(t);
   const float w = t - fl;
   const int i = static_cast<int>(fl);
@@ -358,8 +374,16 @@ __device__ __forceinline__ void sd_theta(
     float A, float B, float C, float P, float Q, float g1, float mg2,
     const float cR[3], const float cM[3], const SdTheta& th, float& acc0,
     float& acc1, float& acc2) {
+  // nu table coordinate of the incident direction at azimuth phi:
+  // t = (nu1 + 1) / 2 * (NU - 1) with nu1 = A + B cos(phi) +- C sin(phi),
+  // as one FMA from per-angle constants; likewise the ground-irradiance
+  // table coordinate (mu_g + 1) / 2 * (E_W - 1) with mu_g = ga + gb nu1.
   const float hn = 0.5f * nu_max;
+  const float inv_hn = 1.0f / hn;
+  const float tA = fmaf(A, hn, hn), tB = B * hn, tC = C * hn;
   const float he = 0.5f * e_max;
+  const float eB = th.gb * inv_hn * he;
+  const float eA = fmaf(th.ga - th.gb, he, he);
 #pragma unroll
   for (int m = 0; m < 16; ++m) {
     const float c = kCosPhi[m], s = kSinPhi[m];
@@ -372,16 +396,17 @@ __device__ __forceinline__ void sd_theta(
     const float p0 = x2 * fmaf(cM[0], rs3, cR[0]);
     const float p1 = x2 * fmaf(cM[1], rs3, cR[1]);
     const float p2 = x2 * fmaf(cM[2], rs3, cR[2]);
-    const float base = fmaf(B, c, A);
-    float l0 = 0.f, l1 = 0.f, l2 = 0.f;
+    const float tb = fmaf(tB, c, tA);
+    float l0, l1, l2;
 #pragma unroll
     for (int sgn = 0; sgn < 2; ++sgn) {
-      const float nu1 = sgn == 0 ? fmaf(C, s, base) : fmaf(-C, s, base);
-      float3 v = table_lerp(tab_a, nu_max, fmaf(nu1, hn, hn));
+      const float t = sgn == 0 ? fmaf(tC, s, tb) : fmaf(-tC, s, tb);
+      float3 v = table_lerp(tab_a, nu_max, t);
       if (kOrder2) {
         // Order-1 incident radiance: Rayleigh and Mie single scattering
         // times their phase functions of nu1 (constants folded in tables).
-        const float3 vm = table_lerp(tab_b, nu_max, fmaf(nu1, hn, hn));
+        const float nu1 = fmaf(t, inv_hn, -1.0f);
+        const float3 vm = table_lerp(tab_b, nu_max, t);
         const float r1 = rsq_f(fmaf(mg2, nu1, g1));
         const float q = r1 * r1 * r1;
         const float x1 = fmaf(nu1, nu1, 1.0f);
@@ -390,15 +415,20 @@ __device__ __forceinline__ void sd_theta(
         v.z = x1 * fmaf(vm.z, q, v.z);
       }
       if (kGround) {
-        const float mug = fmaf(th.gb, nu1, th.ga);
-        const float3 e = table_lerp(etab, e_max, fmaf(mug, he, he));
+        const float3 e = table_lerp(etab, e_max, fmaf(eB, t, eA));
         v.x = fmaf(th.gf[0], e.x, v.x);
         v.y = fmaf(th.gf[1], e.y, v.y);
         v.z = fmaf(th.gf[2], e.z, v.z);
       }
-      l0 += v.x;
-      l1 += v.y;
-      l2 += v.z;
+      if (sgn == 0) {
+        l0 = v.x;
+        l1 = v.y;
+        l2 = v.z;
+      } else {
+        l0 += v.x;
+        l1 += v.y;
+        l2 += v.z;
+      }
     }
     acc0 = fmaf(l0, p0, acc0);
     acc1 = fmaf(l1, p1, acc1);
@@ -478,8 +508,8 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
     const float wy = s_wy[l], wy0 = 1.0f - wy;
     const float fa = s_dw[l] * (two ? kR : 1.0f);
     const float4* src = J.dR;
-    float4 a0 = src[r0 + c0], a1 = src[r1 + c0];
-    float4 b0 = src[r0 + c1], b1 = src[r1 + c1];
+    float4 a0 = gload(src, r0 + c0), a1 = gload(src, r1 + c0);
+    float4 b0 = gload(src, r0 + c1), b1 = gload(src, r1 + c1);
     float3 v = make_float3((wy0 * a0.x + wy * a1.x) * fa, (wy0 * a0.y + wy * a1.y) * fa,
                            (wy0 * a0.z + wy * a1.z) * fa);
     float3 vn = make_float3((wy0 * b0.x + wy * b1.x) * fa, (wy0 * b0.y + wy * b1.y) * fa,
@@ -489,10 +519,10 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
     if (two) {
       const float fb = s_dw[l] * kM;
       src = J.dM;
-      a0 = src[r0 + c0];
-      a1 = src[r1 + c0];
-      b0 = src[r0 + c1];
-      b1 = src[r1 + c1];
+      a0 = gload(src, r0 + c0);
+      a1 = gload(src, r1 + c0);
+      b0 = gload(src, r0 + c1);
+      b1 = gload(src, r1 + c1);
       v = make_float3((wy0 * a0.x + wy * a1.x) * fb, (wy0 * a0.y + wy * a1.y) * fb,
                       (wy0 * a0.z + wy * a1.z) * fb);
       vn = make_float3((wy0 * b0.x + wy * b1.x) * fb, (wy0 * b0.y + wy * b1.y) * fb,
@@ -504,8 +534,8 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
   // Row 0 of delta irradiance (r = rb): the ground-irradiance lookup of the
   // reference at bottom_radius reads exactly this row.
   for (int e = tid; e < J.ew; e += kBlock) {
-    const float4 v = J.dE[e];
-    const float4 vn = J.dE[e + 1 < J.ew ? e + 1 : e];
+    const float4 v = gload(J.dE, e);
+    const float4 vn = gload(J.dE, e + 1 < J.ew ? e + 1 : e);
     etab[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
     etab[2 * e + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
   }
@@ -562,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
                                cM, t, acc0, acc1, acc2);
     }
   }
-  J.dJ[(layer + y) * W + nui * nmus + imus] = make_float4(acc0, acc1, acc2, 0.0f);
+  gstore(J.dJ, (layer + y) * W + nui * nmus + imus, make_float4(acc0, acc1, acc2, 0.0f));
 }
 
 // ---------------------------------------------------------------------------
@@ -616,7 +646,7 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
             for (int c = 0; c < 2; ++c) {
               const float w = wz[a] * wy[b] * wx[c];
               const int64_t base = (static_cast<int64_t>(zs[a]) * nmu + ys[b]) * W + xs[c];
-              const float4 p = src[base + n * nmus], q = src[base + n1 * nmus];
+              const float4 p = gload(src, base + n * nmus), q = gload(src, base + n1 * nmus);
               v.x += w * p.x;
               v.y += w * p.y;
               v.z += w * p.z;
@@ -672,13 +702,13 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
   }
   if (lane == 0) {
     const float3 de = make_float3(a0, a1, a2);
-    J.dE[texel] = make_float4(de.x, de.y, de.z, 0.0f);
+    gstore(J.dE, texel, make_float4(de.x, de.y, de.z, 0.0f));
     const float3 le = apply_lum(J.lum, de);
-    float4 e = J.E[texel];
+    float4 e = gload(J.E, texel);
     e.x += le.x;
     e.y += le.y;
     e.z += le.z;
-    J.E[texel] = e;
+    gstore(J.E, texel, e);
   }
 }
 
@@ -729,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   // bounds should it not.
   if (rpb == 0 || W > kBlock * kTpt) return;
   const int nrows = J.nr * nmu;
-  const int row0 = lb * rpb;
+  const int row0 = xcd_contiguous(lb, (nrows + rpb - 1) / rpb) * rpb;
   const int tid = threadIdx.x;
 
   // Per-row sample geometry: thread t < rpb * kMarch sets up sample
@@ -817,6 +847,22 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   const float invA = static_cast<float>(1.0 / J.A);
   const float xmax = static_cast<float>(nmus - 1);
   const float4* __restrict__ D = J.dJ;
+  // Source texels of the next sample, loaded one sample ahead so that the
+  // loads are in flight during the barrier and the gather of the current one.
+  float3 src[kTpt][4];
+  auto fetch = [&](int i) {
+#pragma unroll
+    for (int j = 0; j < kTpt; ++j) {
+      if (slot_of[j] < 0) continue;
+      const MsRowSample& S = smp[slot_of[j]][i];
+      const int x = x_of[j];
+      src[j][0] = make3(gload(D, S.o00 + x));
+      src[j][1] = make3(gload(D, S.o01 + x));
+      src[j][2] = make3(gload(D, S.o10 + x));
+      src[j][3] = make3(gload(D, S.o11 + x));
+    }
+  };
+  fetch(0);
   for (int i = 0; i < kMarch; ++i) {
     float4* B = buf[i & 1];
     // Blend the four (r, mu) source rows of sample i into LDS.
@@ -824,9 +870,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     for (int j = 0; j < kTpt; ++j) {
       if (slot_of[j] < 0) continue;
       const MsRowSample& S = smp[slot_of[j]][i];
-      const int x = x_of[j];
-      const float4 a = D[S.o00 + x], b = D[S.o01 + x];
-      const float4 c = D[S.o10 + x], e = D[S.o11 + x];
+      const float3 a = src[j][0], b = src[j][1], c = src[j][2], e = src[j][3];
       float4 v;
       v.x = fmaf(S.w11, e.x, fmaf(S.w10, c.x, fmaf(S.w01, b.x, S.w00 * a.x)));
       v.y = fmaf(S.w11, e.y, fmaf(S.w10, c.y, fmaf(S.w01, b.y, S.w00 * a.y)));
@@ -834,6 +878,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
       v.w = 0.0f;
       B[tid + j * kBlock] = v;
     }
+    if (i + 1 < kMarch) fetch(i + 1);
     // One barrier per sample: buffer i & 1 is written again at sample i + 2,
     // after every thread has passed the barrier of sample i + 1 and so has
     // finished reading it.
@@ -870,15 +915,15 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   for (int j = 0; j < kTpt; ++j) {
     if (slot_of[j] < 0) continue;
     const int64_t t_idx = static_cast<int64_t>(row0 + slot_of[j]) * W + x_of[j];
-    J.dR[t_idx] = make_float4(acc[j][0], acc[j][1], acc[j][2], 0.0f);
+    gstore(J.dR, t_idx, make_float4(acc[j][0], acc[j][1], acc[j][2], 0.0f));
     const float3 l = apply_lum(J.lum, make_float3(acc[j][0], acc[j][1], acc[j][2]));
     const double nu = nu_of[j];
     const float inv_pr = static_cast<float>(1.0 / ((3.0 / (16.0 * kPi)) * (1.0 + nu * nu)));
-    float4 sv = J.S[t_idx];
+    float4 sv = gload(J.S, t_idx);
     sv.x += l.x * inv_pr;
     sv.y += l.y * inv_pr;
     sv.z += l.z * inv_pr;
-    J.S[t_idx] = sv;
+    gstore(J.S, t_idx, sv);
   }
 }
 
@@ -990,7 +1035,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
       float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const float4 v = D[rows[q] + cols[c]];
+        const float4 v = gload(D, rows[q] + cols[c]);
         r0 = fmaf(wc[c], v.x, r0);
         r1 = fmaf(wc[c], v.y, r1);
         r2 = fmaf(wc[c], v.z, r2);
@@ -1004,14 +1049,14 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
     a2 = fmaf(j2, S.f[2], a2);
   }
   const int64_t t_idx = static_cast<int64_t>(row) * W + x;
-  J.dR[t_idx] = make_float4(a0, a1, a2, 0.0f);
+  gstore(J.dR, t_idx, make_float4(a0, a1, a2, 0.0f));
   const float3 l = apply_lum(J.lum, make_float3(a0, a1, a2));
   const float inv_pr = static_cast<float>(1.0 / ((3.0 / (16.0 * kPi)) * (1.0 + nu * nu)));
-  float4 sv = J.S[t_idx];
+  float4 sv = gload(J.S, t_idx);
   sv.x += l.x * inv_pr;
   sv.y += l.y * inv_pr;
   sv.z += l.z * inv_pr;
-  J.S[t_idx] = sv;
+  gstore(J.S, t_idx, sv);
 }
 
 // ---------------------------------------------------------------------------
@@ -1042,8 +1087,8 @@ __global__ __launch_bounds__(kBlock) void k_lut_max(const Job* __restrict__ jobs
   const int64_t t = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
   float4 s = make_float4(0.f, 0.f, 0.f, 0.f), m = s;
   if (t < n) {
-    s = J.S[t];
-    if (!J.combined) m = J.SM[t];
+    s = gload(J.S, t);
+    if (!J.combined) m = gload(J.SM, t);
   }
   // |value| as non-negative float bits compare like unsigned integers.
   const float v[8] = {fabsf(s.x), fabsf(s.y), fabsf(s.z), fabsf(s.w),
@@ -1077,11 +1122,11 @@ __global__ __launch_bounds__(kBlock) void k_to_half(const Job* __restrict__ jobs
   const int64_t n = static_cast<int64_t>(J.nr) * J.nmu * J.nmus * J.nnu;
   const int64_t t = static_cast<int64_t>(lb) * kBlock + threadIdx.x;
   if (t >= n) return;
-  const float4 s = J.S[t];
-  J.S16[t] = pack_half4(make_float4(s.x * sc[0], s.y * sc[1], s.z * sc[2], s.w * sc[3]));
+  const float4 s = gload(J.S, t);
+  gstore(J.S16, t, pack_half4(make_float4(s.x * sc[0], s.y * sc[1], s.z * sc[2], s.w * sc[3])));
   if (!J.combined) {
-    const float4 m = J.SM[t];
-    J.SM16[t] = pack_half4(make_float4(m.x * sc[4], m.y * sc[5], m.z * sc[6], 0.0f));
+    const float4 m = gload(J.SM, t);
+    gstore(J.SM16, t, pack_half4(make_float4(m.x * sc[4], m.y * sc[5], m.z * sc[6], 0.0f)));
   }
 }
 
