@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -98,8 +99,9 @@ int ThreadCount() {
   if (g_threads > 0) return g_threads;
   const char* env = std::getenv("ORACLE_THREADS");
   if (env != nullptr && std::atoi(env) > 0) return std::atoi(env);
+  // Default: the host's threads, at most 16 (a GPU box's CPU share).
   unsigned hc = std::thread::hardware_concurrency();
-  return hc == 0 ? 1 : static_cast<int>(hc);
+  return hc == 0 ? 1 : static_cast<int>(hc < 16 ? hc : 16);
 }
 
 template <class F>
@@ -936,9 +938,12 @@ void TexelXYZ(const Sizes& s, size_t idx, int* x, int* y, int* z) {
 // atmosphere/model.cc:1048-1215 with the GL programs of model.cc:125-208.
 // luminance_from_radiance is applied as in the shaders; 'blend' adds into the
 // outputs (glBlendFunc(ONE, ONE)) instead of overwriting them.
+// 'after_order', if set, is called once the outputs hold orders 1..n, for
+// n = 1..orders.
 void Precompute(const Atmo& a, const Sizes& s, const double* L, bool blend,
                 unsigned orders, bool combined, double* transmittance,
-                double* scattering, double* single_mie, double* irradiance) {
+                double* scattering, double* single_mie, double* irradiance,
+                const std::function<void(unsigned)>& after_order = nullptr) {
   const int tw = s.transmittance_width, th = s.transmittance_height;
   const int ew = s.irradiance_width, eh = s.irradiance_height;
   const size_t n_s = ScatteringTexels(s);
@@ -998,6 +1003,7 @@ void Precompute(const Atmo& a, const Sizes& s, const double* L, bool blend,
         }
       }
   });
+  if (after_order) after_order(1);
 
   for (unsigned order = 2; order <= orders; ++order) {
     ParallelFor(s.scattering_r, [&](int z) {
@@ -1048,6 +1054,7 @@ void Precompute(const Atmo& a, const Sizes& s, const double* L, bool blend,
           Store(scattering, t, Add(Fetch(scattering, t), add));
         }
     });
+    if (after_order) after_order(order);
   }
 }
 
@@ -1559,6 +1566,47 @@ int or_model_init(const or_model_params* p, const or_sizes* s,
   }
   if (p->num_precomputed_wavelengths > 3) {
     // model.cc:951-963: the transmittance is recomputed at 680/550/440 nm.
+    const double rgb[3] = {kLambdaR, kLambdaG, kLambdaB};
+    Atmo a;
+    Lower(*p, rgb, &a);
+    or_pass_transmittance(&a, s, transmittance);
+  }
+  return 0;
+}
+
+// Model::Init for every order count 1..orders in one pass: out_*[n - 1]
+// receive the LUTs of Init(n). Each iteration is computed on its own
+// (blend off) and the per-order states are summed over iterations, which is
+// what blending the iterations into one LUT computes.
+int or_model_init_orders(const or_model_params* p, const or_sizes* s,
+                         unsigned orders, double* transmittance,
+                         double* const* scattering, double* const* single_mie,
+                         double* const* irradiance) {
+  if (p == nullptr || s == nullptr || p->num_wavelengths < 1 || orders < 1) return 1;
+  const bool combined = p->combine_scattering_textures != 0;
+  const size_t n_s = ScatteringTexels(*s) * 4;
+  const size_t n_e = static_cast<size_t>(s->irradiance_width) * s->irradiance_height * 4;
+  for (unsigned n = 0; n < orders; ++n) {
+    std::fill(scattering[n], scattering[n] + n_s, 0.0);
+    if (!combined) std::fill(single_mie[n], single_mie[n] + n_s, 0.0);
+    std::fill(irradiance[n], irradiance[n] + n_e, 0.0);
+  }
+  std::vector<double> S(n_s), M(n_s), E(n_e);
+  const int iters = NumIterations(*p);
+  for (int i = 0; i < iters; ++i) {
+    double lambdas[3], L[9];
+    Iteration(*p, i, lambdas, L);
+    Atmo a;
+    Lower(*p, lambdas, &a);
+    Precompute(a, *s, L, false, orders, combined, transmittance, S.data(), M.data(),
+               E.data(), [&](unsigned n) {
+                 for (size_t k = 0; k < n_s; ++k) scattering[n - 1][k] += S[k];
+                 if (!combined)
+                   for (size_t k = 0; k < n_s; ++k) single_mie[n - 1][k] += M[k];
+                 for (size_t k = 0; k < n_e; ++k) irradiance[n - 1][k] += E[k];
+               });
+  }
+  if (p->num_precomputed_wavelengths > 3) {
     const double rgb[3] = {kLambdaR, kLambdaG, kLambdaB};
     Atmo a;
     Lower(*p, rgb, &a);

@@ -233,6 +233,12 @@ int or_model_init(const or_model_params* p, const or_sizes* s,
                   unsigned num_scattering_orders, double* transmittance_out,
                   double* scattering_out, double* single_mie_out,
                   double* irradiance_out);
+/* Init(n) for every n = 1..orders in one run: scattering[n-1] etc. receive
+ * the LUTs of Init(n) (arrays of 'orders' pointers). */
+int or_model_init_orders(const or_model_params* p, const or_sizes* s,
+                         unsigned orders, double* transmittance,
+                         double* const* scattering, double* const* single_mie,
+                         double* const* irradiance);
 /* One Model::Precompute call (atmosphere/model.cc:1048-1215) on
  * caller-provided LUTs, so illuminance iterations can be checked one by one.
  * matrix9 is luminance_from_radiance (row-major). */

@@ -132,6 +132,8 @@ def _declare(L):
         "or_num_iterations": (i, [C.POINTER(ModelParams)]),
         "or_iteration": (None, [C.POINTER(ModelParams), i, DP, DP]),
         "or_model_init": (i, [C.POINTER(ModelParams), S, C.c_uint, DP, DP, DP, DP]),
+        "or_model_init_orders": (i, [C.POINTER(ModelParams), S, C.c_uint, DP,
+                                     C.POINTER(DP), C.POINTER(DP), C.POINTER(DP)]),
         "or_precompute": (None, [A, S, DP, i, C.c_uint, i, DP, DP, DP, DP]),
         "or_sky_radiance": (None, [A, S, DP, DP, DP, i, i, DP, DP, DP, DP, DP, DP]),
         "or_sky_radiance_to_point": (None, [A, S, DP, DP, DP, i, i, DP, DP, DP, DP,
@@ -278,6 +280,26 @@ def model_init(spec, s, orders):
     out = dict(transmittance=T, scattering=S, irradiance=E)
     if not spec.combine_scattering_textures:
         out["single_mie_scattering"] = M
+    return out
+
+
+def model_init_orders(spec, s, orders):
+    """FP64 Model::Init(n) for n = 1..orders in one run: a list whose entry
+    n - 1 is the dict of LUTs of Init(n)."""
+    T = np.zeros(transmittance_shape(s))
+    Ss = [np.zeros(scattering_shape(s)) for _ in range(orders)]
+    Ms = [np.zeros(scattering_shape(s)) for _ in range(orders)]
+    Es = [np.zeros(irradiance_shape(s)) for _ in range(orders)]
+    arr = lambda xs: (DP * orders)(*[ptr(x) for x in xs])
+    rc = lib().or_model_init_orders(C.byref(spec.c), s, orders, ptr(T), arr(Ss), arr(Ms),
+                                    arr(Es))
+    assert rc == 0
+    out = []
+    for n in range(orders):
+        d = dict(transmittance=T, scattering=Ss[n], irradiance=Es[n])
+        if not spec.combine_scattering_textures:
+            d["single_mie_scattering"] = Ms[n]
+        out.append(d)
     return out
 
 

@@ -64,12 +64,6 @@ struct Job {
   // mu_s mapping (functions.glsl:819-821).
   double H;
   double A;
-  // Lookup constants derived from the above on the host: squared radii and
-  // FP32 copies for the lookups' FP32 stages (1/H; mu_s mapping rb, H^2,
-  // rt - rb, 1 / (H - (rt - rb)), 1/A; phase-function constants).
-  double rb2, rt2, H2;
-  float q_inv_H, q_rb, q_H2, q_dmin, q_inv_range, q_inv_A;
-  float q_kR, q_kM, q_g1, q_mg2;
   // luminance_from_radiance, row-major (atmosphere/model.cc:909, 939-943).
   float lum[9];
   // SKY/SUN_SPECTRAL_RADIANCE_TO_LUMINANCE (atmosphere/model.cc:668-686,

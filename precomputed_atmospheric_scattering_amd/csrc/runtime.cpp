@@ -161,20 +161,6 @@ void lower(const Spec& s, const double* lambdas, atmo::Job* j) {
   const double d_min = j->rt - j->rb, d_max = j->H;
   const double D = atmo::dist_top(*j, j->rb, j->mu_s_min);
   j->A = (D - d_min) / (d_max - d_min);
-  j->rb2 = j->rb * j->rb;
-  j->rt2 = j->rt * j->rt;
-  j->H2 = j->H * j->H;
-  j->q_inv_H = static_cast<float>(1.0 / j->H);
-  j->q_rb = static_cast<float>(j->rb);
-  j->q_H2 = static_cast<float>(j->H2);
-  j->q_dmin = static_cast<float>(d_min);
-  j->q_inv_range = static_cast<float>(1.0 / (d_max - d_min));
-  j->q_inv_A = static_cast<float>(1.0 / j->A);
-  const double g = j->g;
-  j->q_kR = static_cast<float>(3.0 / (16.0 * atmo::kPi));
-  j->q_kM = static_cast<float>(3.0 / (8.0 * atmo::kPi) * (1.0 - g * g) / (2.0 + g * g));
-  j->q_g1 = static_cast<float>(1.0 + g * g);
-  j->q_mg2 = static_cast<float>(-2.0 * g);
 }
 
 void set_sizes(atmo::Job* j, const Sizes& z) {

@@ -221,3 +221,20 @@ def test_ground_albedo_adds_light():
     assert d.min() >= -1e-15 and d.max() > 0
     # Single scattering does not see the ground.
     assert np.array_equal(bright["single_mie_scattering"], dark["single_mie_scattering"])
+
+
+@pytest.mark.parametrize("make", [configs.earth, configs.spectral])
+def test_per_order_snapshots_equal_separate_inits(make):
+    # or_model_init_orders computes every iteration on its own and sums the
+    # per-order states; that must equal Init(n) with blended iterations.
+    cfg = make(lut_sizes=TINY)
+    kw, _ = configs.model_kwargs(cfg)
+    kw.pop("lut_sizes")
+    spec = ol.ModelSpec(**kw)
+    s = ol.sizes(**TINY)
+    snaps = ol.model_init_orders(spec, s, 3)
+    for n in (1, 2, 3):
+        ref = ol.model_init(spec, s, n)
+        for k, r in ref.items():
+            scale = max(np.abs(r).max(), 1e-300)
+            assert np.abs(snaps[n - 1][k] - r).max() <= 1e-12 * scale, (n, k)
