@@ -1174,10 +1174,12 @@ void Iteration(const or_model_params& p, int i, double* lambdas, double* L) {
   for (int row = 0; row < 3; ++row)
     for (int col = 0; col < 3; ++col) {
       double l = lambdas[col];
-      L[3 * row + col] = (m[3 * row] * CieValue(l, 1) +
-                          m[3 * row + 1] * CieValue(l, 2) +
-                          m[3 * row + 2] * CieValue(l, 3)) *
-                         dlambda;
+      // The reference rounds each coefficient to float (model.cc:934): the
+      // GL programs receive luminance_from_radiance as a float mat3.
+      L[3 * row + col] = static_cast<double>(static_cast<float>(
+          (m[3 * row] * CieValue(l, 1) + m[3 * row + 1] * CieValue(l, 2) +
+           m[3 * row + 2] * CieValue(l, 3)) *
+          dlambda));
     }
 }
 

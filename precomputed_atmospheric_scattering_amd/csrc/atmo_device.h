@@ -35,6 +35,19 @@ ATMO_HD void gstore(T* p, int64_t i, const T& v) {
   p[i] = v;
 #endif
 }
+// Load at an unsigned 32-bit element index (LUTs hold at most 2^28 texels,
+// so the byte offset fits 32 bits): with a uniform base the compiler uses
+// the scalar-base + 32-bit vector-offset form, no 64-bit address per lane.
+template <class T>
+ATMO_HD T gload32(const T* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __attribute__((address_space(1))) char* b =
+      (const __attribute__((address_space(1))) char*)p;
+  return *(const __attribute__((address_space(1))) T*)(b + i * static_cast<uint32_t>(sizeof(T)));
+#else
+  return p[i];
+#endif
+}
 
 namespace atmo {
 

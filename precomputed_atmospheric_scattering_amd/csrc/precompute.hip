@@ -181,16 +181,16 @@ __global__ __launch_bounds__(kBlock) void k_direct_irradiance(
 constexpr int kMarch = 51;  // 50 trapezoid intervals
 
 struct SsSample {
-  double d;        // distance along the view ray
-  double inv_r;    // 1 / r_d
-  double rho;      // sqrt(r_d^2 - rb^2)
+  float df;        // distance along the view ray
+  float inv_rf;    // 1 / r_d
+  float rhof;      // sqrt(r_d^2 - rb^2)
   float rf;        // r_d
   float c;         // rt^2 - r_d^2
   float dmin;      // rt - r_d
   float inv_range; // 1 / (rho + H - dmin)
   float e;         // sin(theta_h) * sun angular radius
   float inv_span;  // 1 / (2 e)
-  int y0, y1;      // transmittance LUT rows of r_d
+  int t0, t1;      // first texels of the transmittance LUT rows of r_d
   float wy;
   float fr[3];     // trapezoid weight * view transmittance * Rayleigh density
   float fm[3];     // same with the Mie density
@@ -211,8 +211,9 @@ __device__ __forceinline__ float3 tsun_lookup(const Job& J, const SsSample& S,
   const int i0 = static_cast<int>(fl);
   const int i1 = min(i0 + 1, J.tw - 1);
   const float4* __restrict__ T = J.T;
-  const float4 a = gload(T, S.y0 * J.tw + i0), b = gload(T, S.y0 * J.tw + i1);
-  const float4 c = gload(T, S.y1 * J.tw + i0), d = gload(T, S.y1 * J.tw + i1);
+  const uint32_t r0 = static_cast<uint32_t>(S.t0), r1 = static_cast<uint32_t>(S.t1);
+  const float4 a = gload32(T, r0 + i0), b = gload32(T, r0 + i1);
+  const float4 c = gload32(T, r1 + i0), d = gload32(T, r1 + i1);
   const float wy1 = S.wy, wy0 = 1.0f - S.wy;
   float3 o;
   o.x = wy0 * fmaf(w, b.x - a.x, a.x) + wy1 * fmaf(w, d.x - c.x, c.x);
@@ -247,9 +248,9 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
     const double r_d = clamp_radius(J, sqrt(d * d + 2.0 * r * mu * d + r * r));
     const float3 tv = transmittance(J, J.T, r, mu, d, ground);
     const double rho_d = safe_sqrt(r_d * r_d - J.rb * J.rb);
-    S.d = d;
-    S.inv_r = 1.0 / r_d;
-    S.rho = rho_d;
+    S.df = static_cast<float>(d);
+    S.inv_rf = static_cast<float>(1.0 / r_d);
+    S.rhof = static_cast<float>(rho_d);
     S.rf = static_cast<float>(r_d);
     S.c = static_cast<float>(J.rt * J.rt - r_d * r_d);
     S.dmin = static_cast<float>(J.rt - r_d);
@@ -258,8 +259,8 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
     S.e = static_cast<float>(e);
     S.inv_span = static_cast<float>(0.5 / e);
     const Taps ty = axis_taps(unit_to_texel(rho_d / J.H, J.th), J.th);
-    S.y0 = ty.i0;
-    S.y1 = ty.i1;
+    S.t0 = ty.i0 * J.tw;
+    S.t1 = ty.i1 * J.tw;
     S.wy = ty.w;
     const double w = (tid == 0 || tid == kMarch - 1) ? 0.5 : 1.0;
     const float dr = static_cast<float>(w * profile_density(J.prof_r, r_d - J.rb));
@@ -277,13 +278,17 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
   if (x >= W) return;
   const double mu_s = s_col_mus(J, x % J.nmus);
   const double nu = s_slice_nu(J, x / J.nmus, mu, mu_s);
-  const double rmus = r * mu_s;
+  // r mu_s + d nu = r_d mu_s_d in FP32: its absolute error (~5e-4 length
+  // units at r ~ 6400) divided by r_d moves mu_s_d by ~1e-7, far below a
+  // texel and below the visible-disc smoothstep width (2 rb / r_d times the
+  // sun angular radius).
+  const float rmus = static_cast<float>(r * mu_s), nuf = static_cast<float>(nu);
   float ar0 = 0.f, ar1 = 0.f, ar2 = 0.f, am0 = 0.f, am1 = 0.f, am2 = 0.f;
   for (int s = 0; s < kMarch; ++s) {
     const SsSample& S = smp[s];
-    const double t = fma(S.d, nu, rmus);  // r * mu_s + d * nu = r_d * mu_s_d
-    const float xs = static_cast<float>((t + S.rho) * S.inv_r);  // mu_s_d - cos(theta_h)
-    const float mus = clampf(static_cast<float>(t * S.inv_r), -1.0f, 1.0f);
+    const float t = fmaf(S.df, nuf, rmus);
+    const float xs = (t + S.rhof) * S.inv_rf;  // mu_s_d - cos(theta_h)
+    const float mus = clampf(t * S.inv_rf, -1.0f, 1.0f);
     const float u = clampf((xs + S.e) * S.inv_span, 0.0f, 1.0f);
     const float vis = u * u * (3.0f - 2.0f * u);
     const float3 ts = tsun_lookup(J, S, mus);
@@ -342,6 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
 // table of delta-irradiance row 0 (r = rb).
 // ---------------------------------------------------------------------------
 constexpr int kTheta = 16;
+constexpr int kNuChunk = 4;  // nu outputs per thread
 
 // Table entry i: (v.x, v.y, v.z, dv.x), (dv.y, dv.z, -, -) with
 // dv = v[i + 1] - v[i] (0 for the last entry).
@@ -367,20 +373,30 @@ struct SdTheta {
   int ground;
 };
 
+// One incident zenith angle theta_l for the kNuChunk outputs (same r, mu,
+// mu_s; consecutive nu) of a thread. The phase function of the scattering
+// towards the view direction depends on (mu, theta_l, phi) only, so it is
+// computed once per azimuth pair for all kNuChunk outputs.
 template <bool kOrder2, bool kGround>
 __device__ __forceinline__ void sd_theta(
     const float4* __restrict__ tab_a, const float4* __restrict__ tab_b,
-    const float4* __restrict__ etab, float nu_max, float e_max,
-    float A, float B, float C, float P, float Q, float g1, float mg2,
-    const float cR[3], const float cM[3], const SdTheta& th, float& acc0,
-    float& acc1, float& acc2) {
+    const float4* __restrict__ etab, float nu_max, float e_max, float A, float P,
+    float Q, const float (&Bv)[kNuChunk], const float (&Cv)[kNuChunk], float g1,
+    float mg2, const float cR[3], const float cM[3], const SdTheta& th,
+    float (&acc)[kNuChunk][3]) {
   // nu table coordinate of the incident direction at azimuth phi:
   // t = (nu1 + 1) / 2 * (NU - 1) with nu1 = A + B cos(phi) +- C sin(phi),
   // as one FMA from per-angle constants; likewise the ground-irradiance
   // table coordinate (mu_g + 1) / 2 * (E_W - 1) with mu_g = ga + gb nu1.
   const float hn = 0.5f * nu_max;
   const float inv_hn = 1.0f / hn;
-  const float tA = fmaf(A, hn, hn), tB = B * hn, tC = C * hn;
+  const float tA = fmaf(A, hn, hn);
+  float tB[kNuChunk], tC[kNuChunk];
+#pragma unroll
+  for (int j = 0; j < kNuChunk; ++j) {
+    tB[j] = Bv[j] * hn;
+    tC[j] = Cv[j] * hn;
+  }
   const float he = 0.5f * e_max;
   const float eB = th.gb * inv_hn * he;
   const float eA = fmaf(th.ga - th.gb, he, he);
@@ -388,7 +404,7 @@ __device__ __forceinline__ void sd_theta(
   for (int m = 0; m < 16; ++m) {
     const float c = kCosPhi[m], s = kSinPhi[m];
     // Phase function of the scattering towards -omega, shared by phi and
-    // 2 pi - phi.
+    // 2 pi - phi and by the nu outputs.
     const float nu2 = fmaf(Q, c, P);
     const float x2 = fmaf(nu2, nu2, 1.0f);
     const float rs = rsq_f(fmaf(mg2, nu2, g1));
@@ -396,43 +412,46 @@ __device__ __forceinline__ void sd_theta(
     const float p0 = x2 * fmaf(cM[0], rs3, cR[0]);
     const float p1 = x2 * fmaf(cM[1], rs3, cR[1]);
     const float p2 = x2 * fmaf(cM[2], rs3, cR[2]);
-    const float tb = fmaf(tB, c, tA);
-    float l0, l1, l2;
 #pragma unroll
-    for (int sgn = 0; sgn < 2; ++sgn) {
-      const float t = sgn == 0 ? fmaf(tC, s, tb) : fmaf(-tC, s, tb);
-      float3 v = table_lerp(tab_a, nu_max, t);
-      if (kOrder2) {
-        // Order-1 incident radiance: Rayleigh and Mie single scattering
-        // times their phase functions of nu1 (constants folded in tables).
-        const float nu1 = fmaf(t, inv_hn, -1.0f);
-        const float3 vm = table_lerp(tab_b, nu_max, t);
-        const float r1 = rsq_f(fmaf(mg2, nu1, g1));
-        const float q = r1 * r1 * r1;
-        const float x1 = fmaf(nu1, nu1, 1.0f);
-        v.x = x1 * fmaf(vm.x, q, v.x);
-        v.y = x1 * fmaf(vm.y, q, v.y);
-        v.z = x1 * fmaf(vm.z, q, v.z);
+    for (int j = 0; j < kNuChunk; ++j) {
+      const float tb = fmaf(tB[j], c, tA);
+      float l0, l1, l2;
+#pragma unroll
+      for (int sgn = 0; sgn < 2; ++sgn) {
+        const float t = sgn == 0 ? fmaf(tC[j], s, tb) : fmaf(-tC[j], s, tb);
+        float3 v = table_lerp(tab_a, nu_max, t);
+        if (kOrder2) {
+          // Order-1 incident radiance: Rayleigh and Mie single scattering
+          // times their phase functions of nu1 (constants folded in tables).
+          const float nu1 = fmaf(t, inv_hn, -1.0f);
+          const float3 vm = table_lerp(tab_b, nu_max, t);
+          const float r1 = rsq_f(fmaf(mg2, nu1, g1));
+          const float q = r1 * r1 * r1;
+          const float x1 = fmaf(nu1, nu1, 1.0f);
+          v.x = x1 * fmaf(vm.x, q, v.x);
+          v.y = x1 * fmaf(vm.y, q, v.y);
+          v.z = x1 * fmaf(vm.z, q, v.z);
+        }
+        if (kGround) {
+          const float3 e = table_lerp(etab, e_max, fmaf(eB, t, eA));
+          v.x = fmaf(th.gf[0], e.x, v.x);
+          v.y = fmaf(th.gf[1], e.y, v.y);
+          v.z = fmaf(th.gf[2], e.z, v.z);
+        }
+        if (sgn == 0) {
+          l0 = v.x;
+          l1 = v.y;
+          l2 = v.z;
+        } else {
+          l0 += v.x;
+          l1 += v.y;
+          l2 += v.z;
+        }
       }
-      if (kGround) {
-        const float3 e = table_lerp(etab, e_max, fmaf(eB, t, eA));
-        v.x = fmaf(th.gf[0], e.x, v.x);
-        v.y = fmaf(th.gf[1], e.y, v.y);
-        v.z = fmaf(th.gf[2], e.z, v.z);
-      }
-      if (sgn == 0) {
-        l0 = v.x;
-        l1 = v.y;
-        l2 = v.z;
-      } else {
-        l0 += v.x;
-        l1 += v.y;
-        l2 += v.z;
-      }
+      acc[j][0] = fmaf(l0, p0, acc[j][0]);
+      acc[j][1] = fmaf(l1, p1, acc[j][1]);
+      acc[j][2] = fmaf(l2, p2, acc[j][2]);
     }
-    acc0 = fmaf(l0, p0, acc0);
-    acc1 = fmaf(l1, p1, acc1);
-    acc2 = fmaf(l2, p2, acc2);
   }
 }
 
@@ -447,7 +466,8 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
   const Job& J = jobs[ji];
   const int lb = blockIdx.x - first_block[ji];
   const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
-  const int plane = nmu * nnu;
+  const int nch = (nnu + kNuChunk - 1) / kNuChunk;
+  const int plane = nmu * nch;  // threads per (r, mu_s) plane
   const int tiles = (plane + kBlock - 1) / kBlock;
   const int tile = lb % tiles;
   const int pl = lb / tiles;
@@ -543,17 +563,26 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
 
   const int flat = tile * kBlock + tid;
   if (flat >= plane) return;
-  const int nui = flat % nnu, y = flat / nnu;
+  const int ch = flat % nch, y = flat / nch;
   double mu;
   bool ground_unused;
   s_row_mu(J, y, r, rho, &mu, &ground_unused);
-  const double nu = s_slice_nu(J, nui, mu, mu_s);
-  // functions.glsl:1181-1185
+  // functions.glsl:1181-1185 for each nu of the chunk. Outputs past the last
+  // nu slice get sx = sy = 0 (in-range table reads) and are not stored.
   const double wx = sqrt(1.0 - mu * mu);
-  const double sx = wx == 0.0 ? 0.0 : (nu - mu * mu_s) / wx;
-  const double sy = sqrt(fmax(1.0 - sx * sx - mu_s * mu_s, 0.0));
-  const float fmus = static_cast<float>(mu_s), fsx = static_cast<float>(sx),
-              fsy = static_cast<float>(sy), fmu = static_cast<float>(mu),
+  float fsx[kNuChunk], fsy[kNuChunk];
+#pragma unroll
+  for (int jn = 0; jn < kNuChunk; ++jn) {
+    const int nui = ch * kNuChunk + jn;
+    fsx[jn] = fsy[jn] = 0.0f;
+    if (nui >= nnu) continue;
+    const double nu = s_slice_nu(J, nui, mu, mu_s);
+    const double sx = wx == 0.0 ? 0.0 : (nu - mu * mu_s) / wx;
+    const double sy = sqrt(fmax(1.0 - sx * sx - mu_s * mu_s, 0.0));
+    fsx[jn] = static_cast<float>(sx);
+    fsy[jn] = static_cast<float>(sy);
+  }
+  const float fmus = static_cast<float>(mu_s), fmu = static_cast<float>(mu),
               fwx = static_cast<float>(wx);
   const double h = r - J.rb;
   const double dr = profile_density(J.prof_r, h);
@@ -568,31 +597,44 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
   const float mg2 = static_cast<float>(-2.0 * gd);
   const float nu_max = static_cast<float>(nnu - 1);
   const float e_max = static_cast<float>(J.ew - 1);
-  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+  float acc[kNuChunk][3];
+#pragma unroll
+  for (int jn = 0; jn < kNuChunk; ++jn) acc[jn][0] = acc[jn][1] = acc[jn][2] = 0.0f;
   for (int l = 0; l < kTheta; ++l) {
     const SdTheta t = th[l];
-    const float A = fmus * t.cos_t, B = fsx * t.sin_t, C = fsy * t.sin_t;
-    const float P = fmu * t.cos_t, Q = fwx * t.sin_t;
+    const float A = fmus * t.cos_t, P = fmu * t.cos_t, Q = fwx * t.sin_t;
+    float Bv[kNuChunk], Cv[kNuChunk];
+#pragma unroll
+    for (int jn = 0; jn < kNuChunk; ++jn) {
+      Bv[jn] = fsx[jn] * t.sin_t;
+      Cv[jn] = fsy[jn] * t.sin_t;
+    }
     const float4* ta = tab_a + 2 * l * nnu;
     const float4* tb = tab_b + 2 * l * nnu;
     const bool g = __builtin_amdgcn_readfirstlane(t.ground) != 0;
     if (two) {
       if (g)
-        sd_theta<true, true>(ta, tb, etab, nu_max, e_max, A, B, C, P, Q, g1, mg2, cR,
-                             cM, t, acc0, acc1, acc2);
+        sd_theta<true, true>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR, cM,
+                             t, acc);
       else
-        sd_theta<true, false>(ta, tb, etab, nu_max, e_max, A, B, C, P, Q, g1, mg2, cR,
-                              cM, t, acc0, acc1, acc2);
+        sd_theta<true, false>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR,
+                              cM, t, acc);
     } else {
       if (g)
-        sd_theta<false, true>(ta, tb, etab, nu_max, e_max, A, B, C, P, Q, g1, mg2, cR,
-                              cM, t, acc0, acc1, acc2);
+        sd_theta<false, true>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR,
+                              cM, t, acc);
       else
-        sd_theta<false, false>(ta, tb, etab, nu_max, e_max, A, B, C, P, Q, g1, mg2, cR,
-                               cM, t, acc0, acc1, acc2);
+        sd_theta<false, false>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR,
+                               cM, t, acc);
     }
   }
-  gstore(J.dJ, (layer + y) * W + nui * nmus + imus, make_float4(acc0, acc1, acc2, 0.0f));
+#pragma unroll
+  for (int jn = 0; jn < kNuChunk; ++jn) {
+    const int nui = ch * kNuChunk + jn;
+    if (nui < nnu)
+      gstore(J.dJ, (layer + y) * W + nui * nmus + imus,
+             make_float4(acc[jn][0], acc[jn][1], acc[jn][2], 0.0f));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -734,8 +776,8 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
 // depends on its own row length only.
 // ---------------------------------------------------------------------------
 struct MsRowSample {
-  double d;      // distance along the view ray
-  double inv_r;  // 1 / r_i
+  float df;      // distance along the view ray
+  float inv_rf;  // 1 / r_i
   int o00, o01, o10, o11;  // texel offsets of the (r, mu) source rows
   float w00, w01, w10, w11;
   float f[3];    // trapezoid weight * dx * view transmittance
@@ -782,8 +824,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     const float3 tv = transmittance(J, J.T, r, mu, d, ground);
     const float w = static_cast<float>(((i == 0 || i == kMarch - 1) ? 0.5 : 1.0) * dx);
     MsRowSample& S = smp[slot][i];
-    S.d = d;
-    S.inv_r = 1.0 / r_i;
+    S.df = static_cast<float>(d);
+    S.inv_rf = static_cast<float>(1.0 / r_i);
     S.o00 = (tz.i0 * nmu + ty.i0) * W;
     S.o01 = (tz.i0 * nmu + ty.i1) * W;
     S.o10 = (tz.i1 * nmu + ty.i0) * W;
@@ -801,8 +843,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   // Per-texel constants: the texel's nu taps (fixed for all samples,
   // functions.glsl:967-973) and r mu_s.
   int slot_of[kTpt], x_of[kTpt], c0_of[kTpt], c1_of[kTpt];
-  float ws_of[kTpt];
-  double nu_of[kTpt], rmus_of[kTpt];
+  float ws_of[kTpt], nuf_of[kTpt], rmusf_of[kTpt];
+  double nu_of[kTpt];
   float acc[kTpt][3];
 #pragma unroll
   for (int j = 0; j < kTpt; ++j) {
@@ -812,8 +854,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     x_of[j] = f - slot * W;
     acc[j][0] = acc[j][1] = acc[j][2] = 0.0f;
     c0_of[j] = c1_of[j] = 0;
-    ws_of[j] = 0.0f;
-    nu_of[j] = rmus_of[j] = 0.0;
+    ws_of[j] = nuf_of[j] = rmusf_of[j] = 0.0f;
+    nu_of[j] = 0.0;
     if (slot_of[j] < 0) continue;
     const int row = row0 + slot;
     double r, rho, mu;
@@ -835,7 +877,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     c0_of[j] = slot * W + s0 * nmus;
     c1_of[j] = slot * W + s1 * nmus;
     nu_of[j] = nu;
-    rmus_of[j] = r * mu_s;
+    nuf_of[j] = static_cast<float>(nu);
+    rmusf_of[j] = static_cast<float>(r * mu_s);
   }
   __syncthreads();
 
@@ -856,10 +899,10 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
       if (slot_of[j] < 0) continue;
       const MsRowSample& S = smp[slot_of[j]][i];
       const int x = x_of[j];
-      src[j][0] = make3(gload(D, S.o00 + x));
-      src[j][1] = make3(gload(D, S.o01 + x));
-      src[j][2] = make3(gload(D, S.o10 + x));
-      src[j][3] = make3(gload(D, S.o11 + x));
+      src[j][0] = make3(gload32(D, static_cast<uint32_t>(S.o00 + x)));
+      src[j][1] = make3(gload32(D, static_cast<uint32_t>(S.o01 + x)));
+      src[j][2] = make3(gload32(D, static_cast<uint32_t>(S.o10 + x)));
+      src[j][3] = make3(gload32(D, static_cast<uint32_t>(S.o11 + x)));
     }
   };
   fetch(0);
@@ -887,8 +930,10 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     for (int j = 0; j < kTpt; ++j) {
       if (slot_of[j] < 0) continue;
       const MsRowSample& S = smp[slot_of[j]][i];
+      // mu_s at the sample, (r mu_s + d nu) / r_i (functions.glsl:1311):
+      // FP32 keeps it to ~1e-7 absolute, far below a texel.
       const float mus =
-          clampf(static_cast<float>(fma(S.d, nu_of[j], rmus_of[j]) * S.inv_r), -1.0f, 1.0f);
+          clampf(fmaf(S.df, nuf_of[j], rmusf_of[j]) * S.inv_rf, -1.0f, 1.0f);
       const float tt = rbf * mus;
       const float sq = sqrt_f(fmaf(tt, tt, H2));
       const float dd = tt > 0.0f ? H2 * rcp_f(sq + tt) : sq - tt;
@@ -1147,7 +1192,7 @@ int blocks_single_scattering(const Job& j) {
   return ((w + kBlock - 1) / kBlock) * j.nmu * j.nr;
 }
 int blocks_scattering_density(const Job& j) {
-  const int plane = j.nmu * j.nnu;
+  const int plane = j.nmu * ((j.nnu + kNuChunk - 1) / kNuChunk);
   return ((plane + kBlock - 1) / kBlock) * j.nmus * j.nr;
 }
 int blocks_indirect_irradiance(const Job& j) { return j.ew * j.eh; }

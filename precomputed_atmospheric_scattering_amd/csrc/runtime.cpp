@@ -414,8 +414,10 @@ atmo_status validate_sizes(const atmo_lut_sizes& z) {
     return fail(ATMO_ERROR_INVALID_ARGUMENT, "irradiance_width too large");
   const double texels = static_cast<double>(z.scattering_r) * z.scattering_mu *
                         z.scattering_mu_s * z.scattering_nu;
-  if (texels > 1.0e9)
-    return fail(ATMO_ERROR_INVALID_ARGUMENT, "scattering LUT too large");
+  // At most 2^28 texels (4 GiB in FP32 RGBA): kernels index texels with
+  // 32-bit byte offsets.
+  if (texels > 268435456.0)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "scattering LUT too large (> 2^28 texels)");
   return ATMO_OK;
 }
 

@@ -110,6 +110,9 @@ def test_illuminance_iterations():
     L().or_convert_spectrum_to_linear_srgb(len(wl), ol.ptr(wl), ol.ptr(np.ones(len(wl))),
                                            ol.ptr(rgb))
     assert np.allclose(total, rgb / 683.0, rtol=0.03)
+    # Each coefficient is a float value (model.cc:934 rounds them to float).
+    for _, m in its:
+        assert np.array_equal(m, m.astype(np.float32).astype(np.float64))
     # 3 wavelengths: identity matrix at 680/550/440 nm (model.cc:907-913).
     kw, _ = configs.model_kwargs(configs.earth())
     kw.pop("lut_sizes")
