@@ -103,6 +103,14 @@ struct Job {
   // FP32 in S / SM during Init, then stored as 4 x FP16 texels times a
   // per-channel power-of-two scale: S16[t].c = S[t].c * scale[c],
   // scales[0..3] for S16, scales[4..7] for SM16.
+  // Illuminance mode (several Precompute iterations): S and SM accumulate
+  // in FP64 (4 doubles per texel) over the iterations and orders, and are
+  // rounded to FP32 once at the end. Luminance channels can be small
+  // differences of the iterations' contributions (the XYZ -> sRGB matrix has
+  // negative entries), and FP32 partial sums lose them.
+  int acc64;
+  double* S64;
+  double* SM64;
   int half_luts;
   uint2* S16;
   uint2* SM16;
@@ -611,6 +619,29 @@ ATMO_HD float mie_phase(float g, float nu) {
   float k = 3.0f / (8.0f * 3.14159265358979f) * (1.0f - g * g) / (2.0f + g * g);
   float y = 1.0f + g * g - 2.0f * g * nu;
   return k * (1.0f + nu * nu) / (y * sqrtf(y));
+}
+
+// D4 texels of the FP64 accumulators (4 doubles per texel, 32 bytes).
+ATMO_HD D4 load_d4(const double* p, int64_t t) {
+  const double2 a = gload(reinterpret_cast<const double2*>(p), 2 * t);
+  const double2 b = gload(reinterpret_cast<const double2*>(p), 2 * t + 1);
+  return D4{a.x, a.y, b.x, b.y};
+}
+ATMO_HD void store_d4(double* p, int64_t t, D4 v) {
+  gstore(reinterpret_cast<double2*>(p), 2 * t, make_double2(v.x, v.y));
+  gstore(reinterpret_cast<double2*>(p), 2 * t + 1, make_double2(v.z, v.w));
+}
+ATMO_HD D4 add_d4(D4 a, D4 b) { return D4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+// luminance_from_radiance * v in FP64 (the products of FP32 factors are
+// exact in FP64).
+ATMO_HD D4 apply_lum_d(const float* L, const double* v) {
+  const double x = v[0], y = v[1], z = v[2];
+  return D4{L[0] * x + L[1] * y + L[2] * z, L[3] * x + L[4] * y + L[5] * z,
+            L[6] * x + L[7] * y + L[8] * z, 0.0};
+}
+ATMO_HD D4 apply_lum_d(const float* L, float3 v) {
+  const double w[3] = {v.x, v.y, v.z};
+  return apply_lum_d(L, w);
 }
 
 ATMO_HD float3 apply_lum(const float* L, float3 v) {

@@ -18,7 +18,11 @@ struct PassLaunch {
   int max_nnu;  // largest nu size among the jobs (LDS sizing)
   int max_ew;   // largest irradiance width among the jobs (LDS sizing)
   int max_w;    // largest packed row (nu x mu_s texels) among the jobs
+  int kinds;    // kJobsF32 | kJobsAcc64: which kinds of jobs the launch holds
 };
+
+constexpr int kJobsF32 = 1;    // jobs accumulating S and SM in FP32
+constexpr int kJobsAcc64 = 2;  // illuminance-mode jobs with FP64 accumulators
 
 constexpr int kBlock = 256;
 
@@ -57,6 +61,9 @@ hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s);
 // launch_to_half (blocks_half_conversion blocks per job) computes the maxima
 // and writes the scaled FP16 texels.
 hipError_t launch_clear_max(const PassLaunch& p, hipStream_t s);
+// Rounding of the FP64 accumulators of illuminance-mode jobs into S and SM
+// (blocks_half_conversion blocks per job).
+hipError_t launch_acc64_to_f32(const PassLaunch& p, hipStream_t s);
 hipError_t launch_to_half(const PassLaunch& p, hipStream_t s);
 
 // Lookups (one ray / point per thread). 'job' is a device pointer. With

@@ -7,6 +7,8 @@
 // batch of models with different LUT sizes. Blocks are 256 threads.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "atmo_device.h"
 #include "kernels.h"
 
@@ -180,27 +182,32 @@ __global__ __launch_bounds__(kBlock) void k_direct_irradiance(
 // ---------------------------------------------------------------------------
 constexpr int kMarch = 51;  // 50 trapezoid intervals
 
+// FP32 path (radiance mode) and FP64 path (illuminance mode, whose S and SM
+// accumulate in FP64): per-sample geometry in Real, the r mu_s + d nu term
+// and the horizon term of every texel in FP64 in both.
+template <class Real>
 struct SsSample {
-  float df;        // distance along the view ray
-  float inv_rf;    // 1 / r_d
-  float rhof;      // sqrt(r_d^2 - rb^2)
-  float rf;        // r_d
-  float c;         // rt^2 - r_d^2
-  float dmin;      // rt - r_d
-  float inv_range; // 1 / (rho + H - dmin)
-  float e;         // sin(theta_h) * sun angular radius
-  float inv_span;  // 1 / (2 e)
+  double d;        // distance along the view ray
+  double inv_r;    // 1 / r_d
+  double rho;      // sqrt(r_d^2 - rb^2)
+  Real r;          // r_d
+  Real c;          // rt^2 - r_d^2
+  Real dmin;       // rt - r_d
+  Real inv_range;  // 1 / (rho + H - dmin)
+  Real e;          // sin(theta_h) * sun angular radius
+  Real inv_span;   // 1 / (2 e)
+  Real wy;
   int t0, t1;      // first texels of the transmittance LUT rows of r_d
-  float wy;
-  float fr[3];     // trapezoid weight * view transmittance * Rayleigh density
-  float fm[3];     // same with the Mie density
+  Real fr[3];      // trapezoid weight * view transmittance * Rayleigh density
+  Real fm[3];      // same with the Mie density
 };
 
-__device__ __forceinline__ float3 tsun_lookup(const Job& J, const SsSample& S,
-                                              float mus) {
-  // functions.glsl:402-421 at (r_d, mu_s_d): DistanceToTopAtmosphereBoundary
-  // in the cancellation-free form, then the LUT column coordinate.
-  const float tt = S.rf * mus;
+// functions.glsl:402-421 at (r_d, mu_s_d): DistanceToTopAtmosphereBoundary
+// in the cancellation-free form, the LUT column coordinate, and the bilinear
+// transmittance lookup (functions.glsl:473-480).
+__device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<float>& S,
+                                            float mus, float o[3]) {
+  const float tt = S.r * mus;
   const float sq = sqrt_f(fmaf(tt, tt, S.c));
   const float dd = tt > 0.0f ? S.c * rcp_f(sq + tt) : sq - tt;
   const float xm = (dd - S.dmin) * S.inv_range;
@@ -215,19 +222,45 @@ __device__ __forceinline__ float3 tsun_lookup(const Job& J, const SsSample& S,
   const float4 a = gload32(T, r0 + i0), b = gload32(T, r0 + i1);
   const float4 c = gload32(T, r1 + i0), d = gload32(T, r1 + i1);
   const float wy1 = S.wy, wy0 = 1.0f - S.wy;
-  float3 o;
-  o.x = wy0 * fmaf(w, b.x - a.x, a.x) + wy1 * fmaf(w, d.x - c.x, c.x);
-  o.y = wy0 * fmaf(w, b.y - a.y, a.y) + wy1 * fmaf(w, d.y - c.y, c.y);
-  o.z = wy0 * fmaf(w, b.z - a.z, a.z) + wy1 * fmaf(w, d.z - c.z, c.z);
-  return o;
+  o[0] = wy0 * fmaf(w, b.x - a.x, a.x) + wy1 * fmaf(w, d.x - c.x, c.x);
+  o[1] = wy0 * fmaf(w, b.y - a.y, a.y) + wy1 * fmaf(w, d.y - c.y, c.y);
+  o[2] = wy0 * fmaf(w, b.z - a.z, a.z) + wy1 * fmaf(w, d.z - c.z, c.z);
 }
 
+__device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<double>& S,
+                                            double mus, double o[3]) {
+  const double tt = S.r * mus;
+  const double sq = sqrt(fma(tt, tt, S.c));
+  const double dd = tt > 0.0 ? S.c / (sq + tt) : sq - tt;
+  double ut = (dd - S.dmin) * S.inv_range * (J.tw - 1);
+  ut = fmin(fmax(ut, 0.0), static_cast<double>(J.tw - 1));
+  const double fl = floor(ut);
+  const double w = ut - fl;
+  const int i0 = static_cast<int>(fl);
+  const int i1 = min(i0 + 1, J.tw - 1);
+  const float4* __restrict__ T = J.T;
+  const uint32_t r0 = static_cast<uint32_t>(S.t0), r1 = static_cast<uint32_t>(S.t1);
+  const float4 a = gload32(T, r0 + i0), b = gload32(T, r0 + i1);
+  const float4 c = gload32(T, r1 + i0), d = gload32(T, r1 + i1);
+  const double wy1 = S.wy, wy0 = 1.0 - S.wy;
+  o[0] = wy0 * fma(w, double(b.x) - a.x, double(a.x)) + wy1 * fma(w, double(d.x) - c.x, double(c.x));
+  o[1] = wy0 * fma(w, double(b.y) - a.y, double(a.y)) + wy1 * fma(w, double(d.y) - c.y, double(c.y));
+  o[2] = wy0 * fma(w, double(b.z) - a.z, double(a.z)) + wy1 * fma(w, double(d.z) - c.z, double(c.z));
+}
+
+// kF64: the kernel serves the jobs with FP64 accumulators (illuminance
+// mode) and computes the sun-side lookups and the sums in FP64; otherwise it
+// serves the others. Both are launched over the same blocks when a launch
+// holds both kinds; each returns at once on the other's jobs.
+template <bool kF64>
 __global__ __launch_bounds__(kBlock) void k_single_scattering(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
     int njobs) {
-  __shared__ SsSample smp[kMarch];
+  using Real = typename std::conditional<kF64, double, float>::type;
+  __shared__ SsSample<Real> smp[kMarch];
   const int ji = find_job(first_block, njobs, blockIdx.x);
   const Job& J = jobs[ji];
+  if ((J.acc64 != 0) != kF64) return;
   const int lb = blockIdx.x - first_block[ji];
   const int W = J.nnu * J.nmus;
   const int tiles = (W + kBlock - 1) / kBlock;
@@ -243,34 +276,50 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
   const double dx = dist_nearest(J, r, mu, ground) / (kMarch - 1);
 
   if (tid < kMarch) {
-    SsSample& S = smp[tid];
+    SsSample<Real>& S = smp[tid];
     const double d = tid * dx;
     const double r_d = clamp_radius(J, sqrt(d * d + 2.0 * r * mu * d + r * r));
-    const float3 tv = transmittance(J, J.T, r, mu, d, ground);
+    double tv[3];
+    if (kF64) {
+      const D4 t = transmittance_d(J, J.T, r, mu, d, ground);
+      tv[0] = t.x;
+      tv[1] = t.y;
+      tv[2] = t.z;
+    } else {
+      const float3 t = transmittance(J, J.T, r, mu, d, ground);
+      tv[0] = t.x;
+      tv[1] = t.y;
+      tv[2] = t.z;
+    }
     const double rho_d = safe_sqrt(r_d * r_d - J.rb * J.rb);
-    S.df = static_cast<float>(d);
-    S.inv_rf = static_cast<float>(1.0 / r_d);
-    S.rhof = static_cast<float>(rho_d);
-    S.rf = static_cast<float>(r_d);
-    S.c = static_cast<float>(J.rt * J.rt - r_d * r_d);
-    S.dmin = static_cast<float>(J.rt - r_d);
-    S.inv_range = static_cast<float>(1.0 / (rho_d + J.H - (J.rt - r_d)));
+    S.d = d;
+    S.inv_r = 1.0 / r_d;
+    S.rho = rho_d;
+    S.r = static_cast<Real>(r_d);
+    S.c = static_cast<Real>(J.rt * J.rt - r_d * r_d);
+    S.dmin = static_cast<Real>(J.rt - r_d);
+    S.inv_range = static_cast<Real>(1.0 / (rho_d + J.H - (J.rt - r_d)));
     const double e = J.rb / r_d * J.sun_angular_radius;
-    S.e = static_cast<float>(e);
-    S.inv_span = static_cast<float>(0.5 / e);
-    const Taps ty = axis_taps(unit_to_texel(rho_d / J.H, J.th), J.th);
-    S.t0 = ty.i0 * J.tw;
-    S.t1 = ty.i1 * J.tw;
-    S.wy = ty.w;
+    S.e = static_cast<Real>(e);
+    S.inv_span = static_cast<Real>(0.5 / e);
+    const double ty = unit_to_texel(rho_d / J.H, J.th);
+    const Taps tt = axis_taps(ty, J.th);
+    S.t0 = tt.i0 * J.tw;
+    S.t1 = tt.i1 * J.tw;
+    if (kF64) {
+      // The exact weight (Taps rounds it to float).
+      const double cy = fmin(fmax(ty, -1.0), static_cast<double>(J.th));
+      S.wy = static_cast<Real>(cy - floor(cy));
+    } else {
+      S.wy = static_cast<Real>(tt.w);
+    }
     const double w = (tid == 0 || tid == kMarch - 1) ? 0.5 : 1.0;
-    const float dr = static_cast<float>(w * profile_density(J.prof_r, r_d - J.rb));
-    const float dm = static_cast<float>(w * profile_density(J.prof_m, r_d - J.rb));
-    S.fr[0] = dr * tv.x;
-    S.fr[1] = dr * tv.y;
-    S.fr[2] = dr * tv.z;
-    S.fm[0] = dm * tv.x;
-    S.fm[1] = dm * tv.y;
-    S.fm[2] = dm * tv.z;
+    const double dr = w * profile_density(J.prof_r, r_d - J.rb);
+    const double dm = w * profile_density(J.prof_m, r_d - J.rb);
+    for (int c = 0; c < 3; ++c) {
+      S.fr[c] = static_cast<Real>(dr * tv[c]);
+      S.fm[c] = static_cast<Real>(dm * tv[c]);
+    }
   }
   __syncthreads();
 
@@ -278,48 +327,63 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
   if (x >= W) return;
   const double mu_s = s_col_mus(J, x % J.nmus);
   const double nu = s_slice_nu(J, x / J.nmus, mu, mu_s);
-  // r mu_s + d nu = r_d mu_s_d in FP32: its absolute error (~5e-4 length
-  // units at r ~ 6400) divided by r_d moves mu_s_d by ~1e-7, far below a
-  // texel and below the visible-disc smoothstep width (2 rb / r_d times the
-  // sun angular radius).
-  const float rmus = static_cast<float>(r * mu_s), nuf = static_cast<float>(nu);
-  float ar0 = 0.f, ar1 = 0.f, ar2 = 0.f, am0 = 0.f, am1 = 0.f, am2 = 0.f;
+  const double rmus = r * mu_s;
+  Real ar[3] = {0, 0, 0}, am[3] = {0, 0, 0};
   for (int s = 0; s < kMarch; ++s) {
-    const SsSample& S = smp[s];
-    const float t = fmaf(S.df, nuf, rmus);
-    const float xs = (t + S.rhof) * S.inv_rf;  // mu_s_d - cos(theta_h)
-    const float mus = clampf(t * S.inv_rf, -1.0f, 1.0f);
-    const float u = clampf((xs + S.e) * S.inv_span, 0.0f, 1.0f);
-    const float vis = u * u * (3.0f - 2.0f * u);
-    const float3 ts = tsun_lookup(J, S, mus);
-    const float f0 = ts.x * vis, f1 = ts.y * vis, f2 = ts.z * vis;
-    ar0 = fmaf(f0, S.fr[0], ar0);
-    ar1 = fmaf(f1, S.fr[1], ar1);
-    ar2 = fmaf(f2, S.fr[2], ar2);
-    am0 = fmaf(f0, S.fm[0], am0);
-    am1 = fmaf(f1, S.fm[1], am1);
-    am2 = fmaf(f2, S.fm[2], am2);
+    const SsSample<Real>& S = smp[s];
+    const double t = fma(S.d, nu, rmus);  // r * mu_s + d * nu = r_d * mu_s_d
+    const Real xs = static_cast<Real>((t + S.rho) * S.inv_r);  // mu_s_d - cos(theta_h)
+    Real mus = static_cast<Real>(t * S.inv_r);
+    mus = mus < Real(-1) ? Real(-1) : (mus > Real(1) ? Real(1) : mus);
+    Real u = (xs + S.e) * S.inv_span;
+    u = u < Real(0) ? Real(0) : (u > Real(1) ? Real(1) : u);
+    const Real vis = u * u * (Real(3) - Real(2) * u);
+    Real ts[3];
+    tsun_lookup(J, S, mus, ts);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const Real f = ts[c] * vis;
+      ar[c] += f * S.fr[c];
+      am[c] += f * S.fm[c];
+    }
   }
-  const float3 ray = make_float3(ar0 * static_cast<float>(dx * J.solar[0] * J.rayleigh[0]),
-                                 ar1 * static_cast<float>(dx * J.solar[1] * J.rayleigh[1]),
-                                 ar2 * static_cast<float>(dx * J.solar[2] * J.rayleigh[2]));
-  const float3 mie = make_float3(am0 * static_cast<float>(dx * J.solar[0] * J.mie_sca[0]),
-                                 am1 * static_cast<float>(dx * J.solar[1] * J.mie_sca[1]),
-                                 am2 * static_cast<float>(dx * J.solar[2] * J.mie_sca[2]));
+  double ray[3], mie[3];
+  for (int c = 0; c < 3; ++c) {
+    ray[c] = static_cast<double>(ar[c]) * (dx * J.solar[c] * J.rayleigh[c]);
+    mie[c] = static_cast<double>(am[c]) * (dx * J.solar[c] * J.mie_sca[c]);
+  }
   const int64_t t_idx = static_cast<int64_t>(row) * W + x;
-  gstore(J.dR, t_idx, make_float4(ray.x, ray.y, ray.z, 0.0f));
-  gstore(J.dM, t_idx, make_float4(mie.x, mie.y, mie.z, 0.0f));
-  const float3 lr = apply_lum(J.lum, ray);
-  const float3 lm = apply_lum(J.lum, mie);
-  float4 s = make_float4(lr.x, lr.y, lr.z, lm.x);
+  gstore(J.dR, t_idx, make_float4(static_cast<float>(ray[0]), static_cast<float>(ray[1]),
+                                  static_cast<float>(ray[2]), 0.0f));
+  gstore(J.dM, t_idx, make_float4(static_cast<float>(mie[0]), static_cast<float>(mie[1]),
+                                  static_cast<float>(mie[2]), 0.0f));
+  if (kF64) {
+    const D4 lr = apply_lum_d(J.lum, ray), lm = apply_lum_d(J.lum, mie);
+    D4 sv{lr.x, lr.y, lr.z, lm.x};
+    if (J.blend) sv = add_d4(sv, load_d4(J.S64, t_idx));
+    store_d4(J.S64, t_idx, sv);
+    if (!J.combined) {
+      D4 m{lm.x, lm.y, lm.z, 0.0};
+      if (J.blend) m = add_d4(m, load_d4(J.SM64, t_idx));
+      store_d4(J.SM64, t_idx, m);
+    }
+    return;
+  }
+  const float3 rayf = make_float3(static_cast<float>(ray[0]), static_cast<float>(ray[1]),
+                                  static_cast<float>(ray[2]));
+  const float3 mief = make_float3(static_cast<float>(mie[0]), static_cast<float>(mie[1]),
+                                  static_cast<float>(mie[2]));
+  const float3 lr = apply_lum(J.lum, rayf);
+  const float3 lm = apply_lum(J.lum, mief);
+  float4 sv = make_float4(lr.x, lr.y, lr.z, lm.x);
   if (J.blend) {
     const float4 o = gload(J.S, t_idx);
-    s.x += o.x;
-    s.y += o.y;
-    s.z += o.z;
-    s.w += o.w;
+    sv.x += o.x;
+    sv.y += o.y;
+    sv.z += o.z;
+    sv.w += o.w;
   }
-  gstore(J.S, t_idx, s);
+  gstore(J.S, t_idx, sv);
   if (!J.combined) {
     float4 m = make_float4(lm.x, lm.y, lm.z, 0.0f);
     if (J.blend) {
@@ -776,8 +840,8 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
 // depends on its own row length only.
 // ---------------------------------------------------------------------------
 struct MsRowSample {
-  float df;      // distance along the view ray
-  float inv_rf;  // 1 / r_i
+  double d;      // distance along the view ray
+  double inv_r;  // 1 / r_i
   int o00, o01, o10, o11;  // texel offsets of the (r, mu) source rows
   float w00, w01, w10, w11;
   float f[3];    // trapezoid weight * dx * view transmittance
@@ -824,8 +888,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     const float3 tv = transmittance(J, J.T, r, mu, d, ground);
     const float w = static_cast<float>(((i == 0 || i == kMarch - 1) ? 0.5 : 1.0) * dx);
     MsRowSample& S = smp[slot][i];
-    S.df = static_cast<float>(d);
-    S.inv_rf = static_cast<float>(1.0 / r_i);
+    S.d = d;
+    S.inv_r = 1.0 / r_i;
     S.o00 = (tz.i0 * nmu + ty.i0) * W;
     S.o01 = (tz.i0 * nmu + ty.i1) * W;
     S.o10 = (tz.i1 * nmu + ty.i0) * W;
@@ -843,8 +907,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   // Per-texel constants: the texel's nu taps (fixed for all samples,
   // functions.glsl:967-973) and r mu_s.
   int slot_of[kTpt], x_of[kTpt], c0_of[kTpt], c1_of[kTpt];
-  float ws_of[kTpt], nuf_of[kTpt], rmusf_of[kTpt];
-  double nu_of[kTpt];
+  float ws_of[kTpt];
+  double nu_of[kTpt], rmus_of[kTpt];
   float acc[kTpt][3];
 #pragma unroll
   for (int j = 0; j < kTpt; ++j) {
@@ -854,8 +918,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     x_of[j] = f - slot * W;
     acc[j][0] = acc[j][1] = acc[j][2] = 0.0f;
     c0_of[j] = c1_of[j] = 0;
-    ws_of[j] = nuf_of[j] = rmusf_of[j] = 0.0f;
-    nu_of[j] = 0.0;
+    ws_of[j] = 0.0f;
+    nu_of[j] = rmus_of[j] = 0.0;
     if (slot_of[j] < 0) continue;
     const int row = row0 + slot;
     double r, rho, mu;
@@ -877,8 +941,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     c0_of[j] = slot * W + s0 * nmus;
     c1_of[j] = slot * W + s1 * nmus;
     nu_of[j] = nu;
-    nuf_of[j] = static_cast<float>(nu);
-    rmusf_of[j] = static_cast<float>(r * mu_s);
+    rmus_of[j] = r * mu_s;
   }
   __syncthreads();
 
@@ -930,10 +993,9 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     for (int j = 0; j < kTpt; ++j) {
       if (slot_of[j] < 0) continue;
       const MsRowSample& S = smp[slot_of[j]][i];
-      // mu_s at the sample, (r mu_s + d nu) / r_i (functions.glsl:1311):
-      // FP32 keeps it to ~1e-7 absolute, far below a texel.
+      // mu_s at the sample, (r mu_s + d nu) / r_i (functions.glsl:1311).
       const float mus =
-          clampf(fmaf(S.df, nuf_of[j], rmusf_of[j]) * S.inv_rf, -1.0f, 1.0f);
+          clampf(static_cast<float>(fma(S.d, nu_of[j], rmus_of[j]) * S.inv_r), -1.0f, 1.0f);
       const float tt = rbf * mus;
       const float sq = sqrt_f(fmaf(tt, tt, H2));
       const float dd = tt > 0.0f ? H2 * rcp_f(sq + tt) : sq - tt;
@@ -961,8 +1023,18 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     if (slot_of[j] < 0) continue;
     const int64_t t_idx = static_cast<int64_t>(row0 + slot_of[j]) * W + x_of[j];
     gstore(J.dR, t_idx, make_float4(acc[j][0], acc[j][1], acc[j][2], 0.0f));
-    const float3 l = apply_lum(J.lum, make_float3(acc[j][0], acc[j][1], acc[j][2]));
     const double nu = nu_of[j];
+    if (J.acc64) {
+      const double k = 1.0 / ((3.0 / (16.0 * kPi)) * (1.0 + nu * nu));
+      const D4 l = apply_lum_d(J.lum, make_float3(acc[j][0], acc[j][1], acc[j][2]));
+      D4 sv = load_d4(J.S64, t_idx);
+      sv.x += l.x * k;
+      sv.y += l.y * k;
+      sv.z += l.z * k;
+      store_d4(J.S64, t_idx, sv);
+      continue;
+    }
+    const float3 l = apply_lum(J.lum, make_float3(acc[j][0], acc[j][1], acc[j][2]));
     const float inv_pr = static_cast<float>(1.0 / ((3.0 / (16.0 * kPi)) * (1.0 + nu * nu)));
     float4 sv = gload(J.S, t_idx);
     sv.x += l.x * inv_pr;
@@ -1095,6 +1167,16 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
   }
   const int64_t t_idx = static_cast<int64_t>(row) * W + x;
   gstore(J.dR, t_idx, make_float4(a0, a1, a2, 0.0f));
+  if (J.acc64) {
+    const double k = 1.0 / ((3.0 / (16.0 * kPi)) * (1.0 + nu * nu));
+    const D4 l = apply_lum_d(J.lum, make_float3(a0, a1, a2));
+    D4 sv = load_d4(J.S64, t_idx);
+    sv.x += l.x * k;
+    sv.y += l.y * k;
+    sv.z += l.z * k;
+    store_d4(J.S64, t_idx, sv);
+    return;
+  }
   const float3 l = apply_lum(J.lum, make_float3(a0, a1, a2));
   const float inv_pr = static_cast<float>(1.0 / ((3.0 / (16.0 * kPi)) * (1.0 + nu * nu)));
   float4 sv = gload(J.S, t_idx);
@@ -1102,6 +1184,28 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
   sv.y += l.y * inv_pr;
   sv.z += l.z * inv_pr;
   gstore(J.S, t_idx, sv);
+}
+
+// ---------------------------------------------------------------------------
+// Illuminance mode: round the FP64 accumulators into S and SM (FP32; in
+// FP16 mode these are the accumulators that k_to_half then converts).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_acc64_to_f32(const Job* __restrict__ jobs,
+                                                         const int* __restrict__ first_block,
+                                                         int njobs) {
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int64_t n = static_cast<int64_t>(J.nr) * J.nmu * J.nmus * J.nnu;
+  const int64_t t = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const D4 s = load_d4(J.S64, t);
+  gstore(J.S, t, make_float4(static_cast<float>(s.x), static_cast<float>(s.y),
+                             static_cast<float>(s.z), static_cast<float>(s.w)));
+  if (!J.combined) {
+    const D4 m = load_d4(J.SM64, t);
+    gstore(J.SM, t, make_float4(static_cast<float>(m.x), static_cast<float>(m.y),
+                                static_cast<float>(m.z), 0.0f));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1219,8 +1323,12 @@ hipError_t launch_direct_irradiance(const PassLaunch& p, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_single_scattering(const PassLaunch& p, hipStream_t s) {
-  hipLaunchKernelGGL(k_single_scattering, dim3(p.total_blocks), dim3(kBlock), 0, s,
-                     p.jobs, p.first_block, p.njobs);
+  if (p.kinds & kJobsF32)
+    hipLaunchKernelGGL(k_single_scattering<false>, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                       p.jobs, p.first_block, p.njobs);
+  if (p.kinds & kJobsAcc64)
+    hipLaunchKernelGGL(k_single_scattering<true>, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                       p.jobs, p.first_block, p.njobs);
   return hipGetLastError();
 }
 hipError_t launch_scattering_density(const PassLaunch& p, int order,
@@ -1238,6 +1346,11 @@ hipError_t launch_indirect_irradiance(const PassLaunch& p, int order,
       sizeof(float4) * static_cast<size_t>((order == 1 ? 4 : 2) * kTheta * p.max_nnu);
   hipLaunchKernelGGL(k_indirect_irradiance, dim3(p.total_blocks), dim3(64), lds, s,
                      p.jobs, p.first_block, p.njobs, order);
+  return hipGetLastError();
+}
+hipError_t launch_acc64_to_f32(const PassLaunch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_acc64_to_f32, dim3(p.total_blocks), dim3(kBlock), 0, s, p.jobs,
+                     p.first_block, p.njobs);
   return hipGetLastError();
 }
 hipError_t launch_clear_max(const PassLaunch& p, hipStream_t s) {

@@ -183,6 +183,7 @@ int pass_blocks(Pass p, const atmo::Job& j) {
     case kII: return atmo::blocks_indirect_irradiance(j);
     case kMS: return atmo::blocks_multiple_scattering(j);
     case kHalf: return atmo::blocks_half_conversion(j);
+    case kAcc: return atmo::blocks_half_conversion(j);
     case kClear: return 1;
     default: return 0;
   }
@@ -211,6 +212,12 @@ void free_temporaries(atmo_model* m) {
       *b = nullptr;
     }
   }
+  for (double** b : {&m->S64, &m->SM64}) {
+    if (*b != nullptr) {
+      (void)hipFree(*b);
+      *b = nullptr;
+    }
+  }
 }
 
 atmo_status alloc_temporaries(atmo_model* m) {
@@ -224,6 +231,11 @@ atmo_status alloc_temporaries(atmo_model* m) {
     if (m->SM == nullptr && !m->spec.combined)
       ATMO_HIP(hipMalloc(&m->SM, z.s_texels() * sizeof(float4)));
   }
+  if (accumulates_fp64(m->spec)) {
+    if (m->S64 == nullptr) ATMO_HIP(hipMalloc(&m->S64, z.s_texels() * 4 * sizeof(double)));
+    if (m->SM64 == nullptr && !m->spec.combined)
+      ATMO_HIP(hipMalloc(&m->SM64, z.s_texels() * 4 * sizeof(double)));
+  }
   return ATMO_OK;
 }
 
@@ -236,6 +248,8 @@ size_t temporaries_bytes(const atmo_model* m) {
              3 * round256(z.s_texels() * sizeof(float4));
   if (m->spec.half_precision)
     b += (m->spec.combined ? 1 : 2) * round256(z.s_texels() * sizeof(float4));
+  if (accumulates_fp64(m->spec))
+    b += (m->spec.combined ? 1 : 2) * round256(z.s_texels() * 4 * sizeof(double));
   return b;
 }
 
@@ -247,9 +261,20 @@ void use_external_temporaries(atmo_model* m, char* base) {
   m->dR = reinterpret_cast<float4*>(base + es);
   m->dM = reinterpret_cast<float4*>(base + es + ss);
   m->dJ = reinterpret_cast<float4*>(base + es + 2 * ss);
+  size_t off = es + 3 * ss;
   if (m->spec.half_precision) {
-    m->S = reinterpret_cast<float4*>(base + es + 3 * ss);
-    if (!m->spec.combined) m->SM = reinterpret_cast<float4*>(base + es + 4 * ss);
+    m->S = reinterpret_cast<float4*>(base + off);
+    off += ss;
+    if (!m->spec.combined) {
+      m->SM = reinterpret_cast<float4*>(base + off);
+      off += ss;
+    }
+  }
+  if (accumulates_fp64(m->spec)) {
+    const size_t ds = round256(z.s_texels() * 4 * sizeof(double));
+    m->S64 = reinterpret_cast<double*>(base + off);
+    off += ds;
+    if (!m->spec.combined) m->SM64 = reinterpret_cast<double*>(base + off);
   }
   m->external_temporaries = true;
 }
@@ -257,6 +282,7 @@ void use_external_temporaries(atmo_model* m, char* base) {
 void drop_external_temporaries(atmo_model* m) {
   m->dE = m->dR = m->dM = m->dJ = nullptr;
   if (m->spec.half_precision) m->S = m->SM = nullptr;
+  m->S64 = m->SM64 = nullptr;
   m->external_temporaries = false;
 }
 
@@ -274,6 +300,9 @@ void fill_job_luts(atmo_model* m, atmo::Job* j) {
   j->half_luts = m->spec.half_precision ? 1 : 0;
   j->S16 = m->S16;
   j->SM16 = m->SM16;
+  j->acc64 = accumulates_fp64(m->spec) ? 1 : 0;
+  j->S64 = m->S64;
+  j->SM64 = m->SM64;
   j->scales = m->d_scales;
   j->maxbits = m->d_maxbits;
   for (int c = 0; c < 3; ++c) {
@@ -320,6 +349,7 @@ atmo::PassLaunch single_launch(const atmo_model* m, int job_index, Pass p) {
   l.max_nnu = m->sizes.nnu;
   l.max_ew = m->sizes.ew;
   l.max_w = m->sizes.nnu * m->sizes.nmus;
+  l.kinds = accumulates_fp64(m->spec) ? atmo::kJobsAcc64 : atmo::kJobsF32;
   return l;
 }
 
@@ -618,6 +648,10 @@ atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
   m->timed.clear();
   LaunchTimer timer{m, s};
   for (int i = 0; i < iters; ++i) ATMO_TRY(enqueue_precompute(m, i, orders, timer));
+  if (accumulates_fp64(m->spec)) {
+    ATMO_TRY(check_launch(atmo::launch_acc64_to_f32(single_launch(m, iters - 1, kAcc), s),
+                          "fp64 accumulators"));
+  }
   if (m->spec.num_precomputed_wavelengths > 3) {
     ATMO_TRY(timer.run(kT, 0, "transmittance (rgb)", [&] {
       return atmo::launch_transmittance(single_launch(m, iters, kT), s);
@@ -734,8 +768,8 @@ atmo_status atmo_model_footprint(const atmo_model* m, size_t* resident, size_t* 
   const Sizes& z = m->sizes;
   if (resident != nullptr) *resident = lut_bytes(m, !m->spec.combined);
   if (scratch != nullptr) {
-    size_t acc = m->spec.half_precision ? (m->spec.combined ? 1 : 2) : 0;
-    *scratch = ((3 + acc) * z.s_texels() + z.e_texels()) * sizeof(float4);
+    (void)z;
+    *scratch = temporaries_bytes(m);
   }
   return ATMO_OK;
 }
@@ -1067,6 +1101,7 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
     l.max_nnu = 0;
     l.max_ew = 0;
     l.max_w = 0;
+    l.kinds = 0;
     int total = 0;
     for (const auto& mj : sel) {
       const atmo::Job& j = models[mj.first]->jobs[mj.second];
@@ -1076,6 +1111,7 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
       l.max_nnu = std::max(l.max_nnu, j.nnu);
       l.max_ew = std::max(l.max_ew, j.ew);
       l.max_w = std::max(l.max_w, j.nnu * j.nmus);
+      l.kinds |= j.acc64 ? atmo::kJobsAcc64 : atmo::kJobsF32;
     }
     plan->first_block.push_back(total);
     l.total_blocks = total;
@@ -1103,13 +1139,16 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
       add_level(kMS, 0, on);
     }
   }
-  std::vector<std::pair<int, int>> rgb, half;
+  std::vector<std::pair<int, int>> rgb, half, acc;
   for (int i = 0; i < count; ++i) {
     if (models[i]->spec.num_precomputed_wavelengths > 3)
       rgb.push_back({i, num_iterations(models[i]->spec)});
     if (models[i]->spec.half_precision)
       half.push_back({i, num_iterations(models[i]->spec) - 1});
+    if (accumulates_fp64(models[i]->spec))
+      acc.push_back({i, num_iterations(models[i]->spec) - 1});
   }
+  add_level(kAcc, 0, acc);
   add_level(kT, 0, rgb);
   add_level(kClear, 0, half);
   add_level(kHalf, 0, half);
@@ -1126,6 +1165,7 @@ atmo_status enqueue_levels(const LevelPlan& plan, const atmo::Job* d_jobs,
     pl.max_nnu = l.max_nnu;
     pl.max_ew = l.max_ew;
     pl.max_w = l.max_w;
+    pl.kinds = l.kinds;
     hipError_t e = hipSuccess;
     switch (l.pass) {
       case kT: e = atmo::launch_transmittance(pl, s); break;
@@ -1135,6 +1175,7 @@ atmo_status enqueue_levels(const LevelPlan& plan, const atmo::Job* d_jobs,
       case kII: e = atmo::launch_indirect_irradiance(pl, l.order, s); break;
       case kMS: e = atmo::launch_multiple_scattering(pl, s); break;
       case kClear: e = atmo::launch_clear_max(pl, s); break;
+      case kAcc: e = atmo::launch_acc64_to_f32(pl, s); break;
       case kHalf: e = atmo::launch_to_half(pl, s); break;
       default: break;
     }

@@ -37,7 +37,7 @@ struct Sizes {
 };
 
 
-enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kHalf, kClear, kPasses };
+enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kAcc, kHalf, kClear, kPasses };
 
 
 // One launch of a batched pass sequence: pass kind, order argument, and the
@@ -45,7 +45,7 @@ enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kHalf, kClear, kPasses };
 struct Level {
   Pass pass;
   int order;  // SD: order computed; II: order read; others unused
-  int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew, max_w;
+  int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew, max_w, kinds;
 };
 
 // The launch sequence of several models' Init (see plan_levels).
@@ -74,6 +74,8 @@ struct atmo_model {
   // FP16 storage: resident half LUTs; S and SM above are then FP32
   // accumulators that live only during Init, like the deltas.
   uint2 *S16 = nullptr, *SM16 = nullptr;
+  // FP64 accumulators of S and SM in illuminance mode (Init temporaries).
+  double *S64 = nullptr, *SM64 = nullptr;
   float* d_scales = nullptr;
   unsigned* d_maxbits = nullptr;
   // Device job table: [iterations..., rgb transmittance job, query job].
@@ -108,6 +110,8 @@ atmo_status check_launch(hipError_t e, const char* what);
 atmo_status validate_params(const atmo_model_params* p);
 atmo_status validate_sizes(const atmo_lut_sizes& z);
 int num_iterations(const Spec& s);
+// Whether Init accumulates S and SM in FP64 (illuminance mode).
+inline bool accumulates_fp64(const Spec& s) { return s.num_precomputed_wavelengths > 3; }
 int pass_blocks(Pass p, const atmo::Job& j);
 // Rebuild a model's job table for 'orders' and copy it to the device on
 // 'stream' (asynchronous with respect to other streams).

@@ -116,9 +116,11 @@ struct InflightLookup {
 
 size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
-// Resident and scratch bytes of a model with these sizes and storage.
-void footprint(const atmo_lut_sizes& z, bool half, bool combined, size_t* resident,
-               size_t* scratch) {
+// Resident and scratch bytes of a model with these sizes and storage (the
+// same layout as atmo_model_footprint); acc64: illuminance mode, whose Init
+// accumulates S and SM in FP64.
+void footprint(const atmo_lut_sizes& z, bool half, bool combined, bool acc64,
+               size_t* resident, size_t* scratch) {
   const size_t nt = static_cast<size_t>(z.transmittance_width) * z.transmittance_height;
   const size_t ne = static_cast<size_t>(z.irradiance_width) * z.irradiance_height;
   const size_t ns = static_cast<size_t>(z.scattering_r) * z.scattering_mu *
@@ -127,7 +129,8 @@ void footprint(const atmo_lut_sizes& z, bool half, bool combined, size_t* reside
   *resident = round256(nt * 16) + round256(ne * 16) +
               (combined ? 1 : 2) * round256(ns * texel);
   *scratch = round256(ne * 16) + 3 * round256(ns * 16) +
-             (half ? (combined ? 1 : 2) * round256(ns * 16) : 0);
+             (half ? (combined ? 1 : 2) * round256(ns * 16) : 0) +
+             (acc64 ? (combined ? 1 : 2) * round256(ns * 32) : 0);
 }
 
 }  // namespace
@@ -653,7 +656,7 @@ atmo_status atmo_server_submit_precompute(atmo_server* s, const atmo_model_param
   r.sizes = z;
   r.orders = orders;
   footprint(z, params->half_precision != 0, params->combine_scattering_textures != 0,
-            &r.resident, &r.scratch);
+            params->num_precomputed_wavelengths > 3, &r.resident, &r.scratch);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     r.id = s->next_id++;
