@@ -197,6 +197,26 @@ atmo_status atmo_model_copy_lut(const atmo_model* model, atmo_lut_id id,
 atmo_status atmo_model_footprint(const atmo_model* model, size_t* resident,
                                  size_t* init_scratch);
 
+/* LUT files: raw float32 RGBA texels, x fastest, no header, decoded to
+ * physical values - the format the reference's WebGL exporter writes with
+ * glGetTexImage (atmosphere/demo/webgl/precompute.cc:63-106) - named
+ * transmittance.dat, scattering.dat, single_mie_scattering.dat (separate
+ * Mie only) and irradiance.dat in 'directory', as the reference CPU model's
+ * cache (atmosphere/reference/model.cc:74-84, :240-244). Export waits for
+ * the device. */
+atmo_status atmo_model_export_luts(const atmo_model* model,
+                                   const char* directory);
+/* Loads the files into the model's LUTs (their sizes must match the
+ * model's; FP16 models re-encode with the same per-channel scale rule as
+ * Init). The model is then initialised, as after Init. */
+atmo_status atmo_model_import_luts(atmo_model* model, const char* directory);
+/* reference::Model::Init with a cache directory
+ * (atmosphere/reference/model.cc:74-84): loads the LUTs if
+ * directory/transmittance.dat exists, else runs Init and saves them. */
+atmo_status atmo_model_init_cached(atmo_model* model,
+                                   unsigned num_scattering_orders,
+                                   const char* directory, void* stream);
+
 /* GetSolarRadiance (atmosphere/model.cc:227-230): host-side, 3 floats. */
 atmo_status atmo_get_solar_radiance(const atmo_model* model, float* rgb);
 /* GetSkyRadiance (atmosphere/functions.glsl:1705-1769, wrapped at

@@ -301,6 +301,25 @@ class Model:
         return out[0] if v.depth == 1 and name in (
             "transmittance", "irradiance", "delta_irradiance") else out
 
+    def export_luts(self, directory):
+        """Write transmittance.dat, scattering.dat, single_mie_scattering.dat
+        (separate Mie only) and irradiance.dat (raw float32 RGBA)."""
+        check(lib().atmo_model_export_luts(self._handle, str(directory).encode()),
+              "export_luts")
+
+    def import_luts(self, directory):
+        """Load LUT files written by export_luts (or the reference's WebGL
+        exporter); the model is then initialised."""
+        check(lib().atmo_model_import_luts(self._handle, str(directory).encode()),
+              "import_luts")
+
+    def init_cached(self, num_scattering_orders, directory, stream=None):
+        """Init, or load the LUTs from 'directory' when it holds them (the
+        reference CPU model's cache)."""
+        check(lib().atmo_model_init_cached(self._handle, int(num_scattering_orders),
+                                           str(directory).encode(), _stream_handle(stream)),
+              "init_cached")
+
     def footprint(self):
         a, b = C.c_size_t(), C.c_size_t()
         check(lib().atmo_model_footprint(self._handle, C.byref(a), C.byref(b)), "footprint")

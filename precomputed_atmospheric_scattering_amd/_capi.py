@@ -132,6 +132,9 @@ SIGNATURES = {
     "atmo_model_get_lut": (ST, [VP, C.c_int, C.POINTER(LutView)]),
     "atmo_model_copy_lut": (ST, [VP, C.c_int, FP]),
     "atmo_model_footprint": (ST, [VP, C.POINTER(SZ), C.POINTER(SZ)]),
+    "atmo_model_export_luts": (ST, [VP, C.c_char_p]),
+    "atmo_model_import_luts": (ST, [VP, C.c_char_p]),
+    "atmo_model_init_cached": (ST, [VP, C.c_uint, C.c_char_p, VP]),
     "atmo_get_solar_radiance": (ST, [VP, FP]),
     "atmo_get_sky_radiance": (ST, [VP, I64, VP, VP, VP, VP, VP, VP, VP]),
     "atmo_get_sky_radiance_to_point": (ST, [VP, I64, VP, VP, VP, VP, VP, VP, VP]),

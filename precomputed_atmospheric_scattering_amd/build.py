@@ -28,6 +28,7 @@ SOURCES = [
     ("query.hip", True),
     ("runtime.cpp", True),   # host code using the HIP runtime
     ("server.cpp", True),    # serving loop (worker thread, HIP runtime)
+    ("lut_io.cpp", True),    # LUT files (export, import, cached Init)
     ("model.cc", False),     # C++ Model facade, host only
 ]
 

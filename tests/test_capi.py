@@ -181,3 +181,10 @@ def test_counter_based_query_stream_matches_splitmix_sequence():
     assert np.allclose(np.linalg.norm(q["view_ray"], axis=1), 1.0, atol=1e-6)
     r = np.linalg.norm(q["point"].astype(np.float64), axis=1)
     assert (r >= 6360.0 - 1e-3).all() and (r <= 6420.0).all()
+
+
+def test_lut_file_null_arguments():
+    lib = _capi.lib()
+    assert lib.atmo_model_export_luts(None, b"/tmp") == 1
+    assert lib.atmo_model_import_luts(None, b"/tmp") == 1
+    assert lib.atmo_model_init_cached(None, 4, b"/tmp", None) == 1
