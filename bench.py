@@ -561,6 +561,28 @@ def main():
     atmo.synchronize()
     elapsed = time.perf_counter() - t_start
 
+    # The same three lookups on a coherent batch: the rays of one 1920x1080
+    # camera frame (neighbouring pixels share texels), outside the timed
+    # steps.
+    fq = configs.frame_queries(WIDTH, HEIGHT, kw["bottom_radius"] / 1000.0,
+                               kw["top_radius"] / 1000.0)
+    fdev = {k: atmo.DeviceArray.from_host(v) for k, v in fq.items()}
+    fev = [atmo.Event() for _ in range(2)]
+    frame_reps = 5
+    fev[0].record(stream)
+    for _ in range(frame_reps):
+        model.sky_luminance_device(n_rays, fdev["camera"], fdev["view_ray"],
+                                   fdev["shadow_length"], fdev["sun_direction"],
+                                   outs[0], outs[1], stream)
+        model.sky_luminance_to_point_device(n_rays, fdev["camera"], fdev["point"],
+                                            fdev["shadow_length"], fdev["sun_direction"],
+                                            outs[2], outs[3], stream)
+        model.sun_and_sky_illuminance_device(n_rays, fdev["point"], fdev["normal"],
+                                             fdev["sun_direction"], outs[4], outs[5], stream)
+    fev[1].record(stream)
+    stream.synchronize()
+    frame_ms = fev[0].elapsed_ms(fev[1]) / frame_reps
+
     steps = args.steps
     iters = model.params.num_iterations()
     sd_launches = steps * iters * (orders - 1)
@@ -605,6 +627,8 @@ def main():
             ("sky_luminance", "sky_luminance_to_point", "sun_and_sky_illuminance"),
             lookup_ms)},
         "lookup_roofline_frac": round(q_frac, 4),
+        "frame_lookup_ms": round(frame_ms, 4),
+        "frame_sky_rays_per_s": round(3.0 * n_rays / (frame_ms * 1e-3), 1),
         "pass_ms_per_atmosphere": {k: round(v / steps, 4) for k, v in pass_ms.items()},
         "roofline": {
             "kernel": "k_scattering_density",
