@@ -21,6 +21,7 @@ import json
 import math
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -203,10 +204,12 @@ def run_batched(args):
 
     cfgs = configs.batch(args.seed, 64)
     kws = [configs.model_kwargs(c) for c in cfgs]
-    models = [atmo.Model(**kw) for kw, _ in kws]
     orders = [o for _, o in kws]
     n_s_total = sum(texel_counts(kw["lut_sizes"])[1] for kw, _ in kws)
     stream = atmo.Stream()
+
+    models = [atmo.Model(**kw) for kw, _ in kws]
+    atmo.reset_tiles()  # the plain baseline runs at the default tile shapes
 
     # Plain: Model.init one model after the other, as a caller of the
     # reference's API would (temporaries allocated and freed by each Init).
@@ -231,6 +234,19 @@ def run_batched(args):
     stream.synchronize()
     plain_kept_s = time.perf_counter() - t0
 
+    # Tile shapes: the autotuner times every shape per size class of the
+    # batch (once; the table is then reused).
+    sizes = [kw["lut_sizes"] for kw, _ in kws]
+    t0 = time.perf_counter()
+    atmo.autotune(sizes, stream=stream)
+    tune_s = time.perf_counter() - t0
+    shapes = {}
+    for z in sizes:
+        t = atmo.get_tiles(z)
+        key = "sd %dx%d, ms %d" % (t["sd_nu_per_thread"], t["sd_block_threads"],
+                                  t["ms_texels_per_thread"])
+        shapes[key] = shapes.get(key, 0) + 1
+
     def timed_batch(use_graph, steps, warmup):
         b = atmo.Batch(models, orders, use_graph=use_graph)
         for _ in range(warmup):
@@ -251,6 +267,14 @@ def run_batched(args):
     eager_ms, eager_s, _ = timed_batch(False, max(1, args.steps // 2), 1)
     graph_ms, graph_s, launches = timed_batch(True, args.steps, max(1, args.warmup))
     exact = all(np.array_equal(models[i].lut("scattering"), ref[i]) for i in sample)
+    # The same batch with every size class at the default shapes.
+    tuned_table = os.path.join(tempfile.mkdtemp(prefix="atmo_tiles_"), "tiles.txt")
+    atmo.save_tiles(tuned_table)
+    atmo.reset_tiles()
+    _, default_s, default_launches = timed_batch(True, args.steps, max(1, args.warmup))
+    exact = exact and all(np.array_equal(models[i].lut("scattering"), ref[i])
+                          for i in sample)
+    atmo.load_tiles(tuned_table)
     value = 64 * args.steps / graph_s
     plain = 64 / plain_s
     _json_line({
@@ -269,6 +293,10 @@ def run_batched(args):
         "batch_p50_ms": round(percentile(graph_ms, 0.5), 3),
         "batch_p99_ms": round(percentile(graph_ms, 0.99), 3),
         "launches_per_batch": launches,
+        "autotune_s": round(tune_s, 3),
+        "tile_shapes": shapes,
+        "default_tiles_atmospheres_per_s": round(64 * args.steps / default_s, 2),
+        "default_tiles_launches_per_batch": default_launches,
         "plain_atmospheres_per_s": round(plain, 2),
         "plain_kept_temporaries_atmospheres_per_s": round(64 / plain_kept_s, 2),
         "batched_eager_atmospheres_per_s": round(64 * len(eager_ms) / eager_s, 2),
@@ -304,6 +332,11 @@ def run_serving(args):
              for k in atmo.QUERY_FIELDS}
         pool.append((q, [atmo.DeviceArray((n, 3)) for _ in range(6)]))
     gen = atmo.Stream()
+    # Tile shapes of every size class of the request distribution, tuned
+    # before the server starts (the table is shared by the process).
+    t0 = time.perf_counter()
+    atmo.autotune(configs.batch_size_classes(), stream=gen)
+    tune_s = time.perf_counter() - t0
     srv = atmo.Server(max_inflight_batches=3, max_batch_models=16, lookup_streams=2)
 
     # Warm-up: one request and one query batch, not counted.
@@ -389,7 +422,7 @@ def run_serving(args):
         "mean_batch_models": round(sum(batch_sizes) / max(1, len(batch_sizes)), 2),
         "max_batch_models": max(batch_sizes) if batch_sizes else 0,
         "failed": failed, "peak_resident_bytes": peak_resident,
-        "budget_bytes": st["budget_bytes"],
+        "budget_bytes": st["budget_bytes"], "autotune_s": round(tune_s, 3),
     })
 
 
@@ -504,8 +537,12 @@ def main():
     kw, orders = configs.model_kwargs(cfg)
     sizes = kw["lut_sizes"]
     n_t, n_s, n_e = texel_counts(sizes)
-    model = atmo.Model(retain_intermediates=True, pass_timing=True, **kw)
     stream = atmo.Stream()
+    t0 = time.perf_counter()
+    atmo.autotune([sizes], stream=stream)
+    tune_s = time.perf_counter() - t0
+    tiles = atmo.get_tiles(sizes)
+    model = atmo.Model(retain_intermediates=True, pass_timing=True, **kw)
     n_rays = WIDTH * HEIGHT
     q = configs.sky_queries(2024, n_rays, kw["bottom_radius"] / 1000.0,
                             kw["top_radius"] / 1000.0)
@@ -630,6 +667,11 @@ def main():
         "frame_lookup_ms": round(frame_ms, 4),
         "frame_sky_rays_per_s": round(3.0 * n_rays / (frame_ms * 1e-3), 1),
         "pass_ms_per_atmosphere": {k: round(v / steps, 4) for k, v in pass_ms.items()},
+        "tiles": {"scattering_density": "%d nu x %d threads" % (
+                      tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
+                  "multiple_scattering": "%d texels per thread" % tiles[
+                      "ms_texels_per_thread"],
+                  "autotune_s": round(tune_s, 3)},
         "roofline": {
             "kernel": "k_scattering_density",
             "bound": "valu",

@@ -184,6 +184,45 @@ atmo_status atmo_batch_launch(atmo_batch* batch, void* stream);
 int atmo_batch_num_launches(const atmo_batch* batch);
 void atmo_batch_destroy(atmo_batch* batch);
 
+/* Tile shapes and their autotuner (no counterpart in the reference, whose
+ * passes are fragment-shader draws). The scattering-density pass runs with
+ * sd_nu_per_thread nu outputs per thread in blocks of sd_block_threads; the
+ * multiple-scattering pass with ms_texels_per_thread texels per thread.
+ * Every shape computes each texel identically: the LUTs do not depend on
+ * the shapes. Shapes are chosen per LUT size class, (MU, NU) for the
+ * density pass and (MU, MU_S, NU) for multiple scattering, in a table shared
+ * by the process; a model's Init, batch or serving request reads it when its
+ * job table is built. Classes never tuned or set use the defaults (4 nu per
+ * thread, 256 threads; the fewest texels per thread that hold a row). */
+typedef struct atmo_tiles {
+  int sd_nu_per_thread;
+  int sd_block_threads;
+  int ms_texels_per_thread;
+  int tuned; /* atmo_get_tiles: bit 0 density class, bit 1 multiple class set */
+} atmo_tiles;
+/* Candidate shapes (returns the count; fills up to capacity entries). */
+int atmo_sd_tile_candidates(int* nu_per_thread, int* block_threads,
+                            int capacity);
+int atmo_ms_tile_candidates(int* texels_per_thread, int capacity);
+/* Times every candidate shape of the two passes on a batch of Earth-like
+ * atmospheres of each listed size (enough copies to fill the GPU) on
+ * 'stream', and records the fastest shape per size class; a shape replaces
+ * the default only if it is at least 3% faster. Classes already in the
+ * table are skipped unless retune is nonzero. Synchronous. */
+atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
+                          void* stream);
+atmo_status atmo_get_tiles(const atmo_lut_sizes* sizes, atmo_tiles* out);
+/* Pins the shapes of the size classes of 'sizes' (the tuned field is
+ * ignored); an error if a shape is not a candidate or if NU x MU_S texels
+ * per row exceed 256 x ms_texels_per_thread (rows of at most 1024 texels;
+ * longer rows take a gather kernel whatever the shape). */
+atmo_status atmo_set_tiles(const atmo_lut_sizes* sizes, const atmo_tiles* tiles);
+/* The table as text ("sd MU NU nu_per_thread block_threads",
+ * "ms MU MU_S NU texels_per_thread"), merged back by atmo_tune_load. */
+atmo_status atmo_tune_save(const char* path);
+atmo_status atmo_tune_load(const char* path);
+void atmo_tune_reset(void);
+
 /* The device LUTs (the reference binds them to texture units instead,
  * Model::SetProgramUniforms, atmosphere/model.h:289-294). */
 atmo_status atmo_model_get_lut(const atmo_model* model, atmo_lut_id id,

@@ -618,3 +618,65 @@ def init_batch(models, orders, stream=None):
 
 def synchronize():
     check(lib().atmo_device_synchronize(), "atmo_device_synchronize")
+
+
+# ---------------------------------------------------------------------------
+# Tile shapes of the scattering-density and multiple-scattering passes, and
+# their autotuner (atmo_autotune and friends). A shape is
+# {"sd_nu_per_thread", "sd_block_threads", "ms_texels_per_thread"}; sizes are
+# LUT-size dicts (DEFAULT_SIZES keys; missing keys take the defaults).
+# ---------------------------------------------------------------------------
+
+def _sizes(sizes):
+    z = dict(DEFAULT_SIZES)
+    if sizes:
+        z.update(sizes)
+    return _capi.LutSizes(**z)
+
+
+def tile_candidates():
+    """(scattering-density shapes [(nu per thread, block threads)],
+    multiple-scattering shapes [texels per thread])."""
+    n = lib().atmo_sd_tile_candidates(None, None, 0)
+    a, b = (C.c_int * n)(), (C.c_int * n)()
+    lib().atmo_sd_tile_candidates(a, b, n)
+    m = lib().atmo_ms_tile_candidates(None, 0)
+    c = (C.c_int * m)()
+    lib().atmo_ms_tile_candidates(c, m)
+    return list(zip(a, b)), list(c)
+
+
+def autotune(sizes_list, retune=False, stream=None):
+    """Times every tile shape for the size classes of sizes_list and records
+    the fastest (synchronous)."""
+    n = len(sizes_list)
+    arr = (_capi.LutSizes * max(1, n))(*[_sizes(z) for z in sizes_list])
+    check(lib().atmo_autotune(arr, n, int(bool(retune)), _stream_handle(stream)),
+          "atmo_autotune")
+
+
+def get_tiles(sizes=None):
+    t = _capi.Tiles()
+    z = _sizes(sizes)
+    check(lib().atmo_get_tiles(C.byref(z), C.byref(t)), "atmo_get_tiles")
+    return dict(sd_nu_per_thread=t.sd_nu_per_thread, sd_block_threads=t.sd_block_threads,
+                ms_texels_per_thread=t.ms_texels_per_thread,
+                sd_tuned=bool(t.tuned & 1), ms_tuned=bool(t.tuned & 2))
+
+
+def set_tiles(sizes, sd_nu_per_thread, sd_block_threads, ms_texels_per_thread):
+    t = _capi.Tiles(sd_nu_per_thread, sd_block_threads, ms_texels_per_thread, 0)
+    z = _sizes(sizes)
+    check(lib().atmo_set_tiles(C.byref(z), C.byref(t)), "atmo_set_tiles")
+
+
+def save_tiles(path):
+    check(lib().atmo_tune_save(str(path).encode()), "atmo_tune_save")
+
+
+def load_tiles(path):
+    check(lib().atmo_tune_load(str(path).encode()), "atmo_tune_load")
+
+
+def reset_tiles():
+    lib().atmo_tune_reset()

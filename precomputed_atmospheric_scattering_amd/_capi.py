@@ -109,6 +109,11 @@ class ServerStats(C.Structure):
                 ("failed", C.c_uint64)]
 
 
+class Tiles(C.Structure):
+    _fields_ = [("sd_nu_per_thread", C.c_int), ("sd_block_threads", C.c_int),
+                ("ms_texels_per_thread", C.c_int), ("tuned", C.c_int)]
+
+
 REQUEST_PRECOMPUTE, REQUEST_LOOKUP = 0, 1
 LOOKUP_SKY, LOOKUP_SKY_TO_POINT, LOOKUP_SUN_AND_SKY = 0, 1, 2
 
@@ -129,6 +134,14 @@ SIGNATURES = {
     "atmo_batch_launch": (ST, [VP, VP]),
     "atmo_batch_num_launches": (C.c_int, [VP]),
     "atmo_batch_destroy": (None, [VP]),
+    "atmo_sd_tile_candidates": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
+    "atmo_ms_tile_candidates": (C.c_int, [C.POINTER(C.c_int), C.c_int]),
+    "atmo_autotune": (ST, [C.POINTER(LutSizes), C.c_int, C.c_int, VP]),
+    "atmo_get_tiles": (ST, [C.POINTER(LutSizes), C.POINTER(Tiles)]),
+    "atmo_set_tiles": (ST, [C.POINTER(LutSizes), C.POINTER(Tiles)]),
+    "atmo_tune_save": (ST, [C.c_char_p]),
+    "atmo_tune_load": (ST, [C.c_char_p]),
+    "atmo_tune_reset": (None, []),
     "atmo_model_get_lut": (ST, [VP, C.c_int, C.POINTER(LutView)]),
     "atmo_model_copy_lut": (ST, [VP, C.c_int, FP]),
     "atmo_model_footprint": (ST, [VP, C.POINTER(SZ), C.POINTER(SZ)]),

@@ -29,6 +29,7 @@ SOURCES = [
     ("runtime.cpp", True),   # host code using the HIP runtime
     ("server.cpp", True),    # serving loop (worker thread, HIP runtime)
     ("lut_io.cpp", True),    # LUT files (export, import, cached Init)
+    ("autotune.cpp", True),  # tile-shape autotuner
     ("model.cc", False),     # C++ Model facade, host only
 ]
 

@@ -149,6 +149,19 @@ def random_atmosphere(rng):
     return kw
 
 
+def batch_size_classes():
+    """One LUT size per tile-shape class of the batched workload: scattering
+    MU, MU_S and NU each at x{0.5, 1, 2} (27 classes; the tuner's classes
+    are (MU, NU) and (MU, MU_S, NU))."""
+    out = []
+    for fmu in (0.5, 1.0, 2.0):
+        for fmus in (0.5, 1.0, 2.0):
+            for fnu in (0.5, 1.0, 2.0):
+                out.append(scaled_sizes({"scattering_mu": fmu, "scattering_mu_s": fmus,
+                                         "scattering_nu": fnu}))
+    return out
+
+
 def batch(seed, count=64):
     rng = SplitMix64(seed)
     return [random_atmosphere(rng) for _ in range(count)]

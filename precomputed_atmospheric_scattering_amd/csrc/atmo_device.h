@@ -86,6 +86,9 @@ struct Job {
   int blend;     // add into S, S_Mie, E (illuminance iterations after the first)
   int combined;  // COMBINED_SCATTERING_TEXTURES: no separate S_Mie output
   int orders;    // number of scattering orders of this job
+  // Tile shapes (indices into kSdTile and kMsTpt of kernels.h): host-side
+  // block counts and launch grouping; the kernels are instantiated per shape.
+  int sd_tile, ms_tile;
   // LUT sizes.
   int tw, th;
   int nr, nmu, nmus, nnu;

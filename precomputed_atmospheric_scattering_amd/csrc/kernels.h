@@ -18,7 +18,11 @@ struct PassLaunch {
   int max_nnu;  // largest nu size among the jobs (LDS sizing)
   int max_ew;   // largest irradiance width among the jobs (LDS sizing)
   int max_w;    // largest packed row (nu x mu_s texels) among the jobs
+  int min_w;    // smallest packed row among the jobs
+  int max_lds_w;  // longest packed row of at most kMsMaxRow texels (0: none)
   int kinds;    // kJobsF32 | kJobsAcc64: which kinds of jobs the launch holds
+  int tile;     // tile shape of the jobs (scattering density, multiple
+                // scattering): every job of a launch has the same one
 };
 
 constexpr int kJobsF32 = 1;    // jobs accumulating S and SM in FP32
@@ -26,15 +30,44 @@ constexpr int kJobsAcc64 = 2;  // illuminance-mode jobs with FP64 accumulators
 
 constexpr int kBlock = 256;
 
+// Tile shapes of the two passes that dominate Init, picked per LUT size
+// class by the autotuner (autotune.cpp). Every shape computes each texel
+// with the same operations in the same order, so the LUTs do not depend on
+// the choice.
+//
+// Scattering density: nu outputs per thread (they share the phase function
+// of the outgoing direction) and threads per block. A (r, mu_s) plane of
+// MU x ceil(NU / chunk) threads is split into ceil(plane / block) blocks,
+// each of which builds the plane's nu tables in LDS.
+struct SdTile {
+  int chunk, block;
+};
+constexpr int kSdTiles = 6;
+constexpr SdTile kSdTile[kSdTiles] = {{4, 256}, {2, 256}, {1, 256},
+                                      {4, 128}, {2, 128}, {4, 64}};
+
 // Multiple scattering: packed rows of at most kMsMaxRow texels are processed
-// through LDS, ms_rows_per_block(W) rows per block (0: gather kernel path).
-constexpr int kMsRows = 4;
+// through LDS by blocks of kBlock threads, each thread holding kMsTpt[tile]
+// texels: ms_rows_per_block(W, tpt) whole rows per block (0: the row is too
+// long for LDS and takes the gather kernel; -1: it does not fit this tile).
+constexpr int kMsTiles = 3;
+constexpr int kMsTpt[kMsTiles] = {1, 2, 4};
 constexpr int kMsMaxRow = 4 * kBlock;
-ATMO_HD int ms_rows_per_block(int w) {
+// Rows per block are capped at 4 (one texel per thread) or 8, which keeps
+// the block's LDS (2 x kBlock x tpt float4 + 3.2 KB of samples per row)
+// under 64 KB.
+ATMO_HD int ms_rows_per_block(int w, int tpt) {
   if (w > kMsMaxRow) return 0;
-  if (w >= kBlock) return 1;
-  const int r = kBlock / w;
-  return r < kMsRows ? r : kMsRows;
+  if (w > kBlock * tpt) return -1;
+  const int r = kBlock * tpt / w;
+  const int cap = tpt == 1 ? 4 : 8;
+  return r < cap ? r : cap;
+}
+// The smallest texels-per-thread tile that holds a row of w texels.
+ATMO_HD int ms_default_tile(int w) {
+  if (w <= kBlock) return 0;
+  if (w <= 2 * kBlock) return 1;
+  return 2;
 }
 
 // Blocks one job needs for each pass.

@@ -411,7 +411,6 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
 // table of delta-irradiance row 0 (r = rb).
 // ---------------------------------------------------------------------------
 constexpr int kTheta = 16;
-constexpr int kNuChunk = 4;  // nu outputs per thread
 
 // Table entry i: (v.x, v.y, v.z, dv.x), (dv.y, dv.z, -, -) with
 // dv = v[i + 1] - v[i] (0 for the last entry).
@@ -437,17 +436,17 @@ struct SdTheta {
   int ground;
 };
 
-// One incident zenith angle theta_l for the kNuChunk outputs (same r, mu,
+// One incident zenith angle theta_l for the kChunk outputs (same r, mu,
 // mu_s; consecutive nu) of a thread. The phase function of the scattering
 // towards the view direction depends on (mu, theta_l, phi) only, so it is
-// computed once per azimuth pair for all kNuChunk outputs.
-template <bool kOrder2, bool kGround>
+// computed once per azimuth pair for all kChunk outputs.
+template <int kChunk, bool kOrder2, bool kGround>
 __device__ __forceinline__ void sd_theta(
     const float4* __restrict__ tab_a, const float4* __restrict__ tab_b,
     const float4* __restrict__ etab, float nu_max, float e_max, float A, float P,
-    float Q, const float (&Bv)[kNuChunk], const float (&Cv)[kNuChunk], float g1,
+    float Q, const float (&Bv)[kChunk], const float (&Cv)[kChunk], float g1,
     float mg2, const float cR[3], const float cM[3], const SdTheta& th,
-    float (&acc)[kNuChunk][3]) {
+    float (&acc)[kChunk][3]) {
   // nu table coordinate of the incident direction at azimuth phi:
   // t = (nu1 + 1) / 2 * (NU - 1) with nu1 = A + B cos(phi) +- C sin(phi),
   // as one FMA from per-angle constants; likewise the ground-irradiance
@@ -455,9 +454,9 @@ __device__ __forceinline__ void sd_theta(
   const float hn = 0.5f * nu_max;
   const float inv_hn = 1.0f / hn;
   const float tA = fmaf(A, hn, hn);
-  float tB[kNuChunk], tC[kNuChunk];
+  float tB[kChunk], tC[kChunk];
 #pragma unroll
-  for (int j = 0; j < kNuChunk; ++j) {
+  for (int j = 0; j < kChunk; ++j) {
     tB[j] = Bv[j] * hn;
     tC[j] = Cv[j] * hn;
   }
@@ -477,7 +476,7 @@ __device__ __forceinline__ void sd_theta(
     const float p1 = x2 * fmaf(cM[1], rs3, cR[1]);
     const float p2 = x2 * fmaf(cM[2], rs3, cR[2]);
 #pragma unroll
-    for (int j = 0; j < kNuChunk; ++j) {
+    for (int j = 0; j < kChunk; ++j) {
       const float tb = fmaf(tB[j], c, tA);
       float l0, l1, l2;
 #pragma unroll
@@ -519,7 +518,8 @@ __device__ __forceinline__ void sd_theta(
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scattering_density(
+template <int kChunk, int kB>
+__global__ __launch_bounds__(kB) void k_scattering_density(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
     int njobs, int order) {
   extern __shared__ float4 lds[];
@@ -530,9 +530,9 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
   const Job& J = jobs[ji];
   const int lb = blockIdx.x - first_block[ji];
   const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
-  const int nch = (nnu + kNuChunk - 1) / kNuChunk;
+  const int nch = (nnu + kChunk - 1) / kChunk;
   const int plane = nmu * nch;  // threads per (r, mu_s) plane
-  const int tiles = (plane + kBlock - 1) / kBlock;
+  const int tiles = (plane + kB - 1) / kB;
   const int tile = lb % tiles;
   const int pl = lb / tiles;
   const int imus = pl % nmus, k = pl / nmus;
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
   const double gd = J.g;
   const float kM = static_cast<float>(3.0 / (8.0 * kPi) * (1.0 - gd * gd) / (2.0 + gd * gd));
   const int64_t layer = static_cast<int64_t>(k) * nmu;
-  for (int e = tid; e < kTheta * nnu; e += kBlock) {
+  for (int e = tid; e < kTheta * nnu; e += kB) {
     const int l = e / nnu, n = e % nnu;
     const int64_t r0 = (layer + s_y0[l]) * W, r1 = (layer + s_y1[l]) * W;
     const int c0 = n * nmus + imus;
@@ -617,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
   }
   // Row 0 of delta irradiance (r = rb): the ground-irradiance lookup of the
   // reference at bottom_radius reads exactly this row.
-  for (int e = tid; e < J.ew; e += kBlock) {
+  for (int e = tid; e < J.ew; e += kB) {
     const float4 v = gload(J.dE, e);
     const float4 vn = gload(J.dE, e + 1 < J.ew ? e + 1 : e);
     etab[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
@@ -625,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
   }
   __syncthreads();
 
-  const int flat = tile * kBlock + tid;
+  const int flat = tile * kB + tid;
   if (flat >= plane) return;
   const int ch = flat % nch, y = flat / nch;
   double mu;
@@ -634,10 +634,10 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
   // functions.glsl:1181-1185 for each nu of the chunk. Outputs past the last
   // nu slice get sx = sy = 0 (in-range table reads) and are not stored.
   const double wx = sqrt(1.0 - mu * mu);
-  float fsx[kNuChunk], fsy[kNuChunk];
+  float fsx[kChunk], fsy[kChunk];
 #pragma unroll
-  for (int jn = 0; jn < kNuChunk; ++jn) {
-    const int nui = ch * kNuChunk + jn;
+  for (int jn = 0; jn < kChunk; ++jn) {
+    const int nui = ch * kChunk + jn;
     fsx[jn] = fsy[jn] = 0.0f;
     if (nui >= nnu) continue;
     const double nu = s_slice_nu(J, nui, mu, mu_s);
@@ -661,15 +661,15 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
   const float mg2 = static_cast<float>(-2.0 * gd);
   const float nu_max = static_cast<float>(nnu - 1);
   const float e_max = static_cast<float>(J.ew - 1);
-  float acc[kNuChunk][3];
+  float acc[kChunk][3];
 #pragma unroll
-  for (int jn = 0; jn < kNuChunk; ++jn) acc[jn][0] = acc[jn][1] = acc[jn][2] = 0.0f;
+  for (int jn = 0; jn < kChunk; ++jn) acc[jn][0] = acc[jn][1] = acc[jn][2] = 0.0f;
   for (int l = 0; l < kTheta; ++l) {
     const SdTheta t = th[l];
     const float A = fmus * t.cos_t, P = fmu * t.cos_t, Q = fwx * t.sin_t;
-    float Bv[kNuChunk], Cv[kNuChunk];
+    float Bv[kChunk], Cv[kChunk];
 #pragma unroll
-    for (int jn = 0; jn < kNuChunk; ++jn) {
+    for (int jn = 0; jn < kChunk; ++jn) {
       Bv[jn] = fsx[jn] * t.sin_t;
       Cv[jn] = fsy[jn] * t.sin_t;
     }
@@ -678,23 +678,23 @@ __global__ __launch_bounds__(kBlock) void k_scattering_density(
     const bool g = __builtin_amdgcn_readfirstlane(t.ground) != 0;
     if (two) {
       if (g)
-        sd_theta<true, true>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR, cM,
+        sd_theta<kChunk, true, true>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR, cM,
                              t, acc);
       else
-        sd_theta<true, false>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR,
+        sd_theta<kChunk, true, false>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR,
                               cM, t, acc);
     } else {
       if (g)
-        sd_theta<false, true>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR,
+        sd_theta<kChunk, false, true>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR,
                               cM, t, acc);
       else
-        sd_theta<false, false>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR,
+        sd_theta<kChunk, false, false>(ta, tb, etab, nu_max, e_max, A, P, Q, Bv, Cv, g1, mg2, cR,
                                cM, t, acc);
     }
   }
 #pragma unroll
-  for (int jn = 0; jn < kNuChunk; ++jn) {
-    const int nui = ch * kNuChunk + jn;
+  for (int jn = 0; jn < kChunk; ++jn) {
+    const int nui = ch * kChunk + jn;
     if (nui < nnu)
       gstore(J.dJ, (layer + y) * W + nui * nmus + imus,
              make_float4(acc[jn][0], acc[jn][1], acc[jn][2], 0.0f));
@@ -832,9 +832,9 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
 // one row in LDS with coalesced loads (every texel of a row reads the same
 // four rows), and each texel then reads its 4 (mu_s, nu) taps from LDS:
 // 4 coalesced global loads per texel and sample instead of 16 gathers.
-// Rows shorter than 256 texels are packed up to kMsRows per block; longer
-// rows (up to kMsMaxRow texels) give each thread up to kTpt texels. Rows
-// longer than that take the per-texel 16-tap gather of
+// Each thread holds kTpt texels; a block takes ms_rows_per_block(W, kTpt)
+// whole rows (several short rows, or one row of up to kBlock * kTpt
+// texels). Rows longer than kMsMaxRow take the per-texel 16-tap gather of
 // k_multiple_scattering_gather. Both kernels are launched over the same
 // blocks and each returns at once on the other's jobs, so the path of a job
 // depends on its own row length only.
@@ -851,19 +851,23 @@ struct MsRowSample {
 template <int kTpt>
 __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
-    int njobs) {
-  __shared__ MsRowSample smp[kMsRows][kMarch];
-  __shared__ float4 buf[2][kBlock * kTpt];
+    int njobs, int max_rows) {
+  // Dynamic LDS: two buffers of kBlock * kTpt blended texels, then the
+  // samples of max_rows rows (the launch's largest rows per block).
+  extern __shared__ float4 ms_lds[];
+  float4* const buf = ms_lds;
+  MsRowSample(*const smp)[kMarch] =
+      reinterpret_cast<MsRowSample(*)[kMarch]>(ms_lds + 2 * kBlock * kTpt);
   const int ji = find_job(first_block, njobs, blockIdx.x);
   const Job& J = jobs[ji];
   const int lb = blockIdx.x - first_block[ji];
   const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
   const int W = nnu * nmus;
-  const int rpb = ms_rows_per_block(W);
-  // Gather-path job (uniform per block). The launcher picks kTpt so that
-  // every LDS-path row fits; the second test only keeps LDS indexing in
+  const int rpb = ms_rows_per_block(W, kTpt);
+  // Gather-path job (uniform per block). The launcher groups jobs by tile
+  // and sizes max_rows over them; the tests only keep LDS indexing in
   // bounds should it not.
-  if (rpb == 0 || W > kBlock * kTpt) return;
+  if (rpb <= 0 || rpb > max_rows) return;
   const int nrows = J.nr * nmu;
   const int row0 = xcd_contiguous(lb, (nrows + rpb - 1) / rpb) * rpb;
   const int tid = threadIdx.x;
@@ -970,7 +974,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   };
   fetch(0);
   for (int i = 0; i < kMarch; ++i) {
-    float4* B = buf[i & 1];
+    float4* B = buf + (i & 1) * (kBlock * kTpt);
     // Blend the four (r, mu) source rows of sample i into LDS.
 #pragma unroll
     for (int j = 0; j < kTpt; ++j) {
@@ -1063,7 +1067,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
   const int lb = blockIdx.x - first_block[ji];
   const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
   const int W = nnu * nmus;
-  if (ms_rows_per_block(W) != 0) return;  // LDS-path job
+  if (W <= kMsMaxRow) return;  // LDS-path job
   const int tiles = (W + kBlock - 1) / kBlock;
   const int tile = lb % tiles;
   const int row = lb / tiles;
@@ -1296,15 +1300,17 @@ int blocks_single_scattering(const Job& j) {
   return ((w + kBlock - 1) / kBlock) * j.nmu * j.nr;
 }
 int blocks_scattering_density(const Job& j) {
-  const int plane = j.nmu * ((j.nnu + kNuChunk - 1) / kNuChunk);
-  return ((plane + kBlock - 1) / kBlock) * j.nmus * j.nr;
+  const SdTile t = kSdTile[j.sd_tile];
+  const int plane = j.nmu * ((j.nnu + t.chunk - 1) / t.chunk);
+  return ((plane + t.block - 1) / t.block) * j.nmus * j.nr;
 }
 int blocks_indirect_irradiance(const Job& j) { return j.ew * j.eh; }
 int blocks_multiple_scattering(const Job& j) {
   const int w = j.nnu * j.nmus;
-  const int rpb = ms_rows_per_block(w);
+  const int rpb = ms_rows_per_block(w, kMsTpt[j.ms_tile]);
   const int rows = j.nmu * j.nr;
   if (rpb == 0) return ((w + kBlock - 1) / kBlock) * rows;
+  if (rpb < 0) return 0;  // not a tile of this row length (rejected by the host)
   return (rows + rpb - 1) / rpb;
 }
 int blocks_half_conversion(const Job& j) {
@@ -1331,13 +1337,27 @@ hipError_t launch_single_scattering(const PassLaunch& p, hipStream_t s) {
                        p.jobs, p.first_block, p.njobs);
   return hipGetLastError();
 }
+template <int kChunk, int kB>
+void launch_sd_tile(const PassLaunch& p, int order, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((k_scattering_density<kChunk, kB>), dim3(p.total_blocks), dim3(kB), lds,
+                     s, p.jobs, p.first_block, p.njobs, order);
+}
+
 hipError_t launch_scattering_density(const PassLaunch& p, int order,
                                      hipStream_t s) {
   const size_t lds = sizeof(float4) *
                      (static_cast<size_t>((order == 2 ? 4 : 2) * kTheta * p.max_nnu) +
                       2 * static_cast<size_t>(p.max_ew));
-  hipLaunchKernelGGL(k_scattering_density, dim3(p.total_blocks), dim3(kBlock), lds, s,
-                     p.jobs, p.first_block, p.njobs, order);
+  static_assert(kSdTiles == 6, "one case per tile shape");
+  switch (p.tile) {
+    case 0: launch_sd_tile<kSdTile[0].chunk, kSdTile[0].block>(p, order, lds, s); break;
+    case 1: launch_sd_tile<kSdTile[1].chunk, kSdTile[1].block>(p, order, lds, s); break;
+    case 2: launch_sd_tile<kSdTile[2].chunk, kSdTile[2].block>(p, order, lds, s); break;
+    case 3: launch_sd_tile<kSdTile[3].chunk, kSdTile[3].block>(p, order, lds, s); break;
+    case 4: launch_sd_tile<kSdTile[4].chunk, kSdTile[4].block>(p, order, lds, s); break;
+    case 5: launch_sd_tile<kSdTile[5].chunk, kSdTile[5].block>(p, order, lds, s); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 hipError_t launch_indirect_irradiance(const PassLaunch& p, int order,
@@ -1366,18 +1386,29 @@ hipError_t launch_to_half(const PassLaunch& p, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int kTpt>
+void launch_ms_tile(const PassLaunch& p, hipStream_t s) {
+  // Rows per block shrink as rows grow: the shortest row sets the launch's
+  // largest count.
+  const int rows = ms_rows_per_block(p.min_w, kTpt);
+  const size_t lds = sizeof(float4) * 2 * kBlock * kTpt +
+                     sizeof(MsRowSample) * kMarch * static_cast<size_t>(rows);
+  hipLaunchKernelGGL(k_multiple_scattering<kTpt>, dim3(p.total_blocks), dim3(kBlock), lds, s,
+                     p.jobs, p.first_block, p.njobs, rows);
+}
+
 hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s) {
-  // Texels per thread: enough for the longest row that takes the LDS path.
-  const int w = p.max_w < kMsMaxRow ? p.max_w : kMsMaxRow;
-  if (w <= kBlock)
-    hipLaunchKernelGGL(k_multiple_scattering<1>, dim3(p.total_blocks), dim3(kBlock), 0, s,
-                       p.jobs, p.first_block, p.njobs);
-  else if (w <= 2 * kBlock)
-    hipLaunchKernelGGL(k_multiple_scattering<2>, dim3(p.total_blocks), dim3(kBlock), 0, s,
-                       p.jobs, p.first_block, p.njobs);
-  else
-    hipLaunchKernelGGL(k_multiple_scattering<4>, dim3(p.total_blocks), dim3(kBlock), 0, s,
-                       p.jobs, p.first_block, p.njobs);
+  if (p.tile < 0 || p.tile >= kMsTiles) return hipErrorInvalidValue;
+  if (p.min_w <= kMsMaxRow) {
+    // Every LDS-path row of the launch must fit the tile.
+    if (p.max_lds_w > kBlock * kMsTpt[p.tile]) return hipErrorInvalidValue;
+    static_assert(kMsTiles == 3, "one case per tile shape");
+    switch (p.tile) {
+      case 0: launch_ms_tile<kMsTpt[0]>(p, s); break;
+      case 1: launch_ms_tile<kMsTpt[1]>(p, s); break;
+      default: launch_ms_tile<kMsTpt[2]>(p, s); break;
+    }
+  }
   if (p.max_w > kMsMaxRow)
     hipLaunchKernelGGL(k_multiple_scattering_gather, dim3(p.total_blocks), dim3(kBlock), 0,
                        s, p.jobs, p.first_block, p.njobs);

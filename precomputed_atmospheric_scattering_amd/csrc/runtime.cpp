@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -309,6 +310,9 @@ void fill_job_luts(atmo_model* m, atmo::Job* j) {
     j->sky_k[c] = static_cast<float>(m->sky_k[c]);
     j->sun_k[c] = static_cast<float>(m->sun_k[c]);
   }
+  const Tiles t = tiles_for(m->sizes);
+  j->sd_tile = t.sd;
+  j->ms_tile = t.ms;
 }
 
 // Rebuild the host job table (LUT pointers may have changed) and upload it.
@@ -337,6 +341,13 @@ atmo_status upload_jobs(atmo_model* m, unsigned orders, hipStream_t stream) {
   ATMO_HIP(hipMemcpyAsync(m->d_jobs, m->jobs.data(),
                           m->jobs.size() * sizeof(atmo::Job),
                           hipMemcpyHostToDevice, stream));
+  // Block counts of the single-model launches (they follow the tiles).
+  for (int p = 0; p < kPasses; ++p) {
+    m->first_block_host[2 * p] = 0;
+    m->first_block_host[2 * p + 1] = pass_blocks(static_cast<Pass>(p), m->jobs[0]);
+  }
+  ATMO_HIP(hipMemcpyAsync(m->d_first_block, m->first_block_host,
+                          sizeof(m->first_block_host), hipMemcpyHostToDevice, stream));
   return ATMO_OK;
 }
 
@@ -349,7 +360,11 @@ atmo::PassLaunch single_launch(const atmo_model* m, int job_index, Pass p) {
   l.max_nnu = m->sizes.nnu;
   l.max_ew = m->sizes.ew;
   l.max_w = m->sizes.nnu * m->sizes.nmus;
+  l.min_w = l.max_w;
+  l.max_lds_w = l.max_w <= atmo::kMsMaxRow ? l.max_w : 0;
   l.kinds = accumulates_fp64(m->spec) ? atmo::kJobsAcc64 : atmo::kJobsF32;
+  const atmo::Job& j = m->jobs[job_index];
+  l.tile = p == kSD ? j.sd_tile : (p == kMS ? j.ms_tile : 0);
   return l;
 }
 
@@ -598,14 +613,6 @@ atmo_status atmo_model_create(const atmo_model_params* p,
   const int iters = num_iterations(s);
   ATMO_HIP(hipMalloc(&m->d_jobs, (iters + 2) * sizeof(atmo::Job)));
   ATMO_HIP(hipMalloc(&m->d_first_block, 2 * kPasses * sizeof(int)));
-  atmo::Job probe{};
-  set_sizes(&probe, sz);
-  for (int p2 = 0; p2 < kPasses; ++p2) {
-    m->first_block_host[2 * p2] = 0;
-    m->first_block_host[2 * p2 + 1] = pass_blocks(static_cast<Pass>(p2), probe);
-  }
-  ATMO_HIP(hipMemcpy(m->d_first_block, m->first_block_host, sizeof(m->first_block_host),
-                     hipMemcpyHostToDevice));
   ATMO_TRY(upload_jobs(m.get(), 4, nullptr));
   ATMO_HIP(hipStreamSynchronize(nullptr));
   *out = m.release();
@@ -1090,17 +1097,21 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
   plan->jobs.clear();
   plan->first_block.clear();
   plan->levels.clear();
-  auto add_level = [&](Pass p, int order, const std::vector<std::pair<int, int>>& sel) {
+  auto add_one = [&](Pass p, int order, int tile,
+                     const std::vector<std::pair<int, int>>& sel) {
     if (sel.empty()) return;
     Level l;
     l.pass = p;
     l.order = order;
+    l.tile = tile;
     l.job_offset = static_cast<int>(plan->jobs.size());
     l.njobs = static_cast<int>(sel.size());
     l.fb_offset = static_cast<int>(plan->first_block.size());
     l.max_nnu = 0;
     l.max_ew = 0;
     l.max_w = 0;
+    l.min_w = std::numeric_limits<int>::max();
+    l.max_lds_w = 0;
     l.kinds = 0;
     int total = 0;
     for (const auto& mj : sel) {
@@ -1111,11 +1122,31 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
       l.max_nnu = std::max(l.max_nnu, j.nnu);
       l.max_ew = std::max(l.max_ew, j.ew);
       l.max_w = std::max(l.max_w, j.nnu * j.nmus);
+      l.min_w = std::min(l.min_w, j.nnu * j.nmus);
+      if (j.nnu * j.nmus <= atmo::kMsMaxRow)
+        l.max_lds_w = std::max(l.max_lds_w, j.nnu * j.nmus);
       l.kinds |= j.acc64 ? atmo::kJobsAcc64 : atmo::kJobsF32;
     }
     plan->first_block.push_back(total);
     l.total_blocks = total;
     plan->levels.push_back(l);
+  };
+  // Scattering density and multiple scattering: one launch per tile shape
+  // among the jobs (each kernel instance has one shape).
+  auto add_level = [&](Pass p, int order, const std::vector<std::pair<int, int>>& sel) {
+    if (p != kSD && p != kMS) {
+      add_one(p, order, 0, sel);
+      return;
+    }
+    const int ntiles = p == kSD ? atmo::kSdTiles : atmo::kMsTiles;
+    for (int t = 0; t < ntiles; ++t) {
+      std::vector<std::pair<int, int>> part;
+      for (const auto& mj : sel) {
+        const atmo::Job& j = models[mj.first]->jobs[mj.second];
+        if ((p == kSD ? j.sd_tile : j.ms_tile) == t) part.push_back(mj);
+      }
+      add_one(p, order, t, part);
+    }
   };
   int max_iters = 0;
   unsigned max_orders = 0;
@@ -1165,7 +1196,10 @@ atmo_status enqueue_levels(const LevelPlan& plan, const atmo::Job* d_jobs,
     pl.max_nnu = l.max_nnu;
     pl.max_ew = l.max_ew;
     pl.max_w = l.max_w;
+    pl.min_w = l.min_w;
+    pl.max_lds_w = l.max_lds_w;
     pl.kinds = l.kinds;
+    pl.tile = l.tile;
     hipError_t e = hipSuccess;
     switch (l.pass) {
       case kT: e = atmo::launch_transmittance(pl, s); break;

@@ -45,7 +45,9 @@ enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kAcc, kHalf, kClear, kPasses };
 struct Level {
   Pass pass;
   int order;  // SD: order computed; II: order read; others unused
-  int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew, max_w, kinds;
+  int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew, max_w, min_w, max_lds_w,
+      kinds;
+  int tile;   // SD and MS: the tile shape shared by the level's jobs
 };
 
 // The launch sequence of several models' Init (see plan_levels).
@@ -113,6 +115,12 @@ int num_iterations(const Spec& s);
 // Whether Init accumulates S and SM in FP64 (illuminance mode).
 inline bool accumulates_fp64(const Spec& s) { return s.num_precomputed_wavelengths > 3; }
 int pass_blocks(Pass p, const atmo::Job& j);
+// Tile shapes for a model's LUT sizes: the autotuner's choice, or the
+// default shapes for sizes it has not tuned (autotune.cpp).
+struct Tiles {
+  int sd, ms;
+};
+Tiles tiles_for(const Sizes& z);
 // Rebuild a model's job table for 'orders' and copy it to the device on
 // 'stream' (asynchronous with respect to other streams).
 atmo_status upload_jobs(atmo_model* m, unsigned orders, hipStream_t stream);

@@ -1,0 +1,124 @@
+"""Tile shapes of the scattering-density and multiple-scattering passes:
+every shape gives bit-identical LUTs (so the autotuner's choice never
+changes a result), the autotuner records a valid shape per size class, and
+a batch whose jobs have different shapes (one launch per shape) equals each
+atmosphere precomputed alone."""
+import numpy as np
+import pytest
+
+import precomputed_atmospheric_scattering_amd as atmo
+from precomputed_atmospheric_scattering_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("transmittance", "scattering", "single_mie_scattering", "irradiance")
+
+# Size classes with half-empty default blocks (MU x NU = 64 x 4), the
+# reference sizes, and rows of 512 texels (two texels per thread by default).
+SIZES = {
+    "small": dict(transmittance_width=64, transmittance_height=16, scattering_r=8,
+                  scattering_mu=64, scattering_mu_s=16, scattering_nu=4,
+                  irradiance_width=32, irradiance_height=8),
+    "reference": dict(configs.DEFAULT_SIZES),
+    "wide": dict(transmittance_width=64, transmittance_height=16, scattering_r=8,
+                 scattering_mu=32, scattering_mu_s=32, scattering_nu=16,
+                 irradiance_width=32, irradiance_height=8),
+}
+
+
+@pytest.fixture(autouse=True)
+def clean_table():
+    atmo.reset_tiles()
+    yield
+    atmo.reset_tiles()
+
+
+def run(cfg):
+    kw, orders = configs.model_kwargs(cfg)
+    m = atmo.Model(**kw)
+    m.init(orders)
+    out = {n: m.lut(n) for n in NAMES}
+    m.close()
+    return out
+
+
+def shapes_for(sizes):
+    sd, ms = atmo.tile_candidates()
+    d = atmo.get_tiles(sizes)
+    dsd = (d["sd_nu_per_thread"], d["sd_block_threads"])
+    out = [(s, d["ms_texels_per_thread"]) for s in sd if s != dsd]
+    w = sizes["scattering_nu"] * sizes["scattering_mu_s"]
+    out += [(dsd, t) for t in ms if t != d["ms_texels_per_thread"] and w <= 256 * t]
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(SIZES))
+def test_every_shape_gives_identical_luts(name):
+    sizes = SIZES[name]
+    cfg = configs.earth(orders=4, lut_sizes=sizes)
+    ref = run(cfg)
+    for (chunk, block), tpt in shapes_for(sizes):
+        print("case", name, chunk, block, tpt, flush=True)
+        atmo.set_tiles(sizes, chunk, block, tpt)
+        got = run(cfg)
+        for n in NAMES:
+            assert np.array_equal(got[n], ref[n]), (name, chunk, block, tpt, n)
+        atmo.reset_tiles()
+
+
+def test_every_shape_identical_illuminance_mode():
+    # FP64 accumulators of the multiple-scattering epilogue (15 wavelengths).
+    sizes = SIZES["small"]
+    cfg = configs.spectral(orders=3, lut_sizes=sizes)
+    ref = run(cfg)
+    for (chunk, block), tpt in [((1, 256), 1), ((4, 64), 2), ((2, 128), 4)]:
+        print("case spectral", chunk, block, tpt, flush=True)
+        atmo.set_tiles(sizes, chunk, block, tpt)
+        got = run(cfg)
+        for n in NAMES:
+            assert np.array_equal(got[n], ref[n]), (chunk, block, tpt, n)
+
+
+def test_autotune_then_mixed_shape_batch_equals_alone():
+    cfgs = configs.batch(23, 8)
+    sizes = [c["lut_sizes"] for c in cfgs]
+    alone = [run(c) for c in cfgs]  # default shapes
+    atmo.autotune(sizes)
+    sd, ms = atmo.tile_candidates()
+    for z in sizes:
+        t = atmo.get_tiles(z)
+        assert t["sd_tuned"] and t["ms_tuned"]
+        assert (t["sd_nu_per_thread"], t["sd_block_threads"]) in sd
+        w = z["scattering_nu"] * z["scattering_mu_s"]
+        assert t["ms_texels_per_thread"] in ms
+        assert w > 1024 or w <= 256 * t["ms_texels_per_thread"]
+    # Force two shapes on the first two size classes so that the batch has
+    # jobs of different shapes in one pass level whatever the tuner chose.
+    atmo.set_tiles(sizes[0], 1, 256, 4 if sizes[0]["scattering_nu"] *
+                   sizes[0]["scattering_mu_s"] <= 1024 else 1)
+    atmo.set_tiles(sizes[1], 4, 64, 4 if sizes[1]["scattering_nu"] *
+                   sizes[1]["scattering_mu_s"] <= 1024 else 1)
+    models = [atmo.Model(**configs.model_kwargs(c)[0]) for c in cfgs]
+    b = atmo.Batch(models, [c["orders"] for c in cfgs])
+    b.launch()
+    atmo.synchronize()
+    for i, (m, ref) in enumerate(zip(models, alone)):
+        for n in NAMES:
+            assert np.array_equal(m.lut(n), ref[n]), (i, n)
+    b.close()
+    for m in models:
+        m.close()
+
+
+def test_tuned_shape_used_by_init():
+    sizes = SIZES["small"]
+    atmo.autotune([sizes])
+    t1 = atmo.get_tiles(sizes)
+    atmo.autotune([sizes])  # already tuned: unchanged, no re-timing needed
+    assert atmo.get_tiles(sizes) == t1
+    cfg = configs.earth(orders=3, lut_sizes=sizes)
+    got = run(cfg)
+    atmo.reset_tiles()
+    ref = run(cfg)
+    for n in NAMES:
+        assert np.array_equal(got[n], ref[n]), n

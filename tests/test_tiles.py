@@ -1,0 +1,99 @@
+"""The tile-shape table of the autotuner (atmo_get_tiles, atmo_set_tiles,
+atmo_tune_save/load/reset): host-only calls, no GPU needed."""
+import pytest
+
+import precomputed_atmospheric_scattering_amd as atmo
+from precomputed_atmospheric_scattering_amd import AtmoError
+
+SMALL = dict(scattering_r=16, scattering_mu=64, scattering_mu_s=16, scattering_nu=4)
+WIDE = dict(scattering_mu_s=64, scattering_nu=16)   # rows of 1024 texels
+LONG = dict(scattering_mu_s=128, scattering_nu=16)  # 2048: gather kernel
+
+
+@pytest.fixture(autouse=True)
+def clean_table():
+    atmo.reset_tiles()
+    yield
+    atmo.reset_tiles()
+
+
+def test_candidates():
+    sd, ms = atmo.tile_candidates()
+    assert sd[0] == (4, 256)  # the default shape comes first
+    assert len(set(sd)) == len(sd) and all(b % 64 == 0 for _, b in sd)
+    assert ms == [1, 2, 4]
+
+
+def test_defaults_follow_row_length():
+    t = atmo.get_tiles()
+    assert (t["sd_nu_per_thread"], t["sd_block_threads"]) == (4, 256)
+    assert t["ms_texels_per_thread"] == 1  # 8 x 32 = 256 texels per row
+    assert not t["sd_tuned"] and not t["ms_tuned"]
+    assert atmo.get_tiles(dict(scattering_mu_s=64))["ms_texels_per_thread"] == 2
+    assert atmo.get_tiles(WIDE)["ms_texels_per_thread"] == 4
+
+
+def test_set_get_by_size_class():
+    atmo.set_tiles(SMALL, 1, 256, 2)
+    t = atmo.get_tiles(SMALL)
+    assert (t["sd_nu_per_thread"], t["sd_block_threads"], t["ms_texels_per_thread"]) == \
+        (1, 256, 2)
+    assert t["sd_tuned"] and t["ms_tuned"]
+    # The density class is (MU, NU) and the multiple-scattering class
+    # (MU, MU_S, NU): other R, other MU_S.
+    other = dict(SMALL, scattering_r=32, scattering_mu_s=32)
+    t2 = atmo.get_tiles(other)
+    assert (t2["sd_nu_per_thread"], t2["sd_block_threads"]) == (1, 256)
+    assert t2["sd_tuned"] and not t2["ms_tuned"] and t2["ms_texels_per_thread"] == 1
+    assert atmo.get_tiles()["sd_nu_per_thread"] == 4  # default class untouched
+
+
+@pytest.mark.parametrize("shape", [(3, 256, 1), (4, 96, 1), (4, 256, 3), (4, 512, 1)])
+def test_unknown_shapes_rejected(shape):
+    with pytest.raises(AtmoError):
+        atmo.set_tiles(None, *shape)
+
+
+def test_multiple_scattering_tile_must_hold_the_row():
+    with pytest.raises(AtmoError):
+        atmo.set_tiles(WIDE, 4, 256, 2)  # 1024 texels > 2 x 256
+    atmo.set_tiles(WIDE, 4, 256, 4)
+    # Gather-path rows take any shape.
+    atmo.set_tiles(LONG, 4, 256, 1)
+
+
+def test_save_load_round_trip(tmp_path):
+    atmo.set_tiles(SMALL, 2, 128, 4)
+    atmo.set_tiles(WIDE, 4, 64, 4)
+    path = tmp_path / "tiles.txt"
+    atmo.save_tiles(path)
+    before = {k: atmo.get_tiles(z) for k, z in (("s", SMALL), ("w", WIDE), ("d", None))}
+    atmo.reset_tiles()
+    assert not atmo.get_tiles(SMALL)["sd_tuned"]
+    atmo.load_tiles(path)
+    after = {k: atmo.get_tiles(z) for k, z in (("s", SMALL), ("w", WIDE), ("d", None))}
+    assert after == before
+    lines = [l for l in path.read_text().splitlines() if not l.startswith("#")]
+    assert sorted(lines) == sorted(["sd 64 4 2 128", "sd 128 16 4 64",
+                                    "ms 64 16 4 4", "ms 128 64 16 4"])
+
+
+@pytest.mark.parametrize("text", ["sd 64 4 3 256\n", "ms 128 64 16 1\n", "xx 1 2 3 4\n",
+                                  "sd 64 4\n"])
+def test_load_rejects_bad_entries_atomically(tmp_path, text):
+    path = tmp_path / "bad.txt"
+    path.write_text("sd 128 8 1 256\n" + text)
+    with pytest.raises(AtmoError):
+        atmo.load_tiles(path)
+    assert not atmo.get_tiles()["sd_tuned"]  # nothing merged
+
+
+def test_null_and_missing_arguments(tmp_path):
+    L = atmo.lib()
+    assert L.atmo_get_tiles(None, None) != 0
+    assert L.atmo_set_tiles(None, None) != 0
+    assert L.atmo_tune_save(None) != 0
+    assert L.atmo_tune_load(None) != 0
+    assert L.atmo_autotune(None, -1, 0, None) != 0
+    with pytest.raises(AtmoError):
+        atmo.load_tiles(tmp_path / "missing.txt")
