@@ -515,7 +515,7 @@ def main():
 
     import numpy as np
     import precomputed_atmospheric_scattering_amd as atmo
-    from precomputed_atmospheric_scattering_amd import configs
+    from precomputed_atmospheric_scattering_amd import configs, frames
 
     if args.copy_probe > 0:
         nbytes = args.copy_probe << 20
@@ -598,27 +598,44 @@ def main():
     atmo.synchronize()
     elapsed = time.perf_counter() - t_start
 
-    # The same three lookups on a coherent batch: the rays of one 1920x1080
-    # camera frame (neighbouring pixels share texels), outside the timed
-    # steps.
-    fq = configs.frame_queries(WIDTH, HEIGHT, kw["bottom_radius"] / 1000.0,
-                               kw["top_radius"] / 1000.0)
-    fdev = {k: atmo.DeviceArray.from_host(v) for k, v in fq.items()}
+    # The same lookups on coherent batches, outside the timed steps: the
+    # frames of the demo's nine preset views at 1920x1080 (the scene of the
+    # reference's demo, a sphere on the planet with light shafts; views from
+    # 7 km to 12000 km away). Per frame: GetSkyLuminance for every pixel,
+    # GetSunAndSkyIlluminance and GetSkyLuminanceToPoint for the pixels
+    # that hit the sphere or the ground.
+    frame_ms = {}
+    frame_rays = {}
     fev = [atmo.Event() for _ in range(2)]
     frame_reps = 5
-    fev[0].record(stream)
-    for _ in range(frame_reps):
-        model.sky_luminance_device(n_rays, fdev["camera"], fdev["view_ray"],
-                                   fdev["shadow_length"], fdev["sun_direction"],
-                                   outs[0], outs[1], stream)
-        model.sky_luminance_to_point_device(n_rays, fdev["camera"], fdev["point"],
-                                            fdev["shadow_length"], fdev["sun_direction"],
-                                            outs[2], outs[3], stream)
-        model.sun_and_sky_illuminance_device(n_rays, fdev["point"], fdev["normal"],
-                                             fdev["sun_direction"], outs[4], outs[5], stream)
-    fev[1].record(stream)
-    stream.synchronize()
-    frame_ms = fev[0].elapsed_ms(fev[1]) / frame_reps
+    for view in sorted(frames.DEMO_VIEWS):
+        fr, _ = frames.demo_frame(view, kw["bottom_radius"] / 1000.0,
+                                  kw["sun_angular_radius"], WIDTH, HEIGHT)
+        fq = fr.queries()
+        hit = {k: np.concatenate([fq["sphere"][k], fq["ground"][k]])
+               for k in ("camera", "point", "normal", "shadow", "sun")}
+        nh = len(hit["point"])
+        sky = {k: atmo.DeviceArray.from_host(fq["sky"][k])
+               for k in ("camera", "view", "shadow", "sun")}
+        hd = {k: atmo.DeviceArray.from_host(v) for k, v in hit.items()} if nh else None
+        # A pixel can hit both the sphere and the ground behind it.
+        hout = [atmo.DeviceArray((max(nh, 1), 3)) for _ in range(4)]
+        fev[0].record(stream)
+        for _ in range(frame_reps):
+            model.sky_luminance_device(n_rays, sky["camera"], sky["view"], sky["shadow"],
+                                       sky["sun"], outs[0], outs[1], stream)
+            if nh:
+                model.sun_and_sky_illuminance_device(nh, hd["point"], hd["normal"],
+                                                     hd["sun"], hout[0], hout[1], stream)
+                model.sky_luminance_to_point_device(nh, hd["camera"], hd["point"],
+                                                    hd["shadow"], hd["sun"], hout[2],
+                                                    hout[3], stream)
+        fev[1].record(stream)
+        stream.synchronize()
+        frame_ms[view] = fev[0].elapsed_ms(fev[1]) / frame_reps
+        frame_rays[view] = n_rays + 2 * nh
+        for d in list(sky.values()) + (list(hd.values()) if hd else []) + hout:
+            d.free()
 
     steps = args.steps
     iters = model.params.num_iterations()
@@ -664,8 +681,13 @@ def main():
             ("sky_luminance", "sky_luminance_to_point", "sun_and_sky_illuminance"),
             lookup_ms)},
         "lookup_roofline_frac": round(q_frac, 4),
-        "frame_lookup_ms": round(frame_ms, 4),
-        "frame_sky_rays_per_s": round(3.0 * n_rays / (frame_ms * 1e-3), 1),
+        "demo_frames": {
+            "frames_per_s": round(len(frame_ms) / (sum(frame_ms.values()) * 1e-3), 1),
+            "lookups_per_s": round(sum(frame_rays.values()) /
+                                   (sum(frame_ms.values()) * 1e-3), 1),
+            "ms_per_frame": {str(v): round(t, 4) for v, t in frame_ms.items()},
+            "lookups_per_frame": {str(v): n for v, n in frame_rays.items()},
+        },
         "pass_ms_per_atmosphere": {k: round(v / steps, 4) for k, v in pass_ms.items()},
         "tiles": {"scattering_density": "%d nu x %d threads" % (
                       tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),

@@ -246,39 +246,3 @@ def hashed_sky_queries(seed, n, bottom_radius_units, top_radius_units, max_altit
     f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
     return dict(camera=f(cam), view_ray=f(v), sun_direction=f(sun), point=f(pt),
                 normal=f(normal), shadow_length=np.zeros(n, np.float32))
-
-
-def frame_queries(width, height, bottom_radius_units, top_radius_units, sun_zenith_deg=65.0,
-                  camera_altitude=0.5, pitch=math.pi / 30.0, fov_y_deg=50.0):
-    """Coherent query batch: the rays of one width x height pinhole-camera
-    frame (the camera of the reference's image tests, model_test.cc:436-476:
-    500 m above the ground, pitched up by pi/30, 50 degree vertical field of
-    view), in length units relative to the planet centre. view_ray is the
-    pixel's ray; point is where it meets the ground, or 50 units along it
-    (moved inside the atmosphere's shell); normal is the local vertical at
-    the point; shadow length 0. Neighbouring pixels share LUT texels, as in
-    a renderer, unlike the uniformly random rays of sky_queries."""
-    t = math.tan(math.radians(fov_y_deg) / 2.0)
-    j, i = np.mgrid[0:height, 0:width]
-    x = (2.0 * (i + 0.5) / width - 1.0) * t * width / height
-    y = (1.0 - 2.0 * (j + 0.5) / height) * t
-    sp, cp = math.sin(pitch), math.cos(pitch)
-    v = np.stack([x, -sp * y + cp, cp * y + sp], -1).reshape(-1, 3)
-    v /= np.linalg.norm(v, axis=1, keepdims=True)
-    n = width * height
-    cam = np.zeros((n, 3))
-    cam[:, 2] = bottom_radius_units + camera_altitude
-    rmu = v[:, 2] * cam[:, 2]
-    disc = rmu * rmu - cam[:, 2] ** 2 + bottom_radius_units ** 2
-    ground = (rmu < 0) & (disc >= 0)
-    d = np.where(ground, -rmu - np.sqrt(np.maximum(disc, 0.0)), 50.0)
-    pt = cam + v * d[:, None]
-    r = np.linalg.norm(pt, axis=1)
-    lo, hi = bottom_radius_units + 1e-3, top_radius_units - 1e-3
-    pt *= (np.clip(r, lo, hi) / r)[:, None]
-    normal = pt / np.linalg.norm(pt, axis=1, keepdims=True)
-    th = math.radians(sun_zenith_deg)
-    sun = np.broadcast_to(np.array([0.0, math.sin(th), math.cos(th)]), (n, 3))
-    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
-    return dict(camera=f(cam), view_ray=f(v), sun_direction=f(sun), point=f(pt),
-                normal=f(normal), shadow_length=np.zeros(n, np.float32))

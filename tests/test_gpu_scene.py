@@ -4,10 +4,13 @@ model_test.cc:806-870), rendered at 640x360 from the GPU lookups on the GPU
 LUTs and from the FP64 oracle's lookups on the oracle's full-chain LUTs, for
 the reference's radiance test cases. The bars are the reference's own PSNR
 thresholds for its GPU-vs-CPU comparisons (47, 46 and 40 dB; model_test.cc:
-820-866). The geometry is shared (tests/scene.py), so the renders differ
-only by the lookups and the LUTs behind them. Constant albedos (ground 0.1,
-sphere 0.8), as in the reference's constant-albedo cases."""
+820-866). The geometry is shared (the package's frames module), so the
+renders differ only by the lookups and the LUTs behind them. Constant
+albedos (ground 0.1, sphere 0.8), as in the reference's constant-albedo
+cases. The demo's nine preset views (demo.cc:471-489, including the two
+from space) are rendered the same way at 192x108 against the lowest bar."""
 import ctypes as C
+import math
 
 import numpy as np
 import pytest
@@ -15,8 +18,8 @@ import pytest
 import oracle_lib as ol
 import parity
 import precomputed_atmospheric_scattering_amd as atmo
-import scene
 from precomputed_atmospheric_scattering_amd import configs
+from precomputed_atmospheric_scattering_amd import frames
 
 pytestmark = pytest.mark.gpu
 
@@ -90,17 +93,47 @@ def test_scene_psnr(oracle_luts, name, combined, sun_theta, bar):
     kw, orders = configs.model_kwargs(c)
     m = atmo.Model(**kw)
     m.init(orders)
-    sc = scene.Scene(kw["bottom_radius"] / 1000.0, kw["sun_angular_radius"],
-                     scene.sun_direction(sun_theta, 90.0))
+    sc = frames.Frame(kw["bottom_radius"] / 1000.0, kw["sun_angular_radius"],
+                      frames.sun_direction(math.radians(sun_theta), math.pi / 2.0),
+                      frames.Camera.image_test())
     q = sc.queries()
     solar = m.solar_radiance()
     img_gpu = sc.composite(q, gpu_lookups(q, m), solar, GROUND_ALBEDO, SPHERE_ALBEDO)
     img_ref = sc.composite(q, oracle_lookups(q, spec, s, luts, combined), solar,
                            GROUND_ALBEDO, SPHERE_ALBEDO)
     assert np.isfinite(img_gpu).all() and np.isfinite(img_ref).all()
-    p = scene.psnr(scene.to_8bit(img_gpu, EXPOSURE), scene.to_8bit(img_ref, EXPOSURE))
+    p = frames.psnr(frames.to_8bit(img_gpu, EXPOSURE), frames.to_8bit(img_ref, EXPOSURE))
     lin = np.abs(img_gpu - img_ref).max() / np.abs(img_ref).max()
     print("%s: PSNR %.1f dB (bar %.0f), max linear difference %.2e of the image maximum"
           % (name, p, bar, lin), flush=True)
     assert p > bar
     m.close()
+
+
+@pytest.fixture(scope="module")
+def gpu_model(oracle_luts):
+    cfg = oracle_luts[0]
+    kw, orders = configs.model_kwargs(cfg)
+    m = atmo.Model(**kw)
+    m.init(orders)
+    yield m
+    m.close()
+
+
+@pytest.mark.parametrize("view", sorted(frames.DEMO_VIEWS))
+def test_demo_view_psnr(oracle_luts, gpu_model, view):
+    cfg, spec, s, luts = oracle_luts
+    kw, _ = configs.model_kwargs(cfg)
+    fr, exposure = frames.demo_frame(view, kw["bottom_radius"] / 1000.0,
+                                     kw["sun_angular_radius"], 192, 108)
+    q = fr.queries()
+    solar = gpu_model.solar_radiance()
+    img_gpu = fr.composite(q, gpu_lookups(q, gpu_model), solar, GROUND_ALBEDO,
+                           SPHERE_ALBEDO)
+    img_ref = fr.composite(q, oracle_lookups(q, spec, s, luts, False), solar,
+                           GROUND_ALBEDO, SPHERE_ALBEDO)
+    assert np.isfinite(img_gpu).all() and np.isfinite(img_ref).all()
+    p = frames.psnr(frames.to_8bit(img_gpu, exposure), frames.to_8bit(img_ref, exposure))
+    print("demo view %d: PSNR %.1f dB, %d sphere and %d ground pixels"
+          % (view, p, len(q["sphere"]["index"]), len(q["ground"]["index"])), flush=True)
+    assert p > 40.0

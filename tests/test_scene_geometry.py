@@ -1,13 +1,23 @@
-"""CPU checks of the test-scene geometry (tests/scene.py) and of a render
-through the oracle's lookups on small LUTs, so that the GPU image test only
-has to compare lookups."""
+"""CPU checks of the frame geometry (the package's frames module: the
+image tests' camera and the demo's preset views) and of a render through
+the oracle's lookups on small LUTs, so that the GPU image tests only have to
+compare lookups."""
+import math
+
 import numpy as np
+import pytest
 
 import parity
-import scene
 import test_gpu_scene as tgs
 import oracle_lib as ol
 from precomputed_atmospheric_scattering_amd import configs
+from precomputed_atmospheric_scattering_amd import frames
+
+
+def image_test_frame(sun_theta=65.0):
+    return frames.Frame(6360.0, 0.00935,
+                        frames.sun_direction(math.radians(sun_theta), math.pi / 2.0),
+                        frames.Camera.image_test())
 
 TINY = dict(transmittance_width=32, transmittance_height=8, scattering_r=4,
             scattering_mu=16, scattering_mu_s=4, scattering_nu=4,
@@ -15,7 +25,7 @@ TINY = dict(transmittance_width=32, transmittance_height=8, scattering_r=4,
 
 
 def test_geometry():
-    sc = scene.Scene(6360.0, 0.00935, scene.sun_direction(65.0, 90.0))
+    sc = image_test_frame()
     assert sc.sphere_hit.sum() > 1000 and sc.ground_hit.sum() > 10000
     a = sc.sphere_alpha[sc.sphere_hit]
     assert (a > 0).all() and (a <= 1).all()
@@ -30,7 +40,7 @@ def test_geometry():
     # View rays are unit vectors; the image centre looks along +y, pitched
     # up by pi / 30 (6 degrees).
     assert np.allclose(np.linalg.norm(sc.view, axis=-1), 1.0)
-    c = sc.view[scene.HEIGHT // 2, scene.WIDTH // 2]
+    c = sc.view[sc.height // 2, sc.width // 2]
     assert abs(np.degrees(np.arcsin(c[2])) - 6.0) < 0.2 and c[1] > 0.99
 
 
@@ -38,9 +48,44 @@ def test_oracle_render_is_finite():
     cfg = configs.earth(lut_sizes=TINY, orders=2)
     spec, s, orders = parity.oracle_spec(cfg)
     luts = ol.model_init(spec, s, orders)
-    sc = scene.Scene(6360.0, 0.00935, scene.sun_direction(65.0, 90.0))
+    sc = image_test_frame()
     q = sc.queries()
     img = sc.composite(q, tgs.oracle_lookups(q, spec, s, luts, False), (1.0, 1.0, 1.0),
                        tgs.GROUND_ALBEDO, tgs.SPHERE_ALBEDO)
     assert np.isfinite(img).all() and img.min() >= 0.0
-    assert scene.psnr(scene.to_8bit(img, 10.0), scene.to_8bit(img, 10.0)) == float("inf")
+    assert frames.psnr(frames.to_8bit(img, 10.0), frames.to_8bit(img, 10.0)) == float("inf")
+
+
+@pytest.mark.parametrize("view", sorted(frames.DEMO_VIEWS))
+def test_demo_views(view):
+    dist, vz, va, sz, sa, _ = frames.DEMO_VIEWS[view]
+    fr, exposure = frames.demo_frame(view, 6360.0, 0.00935, 96, 54)
+    cam = fr.camera.position
+    # The camera orbits the origin at the view distance, at the view zenith
+    # angle, and the image centre looks back at the origin.
+    assert abs(np.linalg.norm(cam) - dist / 1000.0) < 1e-9 * max(1.0, dist)
+    assert abs(cam[2] - dist / 1000.0 * math.cos(vz)) < 1e-6 * max(1.0, dist)
+    centre = 0.25 * (fr.view[26, 47] + fr.view[27, 47] + fr.view[26, 48] + fr.view[27, 48])
+    assert np.dot(centre / np.linalg.norm(centre), -cam / np.linalg.norm(cam)) > 0.999
+    assert np.allclose(fr.sun, frames.sun_direction(sz, sa))
+    q = fr.queries()
+    n = 96 * 54
+    assert len(q["sky"]["camera"]) == n
+    for name in ("sphere", "ground"):
+        g = q[name]
+        assert len(g["point"]) == len(g["index"]) <= n
+        r = np.linalg.norm(g["point"].astype(np.float64), axis=1)
+        assert (r > 6359.99).all()
+    # From 9 km away the sphere is in view. From 12000 km straight above
+    # (view 9) the planet is a disc of half-angle asin(6360 / 18360) = 20.3
+    # degrees around the image centre: the centre hits the ground, the
+    # corners do not.
+    if dist < 1e5:
+        assert len(q["sphere"]["index"]) > 0 and len(q["ground"]["index"]) > 0
+    if view == 9:
+        hit = fr.ground_hit
+        assert hit[27, 48] and not hit[0, 0] and not hit[-1, -1]
+        half = math.asin(6360.0 / 18360.0)
+        ang = np.arccos(np.clip(fr.view @ np.array([0.0, 0.0, -1.0]), -1.0, 1.0))
+        assert (hit == (ang < half))[np.abs(ang - half) > 0.01].all()
+    assert 0.0 <= fr.fadein <= 1.0
