@@ -23,14 +23,19 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
+# The precompute kernels contract no multiply-add by themselves: each FMA in
+# them is written as fma()/fmaf(). Implicit contraction depends on how many
+# uses a product has, which differs between the tile-shape instantiations of
+# one kernel (several nu outputs per thread share products), and the tile
+# shapes must give bit-identical LUTs.
 SOURCES = [
-    ("precompute.hip", True),
-    ("query.hip", True),
-    ("runtime.cpp", True),   # host code using the HIP runtime
-    ("server.cpp", True),    # serving loop (worker thread, HIP runtime)
-    ("lut_io.cpp", True),    # LUT files (export, import, cached Init)
-    ("autotune.cpp", True),  # tile-shape autotuner
-    ("model.cc", False),     # C++ Model facade, host only
+    ("precompute.hip", True, ["-ffp-contract=off"]),
+    ("query.hip", True, []),
+    ("runtime.cpp", True, []),   # host code using the HIP runtime
+    ("server.cpp", True, []),    # serving loop (worker thread, HIP runtime)
+    ("lut_io.cpp", True, []),    # LUT files (export, import, cached Init)
+    ("autotune.cpp", True, []),  # tile-shape autotuner
+    ("model.cc", False, []),     # C++ Model facade, host only
 ]
 
 
@@ -42,8 +47,8 @@ def hipcc():
 
 
 def _compile(args):
-    src, obj, is_hip = args
-    cmd = [hipcc()] + (["-x", "hip"] if is_hip else []) + HIP_FLAGS + [
+    src, obj, is_hip, extra = args
+    cmd = [hipcc()] + (["-x", "hip"] if is_hip else []) + HIP_FLAGS + extra + [
         "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
@@ -56,11 +61,11 @@ def _compile(args):
 def build(verbose=False):
     with tempfile.TemporaryDirectory(prefix="atmo_build_") as tmp:
         jobs = []
-        for name, is_hip in SOURCES:
+        for name, is_hip, extra in SOURCES:
             src = os.path.join(CSRC, name)
             if not os.path.exists(src):
                 continue
-            jobs.append((src, os.path.join(tmp, name + ".o"), is_hip))
+            jobs.append((src, os.path.join(tmp, name + ".o"), is_hip, extra))
         with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
             outs = list(ex.map(_compile, jobs))
         if verbose:

@@ -222,9 +222,9 @@ __device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<float>&
   const float4 a = gload32(T, r0 + i0), b = gload32(T, r0 + i1);
   const float4 c = gload32(T, r1 + i0), d = gload32(T, r1 + i1);
   const float wy1 = S.wy, wy0 = 1.0f - S.wy;
-  o[0] = wy0 * fmaf(w, b.x - a.x, a.x) + wy1 * fmaf(w, d.x - c.x, c.x);
-  o[1] = wy0 * fmaf(w, b.y - a.y, a.y) + wy1 * fmaf(w, d.y - c.y, c.y);
-  o[2] = wy0 * fmaf(w, b.z - a.z, a.z) + wy1 * fmaf(w, d.z - c.z, c.z);
+  o[0] = fmaf(wy1, fmaf(w, d.x - c.x, c.x), wy0 * fmaf(w, b.x - a.x, a.x));
+  o[1] = fmaf(wy1, fmaf(w, d.y - c.y, c.y), wy0 * fmaf(w, b.y - a.y, a.y));
+  o[2] = fmaf(wy1, fmaf(w, d.z - c.z, c.z), wy0 * fmaf(w, b.z - a.z, a.z));
 }
 
 __device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<double>& S,
@@ -243,9 +243,9 @@ __device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<double>
   const float4 a = gload32(T, r0 + i0), b = gload32(T, r0 + i1);
   const float4 c = gload32(T, r1 + i0), d = gload32(T, r1 + i1);
   const double wy1 = S.wy, wy0 = 1.0 - S.wy;
-  o[0] = wy0 * fma(w, double(b.x) - a.x, double(a.x)) + wy1 * fma(w, double(d.x) - c.x, double(c.x));
-  o[1] = wy0 * fma(w, double(b.y) - a.y, double(a.y)) + wy1 * fma(w, double(d.y) - c.y, double(c.y));
-  o[2] = wy0 * fma(w, double(b.z) - a.z, double(a.z)) + wy1 * fma(w, double(d.z) - c.z, double(c.z));
+  o[0] = fma(wy1, fma(w, double(d.x) - c.x, double(c.x)), wy0 * fma(w, double(b.x) - a.x, double(a.x)));
+  o[1] = fma(wy1, fma(w, double(d.y) - c.y, double(c.y)), wy0 * fma(w, double(b.y) - a.y, double(a.y)));
+  o[2] = fma(wy1, fma(w, double(d.z) - c.z, double(c.z)), wy0 * fma(w, double(b.z) - a.z, double(a.z)));
 }
 
 // kF64: the kernel serves the jobs with FP64 accumulators (illuminance
@@ -337,14 +337,14 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
     mus = mus < Real(-1) ? Real(-1) : (mus > Real(1) ? Real(1) : mus);
     Real u = (xs + S.e) * S.inv_span;
     u = u < Real(0) ? Real(0) : (u > Real(1) ? Real(1) : u);
-    const Real vis = u * u * (Real(3) - Real(2) * u);
+    const Real vis = u * u * fma(Real(-2), u, Real(3));
     Real ts[3];
     tsun_lookup(J, S, mus, ts);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const Real f = ts[c] * vis;
-      ar[c] += f * S.fr[c];
-      am[c] += f * S.fm[c];
+      ar[c] = fma(f, S.fr[c], ar[c]);
+      am[c] = fma(f, S.fm[c], am[c]);
     }
   }
   double ray[3], mie[3];
@@ -483,6 +483,7 @@ __device__ __forceinline__ void sd_theta(
       for (int sgn = 0; sgn < 2; ++sgn) {
         const float t = sgn == 0 ? fmaf(tC[j], s, tb) : fmaf(-tC[j], s, tb);
         float3 v = table_lerp(tab_a, nu_max, t);
+        float x1 = 1.0f;  // order 2: the Rayleigh phase factor 1 + nu1^2
         if (kOrder2) {
           // Order-1 incident radiance: Rayleigh and Mie single scattering
           // times their phase functions of nu1 (constants folded in tables).
@@ -490,21 +491,32 @@ __device__ __forceinline__ void sd_theta(
           const float3 vm = table_lerp(tab_b, nu_max, t);
           const float r1 = rsq_f(fmaf(mg2, nu1, g1));
           const float q = r1 * r1 * r1;
-          const float x1 = fmaf(nu1, nu1, 1.0f);
-          v.x = x1 * fmaf(vm.x, q, v.x);
-          v.y = x1 * fmaf(vm.y, q, v.y);
-          v.z = x1 * fmaf(vm.z, q, v.z);
+          x1 = fmaf(nu1, nu1, 1.0f);
+          v.x = fmaf(vm.x, q, v.x);
+          v.y = fmaf(vm.y, q, v.y);
+          v.z = fmaf(vm.z, q, v.z);
         }
         if (kGround) {
+          if (kOrder2) {
+            v.x = x1 * v.x;
+            v.y = x1 * v.y;
+            v.z = x1 * v.z;
+            x1 = 1.0f;
+          }
           const float3 e = table_lerp(etab, e_max, fmaf(eB, t, eA));
           v.x = fmaf(th.gf[0], e.x, v.x);
           v.y = fmaf(th.gf[1], e.y, v.y);
           v.z = fmaf(th.gf[2], e.z, v.z);
         }
+        // l += x1 * v (x1 = 1 unless order 2 without ground term).
         if (sgn == 0) {
-          l0 = v.x;
-          l1 = v.y;
-          l2 = v.z;
+          l0 = kOrder2 && !kGround ? x1 * v.x : v.x;
+          l1 = kOrder2 && !kGround ? x1 * v.y : v.y;
+          l2 = kOrder2 && !kGround ? x1 * v.z : v.z;
+        } else if (kOrder2 && !kGround) {
+          l0 = fmaf(x1, v.x, l0);
+          l1 = fmaf(x1, v.y, l1);
+          l2 = fmaf(x1, v.z, l2);
         } else {
           l0 += v.x;
           l1 += v.y;
@@ -792,13 +804,14 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
       const float r1 = rsq_f(fmaf(mg2, nu, g1));
       const float q = r1 * r1 * r1;
       const float x1 = fmaf(nu, nu, 1.0f);
-      v.x = x1 * fmaf(vm.x, q, v.x);
-      v.y = x1 * fmaf(vm.y, q, v.y);
-      v.z = x1 * fmaf(vm.z, q, v.z);
+      a0 = fmaf(x1, fmaf(vm.x, q, v.x), a0);
+      a1 = fmaf(x1, fmaf(vm.y, q, v.y), a1);
+      a2 = fmaf(x1, fmaf(vm.z, q, v.z), a2);
+    } else {
+      a0 += v.x;
+      a1 += v.y;
+      a2 += v.z;
     }
-    a0 += v.x;
-    a1 += v.y;
-    a2 += v.z;
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {

@@ -1088,6 +1088,7 @@ struct atmo_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
+  atmo_rt::Fork fork;  // concurrent per-tile launches of one level
 };
 
 namespace atmo_rt {
@@ -1185,35 +1186,85 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
   add_level(kHalf, 0, half);
 }
 
+atmo_status create_fork(Fork* f) {
+  for (hipStream_t& st : f->side)
+    ATMO_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  for (hipEvent_t& e : f->ev) ATMO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return ATMO_OK;
+}
+
+void destroy_fork(Fork* f) {
+  for (hipStream_t& st : f->side) {
+    if (st != nullptr) (void)hipStreamDestroy(st);
+    st = nullptr;
+  }
+  for (hipEvent_t& e : f->ev) {
+    if (e != nullptr) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+}
+
+namespace {
+
+atmo_status launch_level(const Level& l, const atmo::Job* d_jobs, const int* d_first_block,
+                         hipStream_t s) {
+  atmo::PassLaunch pl;
+  pl.jobs = d_jobs + l.job_offset;
+  pl.first_block = d_first_block + l.fb_offset;
+  pl.njobs = l.njobs;
+  pl.total_blocks = l.total_blocks;
+  pl.max_nnu = l.max_nnu;
+  pl.max_ew = l.max_ew;
+  pl.max_w = l.max_w;
+  pl.min_w = l.min_w;
+  pl.max_lds_w = l.max_lds_w;
+  pl.kinds = l.kinds;
+  pl.tile = l.tile;
+  hipError_t e = hipSuccess;
+  switch (l.pass) {
+    case kT: e = atmo::launch_transmittance(pl, s); break;
+    case kE0: e = atmo::launch_direct_irradiance(pl, s); break;
+    case kSS: e = atmo::launch_single_scattering(pl, s); break;
+    case kSD: e = atmo::launch_scattering_density(pl, l.order, s); break;
+    case kII: e = atmo::launch_indirect_irradiance(pl, l.order, s); break;
+    case kMS: e = atmo::launch_multiple_scattering(pl, s); break;
+    case kClear: e = atmo::launch_clear_max(pl, s); break;
+    case kAcc: e = atmo::launch_acc64_to_f32(pl, s); break;
+    case kHalf: e = atmo::launch_to_half(pl, s); break;
+    default: break;
+  }
+  return check_launch(e, "batched pass");
+}
+
+}  // namespace
+
 atmo_status enqueue_levels(const LevelPlan& plan, const atmo::Job* d_jobs,
-                           const int* d_first_block, hipStream_t s) {
-  for (const Level& l : plan.levels) {
-    atmo::PassLaunch pl;
-    pl.jobs = d_jobs + l.job_offset;
-    pl.first_block = d_first_block + l.fb_offset;
-    pl.njobs = l.njobs;
-    pl.total_blocks = l.total_blocks;
-    pl.max_nnu = l.max_nnu;
-    pl.max_ew = l.max_ew;
-    pl.max_w = l.max_w;
-    pl.min_w = l.min_w;
-    pl.max_lds_w = l.max_lds_w;
-    pl.kinds = l.kinds;
-    pl.tile = l.tile;
-    hipError_t e = hipSuccess;
-    switch (l.pass) {
-      case kT: e = atmo::launch_transmittance(pl, s); break;
-      case kE0: e = atmo::launch_direct_irradiance(pl, s); break;
-      case kSS: e = atmo::launch_single_scattering(pl, s); break;
-      case kSD: e = atmo::launch_scattering_density(pl, l.order, s); break;
-      case kII: e = atmo::launch_indirect_irradiance(pl, l.order, s); break;
-      case kMS: e = atmo::launch_multiple_scattering(pl, s); break;
-      case kClear: e = atmo::launch_clear_max(pl, s); break;
-      case kAcc: e = atmo::launch_acc64_to_f32(pl, s); break;
-      case kHalf: e = atmo::launch_to_half(pl, s); break;
-      default: break;
+                           const int* d_first_block, hipStream_t s, const Fork* fork) {
+  const std::vector<Level>& L = plan.levels;
+  for (size_t i = 0; i < L.size();) {
+    // [i, j): the launches of one level (same pass and order, one per tile
+    // shape among its jobs; plan_levels emits them next to each other).
+    size_t j = i + 1;
+    while (j < L.size() && L[j].pass == L[i].pass && L[j].order == L[i].order &&
+           (L[i].pass == kSD || L[i].pass == kMS))
+      ++j;
+    const int k = static_cast<int>(j - i);
+    if (fork == nullptr || k == 1) {
+      for (size_t q = i; q < j; ++q) ATMO_TRY(launch_level(L[q], d_jobs, d_first_block, s));
+    } else {
+      // Fork: side streams start after everything enqueued so far on s,
+      // and s continues after all of them.
+      ATMO_HIP(hipEventRecord(fork->ev[0], s));
+      for (int q = 1; q < k; ++q) {
+        hipStream_t side = fork->side[q - 1];
+        ATMO_HIP(hipStreamWaitEvent(side, fork->ev[0], 0));
+        ATMO_TRY(launch_level(L[i + q], d_jobs, d_first_block, side));
+        ATMO_HIP(hipEventRecord(fork->ev[q], side));
+      }
+      ATMO_TRY(launch_level(L[i], d_jobs, d_first_block, s));
+      for (int q = 1; q < k; ++q) ATMO_HIP(hipStreamWaitEvent(s, fork->ev[q], 0));
     }
-    ATMO_TRY(check_launch(e, "batched pass"));
+    i = j;
   }
   return ATMO_OK;
 }
@@ -1228,6 +1279,7 @@ void destroy_batch(atmo_batch* b) {
   if (b->graph != nullptr) (void)hipGraphDestroy(b->graph);
   if (b->d_jobs != nullptr) (void)hipFree(b->d_jobs);
   if (b->d_first_block != nullptr) (void)hipFree(b->d_first_block);
+  atmo_rt::destroy_fork(&b->fork);
   delete b;
 }
 
@@ -1266,12 +1318,14 @@ atmo_status atmo_batch_create(atmo_model* const* models, const unsigned* orders,
                      hipMemcpyHostToDevice));
   ATMO_HIP(hipMemcpy(b->d_first_block, plan.first_block.data(),
                      plan.first_block.size() * sizeof(int), hipMemcpyHostToDevice));
+  ATMO_TRY(create_fork(&b->fork));
   if (b->use_graph) {
     hipStream_t cap;
     ATMO_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     atmo_status st = ATMO_OK;
-    if (e == hipSuccess) st = enqueue_levels(plan, b->d_jobs, b->d_first_block, cap);
+    if (e == hipSuccess)
+      st = enqueue_levels(plan, b->d_jobs, b->d_first_block, cap, &b->fork);
     hipError_t e2 = hipStreamEndCapture(cap, &b->graph);
     (void)hipStreamDestroy(cap);
     if (e != hipSuccess || e2 != hipSuccess || st != ATMO_OK)
@@ -1289,7 +1343,7 @@ atmo_status atmo_batch_launch(atmo_batch* b, void* stream) {
   if (b->use_graph) {
     ATMO_HIP(hipGraphLaunch(b->exec, s));
   } else {
-    ATMO_TRY(enqueue_levels(b->plan, b->d_jobs, b->d_first_block, s));
+    ATMO_TRY(enqueue_levels(b->plan, b->d_jobs, b->d_first_block, s, &b->fork));
   }
   for (atmo_model* m : b->models) m->initialized = true;
   return ATMO_OK;

@@ -132,9 +132,23 @@ void drop_external_temporaries(atmo_model* m);
 // Plan the levels of the models' Init (their job tables must be current).
 void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
                  LevelPlan* plan);
+// Streams and events on which the per-tile launches of one pass level run
+// side by side: a level split into k launches (one per tile shape) forks
+// k - 1 of them onto side[0..k-2] and joins them back. Default priority,
+// non-blocking streams; ev holds kMaxFork events.
+constexpr int kMaxFork = atmo::kSdTiles;
+struct Fork {
+  hipStream_t side[kMaxFork - 1] = {};
+  hipEvent_t ev[kMaxFork] = {};
+};
+atmo_status create_fork(Fork* f);
+void destroy_fork(Fork* f);
 // Launch a plan whose jobs and block prefixes are at d_jobs / d_first_block.
+// With a fork, the launches of one level that differ only in tile shape run
+// concurrently; without, one after the other.
 atmo_status enqueue_levels(const LevelPlan& plan, const atmo::Job* d_jobs,
-                           const int* d_first_block, hipStream_t stream);
+                           const int* d_first_block, hipStream_t stream,
+                           const Fork* fork = nullptr);
 
 }  // namespace atmo_rt
 
