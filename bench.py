@@ -453,16 +453,18 @@ def run_high_resolution(args):
                 "scattering_density_2" if order == 2 else "scattering_density_n")
             pass_ms[key] = pass_ms.get(key, 0.0) + t
             pass_n[key] = pass_n.get(key, 0) + 1
-    # Compulsory bytes per launch (FP32 float4 texels of the delta LUTs):
-    # SD order 2 reads delta Rayleigh + Mie + irradiance and writes the
-    # density; SD order >= 3 reads the multiple-scattering delta; MS reads the
-    # density and transmittance, writes its delta and updates the FP32
-    # scattering accumulator (read + write).
+    # Compulsory bytes per launch (FP32 float4 texels of the delta LUTs),
+    # each launch covering the 5 Precompute iterations: SD order 2 reads
+    # delta Rayleigh + Mie + irradiance and writes the density; SD order >= 3
+    # reads the multiple-scattering delta; MS reads the density and
+    # transmittance, writes its delta and updates the iteration's FP64
+    # scattering accumulator (32 B read + 32 B write per texel).
     tex = 16.0
+    iters = m.params.num_iterations()
     bytes_per = {
-        "scattering_density_2": (3 * n_s + n_e) * tex,
-        "scattering_density_n": (2 * n_s + n_e) * tex,
-        "multiple_scattering": (4 * n_s + n_t) * tex,
+        "scattering_density_2": iters * (3 * n_s + n_e) * tex,
+        "scattering_density_n": iters * (2 * n_s + n_e) * tex,
+        "multiple_scattering": iters * ((2 * n_s + n_t) * tex + 64.0 * n_s),
     }
     gbs = {k: round(bytes_per[k] * pass_n[k] / (pass_ms[k] * 1e-3) / 1e9, 1)
            for k in bytes_per if k in pass_ms}
@@ -575,6 +577,10 @@ def main():
 
     pre_ms, qry_ms, step_ms, lookup_ms = [], [], [], [[], [], []]
     sd_ms, sd_flops_total, pass_ms = 0.0, 0.0, {}
+    sd_launches = 0
+    # One launch of a pass covers every Precompute iteration (each of N_S
+    # texels).
+    iters = model.params.num_iterations()
     atmo.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
@@ -594,7 +600,8 @@ def main():
             pass_ms[name] = pass_ms.get(name, 0.0) + ms
             if name == "scattering_density":
                 sd_ms += ms
-                sd_flops_total += sd_flops(n_s, order)
+                sd_flops_total += iters * sd_flops(n_s, order)
+                sd_launches += 1
     atmo.synchronize()
     elapsed = time.perf_counter() - t_start
 
@@ -638,8 +645,6 @@ def main():
             d.free()
 
     steps = args.steps
-    iters = model.params.num_iterations()
-    sd_launches = steps * iters * (orders - 1)
     sd_avg_ms = sd_ms / sd_launches
     achieved_tflops = sd_flops_total / (sd_ms * 1e-3) / 1e12
     ms_per_atmo = sum(pre_ms) / steps

@@ -83,7 +83,6 @@ struct Job {
   // 735-742), used by the luminance lookups.
   float sky_k[3];
   float sun_k[3];
-  int blend;     // add into S, S_Mie, E (illuminance iterations after the first)
   int combined;  // COMBINED_SCATTERING_TEXTURES: no separate S_Mie output
   int orders;    // number of scattering orders of this job
   // Tile shapes (indices into kSdTile and kMsTpt of kernels.h): host-side
@@ -111,9 +110,14 @@ struct Job {
   // rounded to FP32 once at the end. Luminance channels can be small
   // differences of the iterations' contributions (the XYZ -> sRGB matrix has
   // negative entries), and FP32 partial sums lose them.
+  // Each iteration job writes its own S64, SM64 and (through E) irradiance
+  // accumulator; the final job sums the niter iterations' accumulators,
+  // which start at S64, SM64 and E_it, into S, SM and E.
   int acc64;
+  int niter;
   double* S64;
   double* SM64;
+  float4* E_it;
   int half_luts;
   uint2* S16;
   uint2* SM16;

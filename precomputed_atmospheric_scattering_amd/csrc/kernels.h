@@ -78,6 +78,7 @@ int blocks_scattering_density(const Job& j);
 int blocks_indirect_irradiance(const Job& j);
 int blocks_multiple_scattering(const Job& j);
 int blocks_half_conversion(const Job& j);
+int blocks_combine(const Job& j);
 
 hipError_t launch_transmittance(const PassLaunch& p, hipStream_t s);
 hipError_t launch_direct_irradiance(const PassLaunch& p, hipStream_t s);
@@ -94,8 +95,9 @@ hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s);
 // launch_to_half (blocks_half_conversion blocks per job) computes the maxima
 // and writes the scaled FP16 texels.
 hipError_t launch_clear_max(const PassLaunch& p, hipStream_t s);
-// Rounding of the FP64 accumulators of illuminance-mode jobs into S and SM
-// (blocks_half_conversion blocks per job).
+// Illuminance mode: sums the Precompute iterations' FP64 accumulators into S
+// and SM, and their irradiance accumulators into E (blocks_combine blocks
+// per job; the job's niter, S64, SM64 and E_it name the accumulators).
 hipError_t launch_acc64_to_f32(const PassLaunch& p, hipStream_t s);
 hipError_t launch_to_half(const PassLaunch& p, hipStream_t s);
 

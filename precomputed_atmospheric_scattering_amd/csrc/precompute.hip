@@ -144,7 +144,8 @@ __global__ __launch_bounds__(kBlock) void k_transmittance(
 
 // ---------------------------------------------------------------------------
 // Direct irradiance (functions.glsl:1443-1461, :1558-1567) into delta E; the
-// irradiance LUT is cleared unless blending (atmosphere/model.cc:132-140).
+// job's irradiance LUT (or its iteration's accumulator) is cleared, the
+// direct term being excluded from it (atmosphere/model.cc:132-140).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_direct_irradiance(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_irradiance(
          make_float4(static_cast<float>(J.solar[0] * avg_cos) * t.x,
                      static_cast<float>(J.solar[1] * avg_cos) * t.y,
                      static_cast<float>(J.solar[2] * avg_cos) * t.z, 0.0f));
-  if (!J.blend) gstore(J.E, texel, make_float4(0.f, 0.f, 0.f, 0.f));
+  gstore(J.E, texel, make_float4(0.f, 0.f, 0.f, 0.f));
 }
 
 // ---------------------------------------------------------------------------
@@ -359,14 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
                                   static_cast<float>(mie[2]), 0.0f));
   if (kF64) {
     const D4 lr = apply_lum_d(J.lum, ray), lm = apply_lum_d(J.lum, mie);
-    D4 sv{lr.x, lr.y, lr.z, lm.x};
-    if (J.blend) sv = add_d4(sv, load_d4(J.S64, t_idx));
-    store_d4(J.S64, t_idx, sv);
-    if (!J.combined) {
-      D4 m{lm.x, lm.y, lm.z, 0.0};
-      if (J.blend) m = add_d4(m, load_d4(J.SM64, t_idx));
-      store_d4(J.SM64, t_idx, m);
-    }
+    store_d4(J.S64, t_idx, D4{lr.x, lr.y, lr.z, lm.x});
+    if (!J.combined) store_d4(J.SM64, t_idx, D4{lm.x, lm.y, lm.z, 0.0});
     return;
   }
   const float3 rayf = make_float3(static_cast<float>(ray[0]), static_cast<float>(ray[1]),
@@ -375,25 +370,8 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
                                   static_cast<float>(mie[2]));
   const float3 lr = apply_lum(J.lum, rayf);
   const float3 lm = apply_lum(J.lum, mief);
-  float4 sv = make_float4(lr.x, lr.y, lr.z, lm.x);
-  if (J.blend) {
-    const float4 o = gload(J.S, t_idx);
-    sv.x += o.x;
-    sv.y += o.y;
-    sv.z += o.z;
-    sv.w += o.w;
-  }
-  gstore(J.S, t_idx, sv);
-  if (!J.combined) {
-    float4 m = make_float4(lm.x, lm.y, lm.z, 0.0f);
-    if (J.blend) {
-      const float4 o = gload(J.SM, t_idx);
-      m.x += o.x;
-      m.y += o.y;
-      m.z += o.z;
-    }
-    gstore(J.SM, t_idx, m);
-  }
+  gstore(J.S, t_idx, make_float4(lr.x, lr.y, lr.z, lm.x));
+  if (!J.combined) gstore(J.SM, t_idx, make_float4(lm.x, lm.y, lm.z, 0.0f));
 }
 
 // ---------------------------------------------------------------------------
@@ -1204,8 +1182,9 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
 }
 
 // ---------------------------------------------------------------------------
-// Illuminance mode: round the FP64 accumulators into S and SM (FP32; in
-// FP16 mode these are the accumulators that k_to_half then converts).
+// Illuminance mode: sum the iterations' FP64 accumulators into S and SM
+// (FP32; in FP16 mode these are the accumulators that k_to_half then
+// converts) and their irradiance accumulators into E.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_acc64_to_f32(const Job* __restrict__ jobs,
                                                          const int* __restrict__ first_block,
@@ -1213,15 +1192,31 @@ __global__ __launch_bounds__(kBlock) void k_acc64_to_f32(const Job* __restrict__
   const int ji = find_job(first_block, njobs, blockIdx.x);
   const Job& J = jobs[ji];
   const int64_t n = static_cast<int64_t>(J.nr) * J.nmu * J.nmus * J.nnu;
+  const int64_t ne = static_cast<int64_t>(J.ew) * J.eh;
   const int64_t t = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
-  if (t >= n) return;
-  const D4 s = load_d4(J.S64, t);
-  gstore(J.S, t, make_float4(static_cast<float>(s.x), static_cast<float>(s.y),
-                             static_cast<float>(s.z), static_cast<float>(s.w)));
-  if (!J.combined) {
-    const D4 m = load_d4(J.SM64, t);
-    gstore(J.SM, t, make_float4(static_cast<float>(m.x), static_cast<float>(m.y),
-                                static_cast<float>(m.z), 0.0f));
+  // The iterations' accumulators are summed in iteration order, in FP64.
+  if (t < n) {
+    D4 s{0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < J.niter; ++k) s = add_d4(s, load_d4(J.S64, k * n + t));
+    gstore(J.S, t, make_float4(static_cast<float>(s.x), static_cast<float>(s.y),
+                               static_cast<float>(s.z), static_cast<float>(s.w)));
+    if (!J.combined) {
+      D4 m{0.0, 0.0, 0.0, 0.0};
+      for (int k = 0; k < J.niter; ++k) m = add_d4(m, load_d4(J.SM64, k * n + t));
+      gstore(J.SM, t, make_float4(static_cast<float>(m.x), static_cast<float>(m.y),
+                                  static_cast<float>(m.z), 0.0f));
+    }
+  }
+  if (t < ne) {
+    double e[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < J.niter; ++k) {
+      const float4 v = gload(J.E_it, k * ne + t);
+      e[0] += v.x;
+      e[1] += v.y;
+      e[2] += v.z;
+    }
+    gstore(J.E, t, make_float4(static_cast<float>(e[0]), static_cast<float>(e[1]),
+                               static_cast<float>(e[2]), 0.0f));
   }
 }
 
@@ -1325,6 +1320,11 @@ int blocks_multiple_scattering(const Job& j) {
   if (rpb == 0) return ((w + kBlock - 1) / kBlock) * rows;
   if (rpb < 0) return 0;  // not a tile of this row length (rejected by the host)
   return (rows + rpb - 1) / rpb;
+}
+int blocks_combine(const Job& j) {
+  const int64_t n = static_cast<int64_t>(j.nr) * j.nmu * j.nmus * j.nnu;
+  const int64_t ne = static_cast<int64_t>(j.ew) * j.eh;
+  return static_cast<int>(((n > ne ? n : ne) + kBlock - 1) / kBlock);
 }
 int blocks_half_conversion(const Job& j) {
   const int64_t n = static_cast<int64_t>(j.nr) * j.nmu * j.nmus * j.nnu;

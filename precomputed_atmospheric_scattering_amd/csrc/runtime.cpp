@@ -184,7 +184,7 @@ int pass_blocks(Pass p, const atmo::Job& j) {
     case kII: return atmo::blocks_indirect_irradiance(j);
     case kMS: return atmo::blocks_multiple_scattering(j);
     case kHalf: return atmo::blocks_half_conversion(j);
-    case kAcc: return atmo::blocks_half_conversion(j);
+    case kAcc: return atmo::blocks_combine(j);
     case kClear: return 1;
     default: return 0;
   }
@@ -196,114 +196,135 @@ using namespace atmo_rt;
 
 namespace atmo_rt {
 
+namespace {
+
+void clear_temporary_pointers(atmo_model* m) {
+  m->T_it = m->E_it = nullptr;
+  m->dE_it = m->dR_it = m->dM_it = m->dJ_it = nullptr;
+  m->dE = m->dR = m->dM = m->dJ = nullptr;
+  if (m->spec.half_precision) m->S = m->SM = nullptr;
+  m->S64 = m->SM64 = nullptr;
+}
+
+size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+// The layout of the temporaries block; with 'base' null, only its size.
+// Illuminance mode (iters > 1): per iteration a transmittance LUT, an
+// irradiance accumulator, the four delta LUTs and the FP64 accumulators of S
+// and SM, each array of 'iters' copies contiguous. FP16 storage adds the
+// FP32 accumulators of S and SM.
+size_t carve(const Sizes& z, bool half, bool combined, int iters, char* base,
+             atmo_model* m) {
+  const bool par = iters > 1;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base == nullptr ? nullptr : base + off;
+    off += round256(bytes);
+    return p;
+  };
+  const size_t P = static_cast<size_t>(iters);
+  char* t_it = par ? take(P * z.t_texels() * sizeof(float4)) : nullptr;
+  char* e_it = par ? take(P * z.e_texels() * sizeof(float4)) : nullptr;
+  char* de = take(P * z.e_texels() * sizeof(float4));
+  char* dr = take(P * z.s_texels() * sizeof(float4));
+  char* dm = take(P * z.s_texels() * sizeof(float4));
+  char* dj = take(P * z.s_texels() * sizeof(float4));
+  char* s32 = half ? take(z.s_texels() * sizeof(float4)) : nullptr;
+  char* sm32 = half && !combined ? take(z.s_texels() * sizeof(float4)) : nullptr;
+  char* s64 = par ? take(P * z.s_texels() * 4 * sizeof(double)) : nullptr;
+  char* sm64 = par && !combined ? take(P * z.s_texels() * 4 * sizeof(double)) : nullptr;
+  if (m != nullptr) {
+    m->T_it = reinterpret_cast<float4*>(t_it);
+    m->E_it = reinterpret_cast<float4*>(e_it);
+    m->dE_it = reinterpret_cast<float4*>(de);
+    m->dR_it = reinterpret_cast<float4*>(dr);
+    m->dM_it = reinterpret_cast<float4*>(dm);
+    m->dJ_it = reinterpret_cast<float4*>(dj);
+    // The retained intermediates: the last iteration's delta LUTs.
+    m->dE = m->dE_it + (P - 1) * z.e_texels();
+    m->dR = m->dR_it + (P - 1) * z.s_texels();
+    m->dM = m->dM_it + (P - 1) * z.s_texels();
+    m->dJ = m->dJ_it + (P - 1) * z.s_texels();
+    if (half) {
+      m->S = reinterpret_cast<float4*>(s32);
+      m->SM = reinterpret_cast<float4*>(sm32);
+    }
+    m->S64 = reinterpret_cast<double*>(s64);
+    m->SM64 = reinterpret_cast<double*>(sm64);
+  }
+  return off;
+}
+
+}  // namespace
+
 // Temporaries are released once the work that reads them has finished.
 void free_temporaries(atmo_model* m) {
   if (m->external_temporaries) {
     drop_external_temporaries(m);
     return;
   }
-  std::vector<float4**> bufs = {&m->dE, &m->dR, &m->dM, &m->dJ};
-  if (m->spec.half_precision) {
-    bufs.push_back(&m->S);
-    bufs.push_back(&m->SM);
+  if (m->temp_block != nullptr) {
+    (void)hipFree(m->temp_block);
+    m->temp_block = nullptr;
   }
-  for (float4** b : bufs) {
-    if (*b != nullptr) {
-      (void)hipFree(*b);
-      *b = nullptr;
-    }
-  }
-  for (double** b : {&m->S64, &m->SM64}) {
-    if (*b != nullptr) {
-      (void)hipFree(*b);
-      *b = nullptr;
-    }
-  }
+  clear_temporary_pointers(m);
 }
 
 atmo_status alloc_temporaries(atmo_model* m) {
-  const Sizes& z = m->sizes;
-  if (m->dE == nullptr) ATMO_HIP(hipMalloc(&m->dE, z.e_texels() * sizeof(float4)));
-  if (m->dR == nullptr) ATMO_HIP(hipMalloc(&m->dR, z.s_texels() * sizeof(float4)));
-  if (m->dM == nullptr) ATMO_HIP(hipMalloc(&m->dM, z.s_texels() * sizeof(float4)));
-  if (m->dJ == nullptr) ATMO_HIP(hipMalloc(&m->dJ, z.s_texels() * sizeof(float4)));
-  if (m->spec.half_precision) {
-    if (m->S == nullptr) ATMO_HIP(hipMalloc(&m->S, z.s_texels() * sizeof(float4)));
-    if (m->SM == nullptr && !m->spec.combined)
-      ATMO_HIP(hipMalloc(&m->SM, z.s_texels() * sizeof(float4)));
-  }
-  if (accumulates_fp64(m->spec)) {
-    if (m->S64 == nullptr) ATMO_HIP(hipMalloc(&m->S64, z.s_texels() * 4 * sizeof(double)));
-    if (m->SM64 == nullptr && !m->spec.combined)
-      ATMO_HIP(hipMalloc(&m->SM64, z.s_texels() * 4 * sizeof(double)));
-  }
+  if (m->temp_block != nullptr || m->external_temporaries) return ATMO_OK;
+  ATMO_HIP(hipMalloc(&m->temp_block, temporaries_bytes(m)));
+  carve(m->sizes, m->spec.half_precision, m->spec.combined, m->iters, m->temp_block, m);
   return ATMO_OK;
 }
 
-// Sizes of the temporaries, each rounded up to 256 bytes.
-static size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+size_t temporaries_bytes_for(const Sizes& z, bool half, bool combined, int iters) {
+  return carve(z, half, combined, iters, nullptr, nullptr);
+}
 
 size_t temporaries_bytes(const atmo_model* m) {
-  const Sizes& z = m->sizes;
-  size_t b = round256(z.e_texels() * sizeof(float4)) +
-             3 * round256(z.s_texels() * sizeof(float4));
-  if (m->spec.half_precision)
-    b += (m->spec.combined ? 1 : 2) * round256(z.s_texels() * sizeof(float4));
-  if (accumulates_fp64(m->spec))
-    b += (m->spec.combined ? 1 : 2) * round256(z.s_texels() * 4 * sizeof(double));
-  return b;
+  return temporaries_bytes_for(m->sizes, m->spec.half_precision, m->spec.combined,
+                               m->iters);
 }
 
 void use_external_temporaries(atmo_model* m, char* base) {
-  const Sizes& z = m->sizes;
-  const size_t es = round256(z.e_texels() * sizeof(float4));
-  const size_t ss = round256(z.s_texels() * sizeof(float4));
-  m->dE = reinterpret_cast<float4*>(base);
-  m->dR = reinterpret_cast<float4*>(base + es);
-  m->dM = reinterpret_cast<float4*>(base + es + ss);
-  m->dJ = reinterpret_cast<float4*>(base + es + 2 * ss);
-  size_t off = es + 3 * ss;
-  if (m->spec.half_precision) {
-    m->S = reinterpret_cast<float4*>(base + off);
-    off += ss;
-    if (!m->spec.combined) {
-      m->SM = reinterpret_cast<float4*>(base + off);
-      off += ss;
-    }
-  }
-  if (accumulates_fp64(m->spec)) {
-    const size_t ds = round256(z.s_texels() * 4 * sizeof(double));
-    m->S64 = reinterpret_cast<double*>(base + off);
-    off += ds;
-    if (!m->spec.combined) m->SM64 = reinterpret_cast<double*>(base + off);
-  }
+  carve(m->sizes, m->spec.half_precision, m->spec.combined, m->iters, base, m);
   m->external_temporaries = true;
 }
 
 void drop_external_temporaries(atmo_model* m) {
-  m->dE = m->dR = m->dM = m->dJ = nullptr;
-  if (m->spec.half_precision) m->S = m->SM = nullptr;
-  m->S64 = m->SM64 = nullptr;
+  clear_temporary_pointers(m);
   m->external_temporaries = false;
 }
 
-void fill_job_luts(atmo_model* m, atmo::Job* j) {
-  j->T = m->T;
+// Job k < iters is Precompute iteration k; k = -1 is a job on the resident
+// LUTs (the final transmittance and accumulator sums, the lookups). In
+// illuminance mode every iteration has its own transmittance, delta LUTs and
+// accumulators.
+void fill_job_luts(atmo_model* m, atmo::Job* j, int k) {
+  const Sizes& z = m->sizes;
+  // (Before the temporaries exist the job table is built for the lookups
+  // only; iteration jobs then point at the resident LUTs.)
+  const bool par = m->iters > 1 && k >= 0 && m->T_it != nullptr;
+  const size_t kk = k >= 0 ? static_cast<size_t>(k) : static_cast<size_t>(m->iters - 1);
+  const bool temps = m->dR_it != nullptr;
+  j->T = par ? m->T_it + kk * z.t_texels() : m->T;
   j->S = m->S;
   j->SM = m->SM;
-  j->E = m->E;
-  j->dE = m->dE;
-  j->dR = m->dR;
-  j->dM = m->dM;
-  j->dJ = m->dJ;
+  j->E = par ? m->E_it + kk * z.e_texels() : m->E;
+  j->E_it = m->E_it;
+  j->dE = temps ? m->dE_it + kk * z.e_texels() : nullptr;
+  j->dR = temps ? m->dR_it + kk * z.s_texels() : nullptr;
+  j->dM = temps ? m->dM_it + kk * z.s_texels() : nullptr;
+  j->dJ = temps ? m->dJ_it + kk * z.s_texels() : nullptr;
+  j->niter = m->iters;
   set_sizes(j, m->sizes);
   j->combined = m->spec.combined ? 1 : 0;
   j->half_luts = m->spec.half_precision ? 1 : 0;
   j->S16 = m->S16;
   j->SM16 = m->SM16;
   j->acc64 = accumulates_fp64(m->spec) ? 1 : 0;
-  j->S64 = m->S64;
-  j->SM64 = m->SM64;
+  j->S64 = par ? m->S64 + 4 * kk * z.s_texels() : m->S64;
+  j->SM64 = par && m->SM64 != nullptr ? m->SM64 + 4 * kk * z.s_texels() : m->SM64;
   j->scales = m->d_scales;
   j->maxbits = m->d_maxbits;
   for (int c = 0; c < 3; ++c) {
@@ -318,15 +339,14 @@ void fill_job_luts(atmo_model* m, atmo::Job* j) {
 // Rebuild the host job table (LUT pointers may have changed) and upload it.
 atmo_status upload_jobs(atmo_model* m, unsigned orders, hipStream_t stream) {
   const Spec& s = m->spec;
-  const int iters = num_iterations(s);
+  const int iters = m->iters;
   m->jobs.assign(iters + 2, atmo::Job{});
   for (int i = 0; i < iters; ++i) {
     double lambdas[3];
     atmo::Job& j = m->jobs[i];
     iteration(s, i, lambdas, j.lum);
     lower(s, lambdas, &j);
-    fill_job_luts(m, &j);
-    j.blend = i > 0 ? 1 : 0;
+    fill_job_luts(m, &j, i);
     j.orders = static_cast<int>(orders);
   }
   // Transmittance at 680/550/440 nm (atmosphere/model.cc:951-963) and the
@@ -335,35 +355,40 @@ atmo_status upload_jobs(atmo_model* m, unsigned orders, hipStream_t stream) {
   for (int k = iters; k < iters + 2; ++k) {
     atmo::Job& j = m->jobs[k];
     lower(s, rgb, &j);
-    fill_job_luts(m, &j);
+    fill_job_luts(m, &j, -1);
     for (int q = 0; q < 9; ++q) j.lum[q] = (q % 4 == 0) ? 1.0f : 0.0f;
   }
   ATMO_HIP(hipMemcpyAsync(m->d_jobs, m->jobs.data(),
                           m->jobs.size() * sizeof(atmo::Job),
                           hipMemcpyHostToDevice, stream));
-  // Block counts of the single-model launches (they follow the tiles).
+  // Block prefixes of the model's launches (they follow the tiles): every
+  // job has the same sizes, so pass p's prefix over the iterations is
+  // {0, b, 2b, ...} with b = pass_blocks(p, job).
+  m->first_block_host.assign(kPasses * (iters + 1), 0);
   for (int p = 0; p < kPasses; ++p) {
-    m->first_block_host[2 * p] = 0;
-    m->first_block_host[2 * p + 1] = pass_blocks(static_cast<Pass>(p), m->jobs[0]);
+    const int b = pass_blocks(static_cast<Pass>(p), m->jobs[0]);
+    for (int i = 0; i <= iters; ++i) m->first_block_host[p * (iters + 1) + i] = i * b;
   }
-  ATMO_HIP(hipMemcpyAsync(m->d_first_block, m->first_block_host,
-                          sizeof(m->first_block_host), hipMemcpyHostToDevice, stream));
+  ATMO_HIP(hipMemcpyAsync(m->d_first_block, m->first_block_host.data(),
+                          m->first_block_host.size() * sizeof(int), hipMemcpyHostToDevice,
+                          stream));
   return ATMO_OK;
 }
 
-atmo::PassLaunch single_launch(const atmo_model* m, int job_index, Pass p) {
+// One launch of pass p over the model's jobs first_job .. first_job + njobs - 1.
+atmo::PassLaunch model_launch(const atmo_model* m, int first_job, int njobs, Pass p) {
   atmo::PassLaunch l;
-  l.jobs = m->d_jobs + job_index;
-  l.first_block = m->d_first_block + 2 * p;
-  l.njobs = 1;
-  l.total_blocks = m->first_block_host[2 * p + 1];
+  l.jobs = m->d_jobs + first_job;
+  l.first_block = m->d_first_block + p * (m->iters + 1);
+  l.njobs = njobs;
+  l.total_blocks = m->first_block_host[p * (m->iters + 1) + njobs];
   l.max_nnu = m->sizes.nnu;
   l.max_ew = m->sizes.ew;
   l.max_w = m->sizes.nnu * m->sizes.nmus;
   l.min_w = l.max_w;
   l.max_lds_w = l.max_w <= atmo::kMsMaxRow ? l.max_w : 0;
   l.kinds = accumulates_fp64(m->spec) ? atmo::kJobsAcc64 : atmo::kJobsF32;
-  const atmo::Job& j = m->jobs[job_index];
+  const atmo::Job& j = m->jobs[first_job];
   l.tile = p == kSD ? j.sd_tile : (p == kMS ? j.ms_tile : 0);
   return l;
 }
@@ -408,29 +433,31 @@ struct LaunchTimer {
   }
 };
 
-// Model::Precompute (atmosphere/model.cc:1048-1215) for job 'ji'.
-atmo_status enqueue_precompute(atmo_model* m, int ji, unsigned orders,
-                               LaunchTimer& t) {
+// Model::Precompute (atmosphere/model.cc:1048-1215) for every iteration at
+// once: each launch covers the jobs of all iterations, which write their
+// own temporaries and accumulators.
+atmo_status enqueue_precompute(atmo_model* m, unsigned orders, LaunchTimer& t) {
   hipStream_t s = t.s;
+  const int n = m->iters;
   ATMO_TRY(t.run(kT, 0, "transmittance", [&] {
-    return atmo::launch_transmittance(single_launch(m, ji, kT), s);
+    return atmo::launch_transmittance(model_launch(m, 0, n, kT), s);
   }));
   ATMO_TRY(t.run(kE0, 0, "direct irradiance", [&] {
-    return atmo::launch_direct_irradiance(single_launch(m, ji, kE0), s);
+    return atmo::launch_direct_irradiance(model_launch(m, 0, n, kE0), s);
   }));
   ATMO_TRY(t.run(kSS, 1, "single scattering", [&] {
-    return atmo::launch_single_scattering(single_launch(m, ji, kSS), s);
+    return atmo::launch_single_scattering(model_launch(m, 0, n, kSS), s);
   }));
   for (unsigned order = 2; order <= orders; ++order) {
     const int o = static_cast<int>(order);
     ATMO_TRY(t.run(kSD, o, "scattering density", [&] {
-      return atmo::launch_scattering_density(single_launch(m, ji, kSD), o, s);
+      return atmo::launch_scattering_density(model_launch(m, 0, n, kSD), o, s);
     }));
     ATMO_TRY(t.run(kII, o - 1, "indirect irradiance", [&] {
-      return atmo::launch_indirect_irradiance(single_launch(m, ji, kII), o - 1, s);
+      return atmo::launch_indirect_irradiance(model_launch(m, 0, n, kII), o - 1, s);
     }));
     ATMO_TRY(t.run(kMS, o, "multiple scattering", [&] {
-      return atmo::launch_multiple_scattering(single_launch(m, ji, kMS), s);
+      return atmo::launch_multiple_scattering(model_launch(m, 0, n, kMS), s);
     }));
   }
   return ATMO_OK;
@@ -611,8 +638,9 @@ atmo_status atmo_model_create(const atmo_model_params* p,
   }
   ATMO_HIP(hipMalloc(&m->E, sz.e_texels() * sizeof(float4)));
   const int iters = num_iterations(s);
+  m->iters = iters;
   ATMO_HIP(hipMalloc(&m->d_jobs, (iters + 2) * sizeof(atmo::Job)));
-  ATMO_HIP(hipMalloc(&m->d_first_block, 2 * kPasses * sizeof(int)));
+  ATMO_HIP(hipMalloc(&m->d_first_block, kPasses * (iters + 1) * sizeof(int)));
   ATMO_TRY(upload_jobs(m.get(), 4, nullptr));
   ATMO_HIP(hipStreamSynchronize(nullptr));
   *out = m.release();
@@ -651,23 +679,25 @@ atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   ATMO_TRY(alloc_temporaries(m));
   ATMO_TRY(upload_jobs(m, orders, s));
-  const int iters = num_iterations(m->spec);
+  const int iters = m->iters;
   m->timed.clear();
   LaunchTimer timer{m, s};
-  for (int i = 0; i < iters; ++i) ATMO_TRY(enqueue_precompute(m, i, orders, timer));
+  ATMO_TRY(enqueue_precompute(m, orders, timer));
+  // Job 'iters' works on the resident LUTs: the sum of the iterations'
+  // accumulators, then the transmittance at 680/550/440 nm.
   if (accumulates_fp64(m->spec)) {
-    ATMO_TRY(check_launch(atmo::launch_acc64_to_f32(single_launch(m, iters - 1, kAcc), s),
+    ATMO_TRY(check_launch(atmo::launch_acc64_to_f32(model_launch(m, iters, 1, kAcc), s),
                           "fp64 accumulators"));
   }
   if (m->spec.num_precomputed_wavelengths > 3) {
     ATMO_TRY(timer.run(kT, 0, "transmittance (rgb)", [&] {
-      return atmo::launch_transmittance(single_launch(m, iters, kT), s);
+      return atmo::launch_transmittance(model_launch(m, iters, 1, kT), s);
     }));
   }
   if (m->spec.half_precision) {
-    ATMO_TRY(check_launch(atmo::launch_clear_max(single_launch(m, iters - 1, kClear), s),
+    ATMO_TRY(check_launch(atmo::launch_clear_max(model_launch(m, iters, 1, kClear), s),
                           "fp16 maxima"));
-    ATMO_TRY(check_launch(atmo::launch_to_half(single_launch(m, iters - 1, kHalf), s),
+    ATMO_TRY(check_launch(atmo::launch_to_half(model_launch(m, iters, 1, kHalf), s),
                           "fp16 conversion"));
   }
   m->initialized = true;
@@ -1149,36 +1179,32 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
       add_one(p, order, t, part);
     }
   };
-  int max_iters = 0;
+  // Every Precompute iteration of every model is a job of each level: the
+  // iterations of one model have their own temporaries and accumulators.
   unsigned max_orders = 0;
+  std::vector<std::pair<int, int>> sel;
   for (int i = 0; i < count; ++i) {
-    max_iters = std::max(max_iters, num_iterations(models[i]->spec));
     max_orders = std::max(max_orders, orders[i]);
+    for (int it = 0; it < models[i]->iters; ++it) sel.push_back({i, it});
   }
-  for (int it = 0; it < max_iters; ++it) {
-    std::vector<std::pair<int, int>> sel;
-    for (int i = 0; i < count; ++i)
-      if (it < num_iterations(models[i]->spec)) sel.push_back({i, it});
-    add_level(kT, 0, sel);
-    add_level(kE0, 0, sel);
-    add_level(kSS, 0, sel);
-    for (unsigned n = 2; n <= max_orders; ++n) {
-      std::vector<std::pair<int, int>> on;
-      for (const auto& mj : sel)
-        if (orders[mj.first] >= n) on.push_back(mj);
-      add_level(kSD, static_cast<int>(n), on);
-      add_level(kII, static_cast<int>(n) - 1, on);
-      add_level(kMS, 0, on);
-    }
+  add_level(kT, 0, sel);
+  add_level(kE0, 0, sel);
+  add_level(kSS, 0, sel);
+  for (unsigned n = 2; n <= max_orders; ++n) {
+    std::vector<std::pair<int, int>> on;
+    for (const auto& mj : sel)
+      if (orders[mj.first] >= n) on.push_back(mj);
+    add_level(kSD, static_cast<int>(n), on);
+    add_level(kII, static_cast<int>(n) - 1, on);
+    add_level(kMS, 0, on);
   }
+  // Job 'iters' of a model works on its resident LUTs.
   std::vector<std::pair<int, int>> rgb, half, acc;
   for (int i = 0; i < count; ++i) {
-    if (models[i]->spec.num_precomputed_wavelengths > 3)
-      rgb.push_back({i, num_iterations(models[i]->spec)});
-    if (models[i]->spec.half_precision)
-      half.push_back({i, num_iterations(models[i]->spec) - 1});
-    if (accumulates_fp64(models[i]->spec))
-      acc.push_back({i, num_iterations(models[i]->spec) - 1});
+    const int last = models[i]->iters;
+    if (models[i]->spec.num_precomputed_wavelengths > 3) rgb.push_back({i, last});
+    if (models[i]->spec.half_precision) half.push_back({i, last});
+    if (accumulates_fp64(models[i]->spec)) acc.push_back({i, last});
   }
   add_level(kAcc, 0, acc);
   add_level(kT, 0, rgb);

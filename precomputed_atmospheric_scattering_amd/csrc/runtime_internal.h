@@ -72,20 +72,33 @@ struct atmo_model {
   bool initialized = false;
   bool external_temporaries = false;  // temporaries carved from caller memory
   float4 *T = nullptr, *S = nullptr, *SM = nullptr, *E = nullptr;
+  // Init temporaries, carved from one block (owned, or caller memory with
+  // external_temporaries). In illuminance mode each Precompute iteration has
+  // its own copy, so that the iterations run as parallel jobs: iteration k's
+  // delta LUTs are at dR_it + k * (scattering texels) and so on, its
+  // transmittance and irradiance accumulators at T_it and E_it. dE, dR, dM,
+  // dJ point at the last iteration's copies (the retained intermediates).
+  char* temp_block = nullptr;
+  float4 *T_it = nullptr, *E_it = nullptr;
+  float4 *dE_it = nullptr, *dR_it = nullptr, *dM_it = nullptr, *dJ_it = nullptr;
   float4 *dE = nullptr, *dR = nullptr, *dM = nullptr, *dJ = nullptr;
   // FP16 storage: resident half LUTs; S and SM above are then FP32
   // accumulators that live only during Init, like the deltas.
   uint2 *S16 = nullptr, *SM16 = nullptr;
-  // FP64 accumulators of S and SM in illuminance mode (Init temporaries).
+  // FP64 accumulators of S and SM in illuminance mode (Init temporaries),
+  // one per iteration: iteration k's at S64 + 4 * k * (scattering texels).
   double *S64 = nullptr, *SM64 = nullptr;
   float* d_scales = nullptr;
   unsigned* d_maxbits = nullptr;
   // Device job table: [iterations..., rgb transmittance job, query job].
   std::vector<atmo::Job> jobs;
   atmo::Job* d_jobs = nullptr;
-  // Per-pass block ranges of a single job: {0, blocks(pass)} for each pass.
+  // Per-pass block prefixes of the Precompute iterations' jobs (all of one
+  // size): pass p's are entries p * (iters + 1) .. p * (iters + 1) + iters,
+  // {0, b, 2b, ...}; a launch of the first n jobs uses the first n + 1.
+  int iters = 1;
   int* d_first_block = nullptr;
-  int first_block_host[2 * kPasses] = {};
+  std::vector<int> first_block_host;
   float solar_radiance[3] = {};
   double sky_k[3] = {}, sun_k[3] = {};
   // Launch timing (ATMO_OPTION_PASS_TIMING).
@@ -124,9 +137,11 @@ Tiles tiles_for(const Sizes& z);
 // Rebuild a model's job table for 'orders' and copy it to the device on
 // 'stream' (asynchronous with respect to other streams).
 atmo_status upload_jobs(atmo_model* m, unsigned orders, hipStream_t stream);
-// Bytes of Init temporaries of a model, and carving them out of caller-owned
-// memory (the model then does not free them).
+// Bytes of Init temporaries of a model (or of a model of these sizes and
+// storage with 'iters' Precompute iterations), and carving them out of
+// caller-owned memory (the model then does not free them).
 size_t temporaries_bytes(const atmo_model* m);
+size_t temporaries_bytes_for(const Sizes& z, bool half, bool combined, int iters);
 void use_external_temporaries(atmo_model* m, char* base);
 void drop_external_temporaries(atmo_model* m);
 // Plan the levels of the models' Init (their job tables must be current).

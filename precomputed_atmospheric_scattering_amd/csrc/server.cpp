@@ -117,9 +117,8 @@ struct InflightLookup {
 size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
 // Resident and scratch bytes of a model with these sizes and storage (the
-// same layout as atmo_model_footprint); acc64: illuminance mode, whose Init
-// accumulates S and SM in FP64.
-void footprint(const atmo_lut_sizes& z, bool half, bool combined, bool acc64,
+// same layout as atmo_model_footprint); iters: its Precompute iterations.
+void footprint(const atmo_lut_sizes& z, bool half, bool combined, int iters,
                size_t* resident, size_t* scratch) {
   const size_t nt = static_cast<size_t>(z.transmittance_width) * z.transmittance_height;
   const size_t ne = static_cast<size_t>(z.irradiance_width) * z.irradiance_height;
@@ -128,9 +127,10 @@ void footprint(const atmo_lut_sizes& z, bool half, bool combined, bool acc64,
   const size_t texel = half ? 8 : 16;
   *resident = round256(nt * 16) + round256(ne * 16) +
               (combined ? 1 : 2) * round256(ns * texel);
-  *scratch = round256(ne * 16) + 3 * round256(ns * 16) +
-             (half ? (combined ? 1 : 2) * round256(ns * 16) : 0) +
-             (acc64 ? (combined ? 1 : 2) * round256(ns * 32) : 0);
+  const Sizes sz{z.transmittance_width, z.transmittance_height, z.scattering_r,
+                 z.scattering_mu,       z.scattering_mu_s,      z.scattering_nu,
+                 z.irradiance_width,    z.irradiance_height};
+  *scratch = temporaries_bytes_for(sz, half, combined, iters);
 }
 
 }  // namespace
@@ -656,7 +656,7 @@ atmo_status atmo_server_submit_precompute(atmo_server* s, const atmo_model_param
   r.sizes = z;
   r.orders = orders;
   footprint(z, params->half_precision != 0, params->combine_scattering_textures != 0,
-            params->num_precomputed_wavelengths > 3, &r.resident, &r.scratch);
+            atmo_num_iterations(params), &r.resident, &r.scratch);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     r.id = s->next_id++;
