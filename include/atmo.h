@@ -124,6 +124,12 @@ typedef struct atmo_lut_view {
 typedef enum atmo_option {
   ATMO_OPTION_RETAIN_INTERMEDIATES = 0, /* keep the delta LUTs after Init */
   ATMO_OPTION_PASS_TIMING = 1,          /* time every kernel launch of Init */
+  /* Separate scattering textures only (default on; no counterpart in the
+   * reference): also keep S and S_Mie interleaved texel by texel for the
+   * lookups, which then fetch both with one access per tap. Costs the size
+   * of S and S_Mie again; the LUT views and files are unchanged. Turning it
+   * on for an initialized model builds the copy at once (synchronous). */
+  ATMO_OPTION_INTERLEAVED_LOOKUPS = 2,
 } atmo_option;
 
 /* Kernel passes of the precomputation (atmosphere/model.cc:1085-1211). */

@@ -226,7 +226,7 @@ class Model:
                  absorption_extinction, ground_albedo, max_sun_zenith_angle,
                  length_unit_in_meters, num_precomputed_wavelengths,
                  combine_scattering_textures, half_precision, lut_sizes=None,
-                 retain_intermediates=False, pass_timing=False):
+                 retain_intermediates=False, pass_timing=False, interleaved_lookups=True):
         self._handle = None
         self.params = ModelParameters(
             wavelengths, solar_irradiance, sun_angular_radius, bottom_radius, top_radius,
@@ -250,6 +250,14 @@ class Model:
         if pass_timing:
             check(lib().atmo_model_set_option(h, _capi.OPTION_PASS_TIMING, 1),
                   "set_option")
+        if not interleaved_lookups:
+            self.set_interleaved_lookups(False)
+
+    def set_interleaved_lookups(self, on):
+        """ATMO_OPTION_INTERLEAVED_LOOKUPS: keep S and S_Mie interleaved for
+        the lookups (separate textures only; synchronous)."""
+        check(lib().atmo_model_set_option(self._handle, _capi.OPTION_INTERLEAVED_LOOKUPS,
+                                          int(bool(on))), "set_option")
 
     def pass_timings(self):
         """[(pass name, order, ms)] of the launches of the last Init (needs

@@ -20,6 +20,7 @@ LUT_IDS = {
 }
 OPTION_RETAIN_INTERMEDIATES = 0
 OPTION_PASS_TIMING = 1
+OPTION_INTERLEAVED_LOOKUPS = 2
 PASS_NAMES = ("transmittance", "direct_irradiance", "single_scattering",
               "scattering_density", "indirect_irradiance", "multiple_scattering")
 

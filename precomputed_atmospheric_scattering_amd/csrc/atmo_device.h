@@ -121,6 +121,9 @@ struct Job {
   int half_luts;
   uint2* S16;
   uint2* SM16;
+  // S and SM interleaved for the lookups: texel t's S at Q[2 t] and SM at
+  // Q[2 t + 1] (float4), or both in the uint4 Q[t] (FP16); null if absent.
+  void* Q;
   float* scales;    // 8 floats, written by the conversion kernel
   unsigned* maxbits;  // 8 words: per-channel max |value| (float bits)
 };
@@ -429,14 +432,16 @@ struct F16Texels {
   }
 };
 
-template <class Texels>
-ATMO_HD D4 s_lookup_d(const Job& j, const Texels& lut, const SCoords& c) {
+// The 16 taps of GetScattering (functions.glsl:958-976) with the GL
+// texture() of each nu slice: f(texel index, FP64 weight) in a fixed order.
+template <class F>
+ATMO_HD void s_taps(const Job& j, const SCoords& c, F&& f) {
   double cnu = c.nu;
   if (!(cnu >= -1.0)) cnu = -1.0;
   if (cnu > static_cast<double>(j.nnu)) cnu = static_cast<double>(j.nnu);
-  const double f = floor(cnu);
-  const double ws = cnu - f;
-  int s0 = static_cast<int>(f);
+  const double fl = floor(cnu);
+  const double ws = cnu - fl;
+  int s0 = static_cast<int>(fl);
   int s1 = s0 + 1;
   s0 = s0 < 0 ? 0 : (s0 > j.nnu - 1 ? j.nnu - 1 : s0);
   s1 = s1 < 0 ? 0 : (s1 > j.nnu - 1 ? j.nnu - 1 : s1);
@@ -458,20 +463,64 @@ ATMO_HD D4 s_lookup_d(const Job& j, const Texels& lut, const SCoords& c) {
   const double wx[2] = {1.0 - fx, fx};
   const int ss[2] = {s0, s1};
   const double wsl[2] = {1.0 - ws, ws};
-  D4 acc{0.0, 0.0, 0.0, 0.0};
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int e = 0; e < 2; ++e)
-        for (int q = 0; q < 2; ++q) {
-          const double wt = wz[a] * wy[b] * wx[e] * wsl[q];
-          const float4 v = lut[(static_cast<int64_t>(zs[a]) * j.nmu + ys[b]) * w +
-                               ss[q] * j.nmus + xs[e]];
-          acc.x += wt * v.x;
-          acc.y += wt * v.y;
-          acc.z += wt * v.z;
-          acc.w += wt * v.w;
-        }
+        for (int q = 0; q < 2; ++q)
+          f((static_cast<int64_t>(zs[a]) * j.nmu + ys[b]) * w + ss[q] * j.nmus + xs[e],
+            wz[a] * wy[b] * wx[e] * wsl[q]);
+}
+
+ATMO_HD void acc_d4(D4* acc, double wt, float4 v) {
+  acc->x = fma(wt, static_cast<double>(v.x), acc->x);
+  acc->y = fma(wt, static_cast<double>(v.y), acc->y);
+  acc->z = fma(wt, static_cast<double>(v.z), acc->z);
+  acc->w = fma(wt, static_cast<double>(v.w), acc->w);
+}
+
+template <class Texels>
+ATMO_HD D4 s_lookup_d(const Job& j, const Texels& lut, const SCoords& c) {
+  D4 acc{0.0, 0.0, 0.0, 0.0};
+  s_taps(j, c, [&](int64_t i, double wt) { acc_d4(&acc, wt, lut[i]); });
   return acc;
+}
+
+// Interleaved S and SM texels (Job::Q): both texels of a tap in one place,
+// FP32 (float4 pairs) or FP16 (one uint4 of two 4 x FP16 texels, with the
+// per-channel inverse scales of S and SM).
+struct F32Pairs {
+  const float4* q;
+  ATMO_HD void load(int64_t i, float4* s, float4* m) const {
+    *s = gload(q, 2 * i);
+    *m = gload(q, 2 * i + 1);
+  }
+};
+struct F16Pairs {
+  const uint4* q;
+  float4 inv_s, inv_m;
+  ATMO_HD void load(int64_t i, float4* s, float4* m) const {
+    const uint4 u = gload(q, i);
+    const float4 a = unpack_half4(make_uint2(u.x, u.y));
+    const float4 b = unpack_half4(make_uint2(u.z, u.w));
+    *s = make_float4(a.x * inv_s.x, a.y * inv_s.y, a.z * inv_s.z, a.w * inv_s.w);
+    *m = make_float4(b.x * inv_m.x, b.y * inv_m.y, b.z * inv_m.z, b.w * inv_m.w);
+  }
+};
+
+// S and SM lookups at the same coordinates, one fetch per tap; the same
+// sums as two s_lookup_d calls.
+template <class Pairs>
+ATMO_HD void s_lookup_pair_d(const Job& j, const Pairs& lut, const SCoords& c, D4* s,
+                             D4* m) {
+  D4 as{0.0, 0.0, 0.0, 0.0}, am{0.0, 0.0, 0.0, 0.0};
+  s_taps(j, c, [&](int64_t i, double wt) {
+    float4 vs, vm;
+    lut.load(i, &vs, &vm);
+    acc_d4(&as, wt, vs);
+    acc_d4(&am, wt, vm);
+  });
+  *s = as;
+  *m = am;
 }
 
 ATMO_HD D4 bilinear_d(const float4* lut, int w, int h, double x, double y) {

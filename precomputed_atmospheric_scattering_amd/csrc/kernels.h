@@ -100,6 +100,9 @@ hipError_t launch_clear_max(const PassLaunch& p, hipStream_t s);
 // per job; the job's niter, S64, SM64 and E_it name the accumulators).
 hipError_t launch_acc64_to_f32(const PassLaunch& p, hipStream_t s);
 hipError_t launch_to_half(const PassLaunch& p, hipStream_t s);
+// The interleaved lookup copy of S and SM (blocks_half_conversion blocks per
+// job).
+hipError_t launch_interleave(const PassLaunch& p, hipStream_t s);
 
 // Lookups (one ray / point per thread). 'job' is a device pointer. With
 // 'luminance' the results are scaled by the job's radiance-to-luminance

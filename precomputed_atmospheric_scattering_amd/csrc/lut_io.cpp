@@ -156,6 +156,10 @@ atmo_status atmo_model_import_luts(atmo_model* m, const char* directory) {
     }
     ATMO_TRY(upload(m->d_scales, sc, sizeof(sc)));
   }
+  ATMO_TRY(build_lookup_copy(m, nullptr));
+  const hipError_t err = hipStreamSynchronize(nullptr);
+  if (err != hipSuccess)
+    return fail(ATMO_ERROR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(err));
   m->initialized = true;
   return ATMO_OK;
 }

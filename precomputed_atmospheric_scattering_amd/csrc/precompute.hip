@@ -1221,6 +1221,29 @@ __global__ __launch_bounds__(kBlock) void k_acc64_to_f32(const Job* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// The interleaved lookup copy of S and SM (ATMO_OPTION_INTERLEAVED_LOOKUPS):
+// texel t's S and SM side by side, so that a lookup tap reads both from one
+// place. FP16 models interleave their FP16 texels.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_interleave(const Job* __restrict__ jobs,
+                                                       const int* __restrict__ first_block,
+                                                       int njobs) {
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int64_t n = static_cast<int64_t>(J.nr) * J.nmu * J.nmus * J.nnu;
+  const int64_t t = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
+  if (t >= n || J.Q == nullptr || J.combined) return;
+  if (J.half_luts) {
+    const uint2 a = gload(J.S16, t), b = gload(J.SM16, t);
+    gstore(static_cast<uint4*>(J.Q), t, make_uint4(a.x, a.y, b.x, b.y));
+  } else {
+    float4* q = static_cast<float4*>(J.Q);
+    gstore(q, 2 * t, gload(J.S, t));
+    gstore(q, 2 * t + 1, gload(J.SM, t));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP16 storage of the scattering LUTs (half_precision): per-channel maximum
 // of the FP32 accumulators, then conversion with power-of-two scales that map
 // each channel's maximum into [2^13, 2^14), well inside the FP16 range, so
@@ -1388,6 +1411,11 @@ hipError_t launch_acc64_to_f32(const PassLaunch& p, hipStream_t s) {
 }
 hipError_t launch_clear_max(const PassLaunch& p, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_max, dim3(p.total_blocks), dim3(64), 0, s, p.jobs,
+                     p.first_block, p.njobs);
+  return hipGetLastError();
+}
+hipError_t launch_interleave(const PassLaunch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_interleave, dim3(p.total_blocks), dim3(kBlock), 0, s, p.jobs,
                      p.first_block, p.njobs);
   return hipGetLastError();
 }

@@ -52,6 +52,20 @@ __device__ D4 combined_scattering(const Job& J, double r, double mu, double mu_s
                                   double nu, bool ground, D4* single_mie) {
   const SCoords c = s_coords(J, r, mu, mu_s, nu, ground);
   D4 s;
+  if (J.Q != nullptr && !J.combined) {
+    // Interleaved copy: the S and SM texels of a tap in one fetch.
+    if (J.half_luts) {
+      const float* sc = J.scales;
+      const F16Pairs pt{static_cast<const uint4*>(J.Q),
+                        make_float4(1.0f / sc[0], 1.0f / sc[1], 1.0f / sc[2], 1.0f / sc[3]),
+                        make_float4(1.0f / sc[4], 1.0f / sc[5], 1.0f / sc[6], 0.0f)};
+      s_lookup_pair_d(J, pt, c, &s, single_mie);
+    } else {
+      s_lookup_pair_d(J, F32Pairs{static_cast<const float4*>(J.Q)}, c, &s, single_mie);
+    }
+    *single_mie = rgb(*single_mie);
+    return s;
+  }
   if (J.half_luts) {
     const float* sc = J.scales;
     const F16Texels st{J.S16, make_float4(1.0f / sc[0], 1.0f / sc[1], 1.0f / sc[2],

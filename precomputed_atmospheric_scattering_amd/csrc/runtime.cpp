@@ -186,6 +186,7 @@ int pass_blocks(Pass p, const atmo::Job& j) {
     case kHalf: return atmo::blocks_half_conversion(j);
     case kAcc: return atmo::blocks_combine(j);
     case kClear: return 1;
+    case kPack: return atmo::blocks_half_conversion(j);
     default: return 0;
   }
 }
@@ -322,6 +323,7 @@ void fill_job_luts(atmo_model* m, atmo::Job* j, int k) {
   j->half_luts = m->spec.half_precision ? 1 : 0;
   j->S16 = m->S16;
   j->SM16 = m->SM16;
+  j->Q = m->Q;
   j->acc64 = accumulates_fp64(m->spec) ? 1 : 0;
   j->S64 = par ? m->S64 + 4 * kk * z.s_texels() : m->S64;
   j->SM64 = par && m->SM64 != nullptr ? m->SM64 + 4 * kk * z.s_texels() : m->SM64;
@@ -391,6 +393,12 @@ atmo::PassLaunch model_launch(const atmo_model* m, int first_job, int njobs, Pas
   const atmo::Job& j = m->jobs[first_job];
   l.tile = p == kSD ? j.sd_tile : (p == kMS ? j.ms_tile : 0);
   return l;
+}
+
+atmo_status build_lookup_copy(atmo_model* m, hipStream_t s) {
+  if (m->Q == nullptr) return ATMO_OK;
+  return check_launch(atmo::launch_interleave(model_launch(m, m->iters, 1, kPack), s),
+                      "interleaved lookup copy");
 }
 
 atmo_status check_launch(hipError_t e, const char* what) {
@@ -578,6 +586,7 @@ size_t lut_bytes(const atmo_model* m, bool with_single_mie) {
   const size_t texel = m->spec.half_precision ? sizeof(uint2) : sizeof(float4);
   size_t b = (z.t_texels() + z.e_texels()) * sizeof(float4) + z.s_texels() * texel;
   if (with_single_mie) b += z.s_texels() * texel;
+  if (m->Q != nullptr) b += interleaved_bytes(z, m->spec.half_precision, m->spec.combined);
   return b;
 }
 
@@ -637,6 +646,8 @@ atmo_status atmo_model_create(const atmo_model_params* p,
     if (!s.combined) ATMO_HIP(hipMalloc(&m->SM, sz.s_texels() * sizeof(float4)));
   }
   ATMO_HIP(hipMalloc(&m->E, sz.e_texels() * sizeof(float4)));
+  if (!s.combined)
+    ATMO_HIP(hipMalloc(&m->Q, interleaved_bytes(sz, s.half_precision, s.combined)));
   const int iters = num_iterations(s);
   m->iters = iters;
   ATMO_HIP(hipMalloc(&m->d_jobs, (iters + 2) * sizeof(atmo::Job)));
@@ -651,7 +662,7 @@ void atmo_model_destroy(atmo_model* m) {
   if (m == nullptr) return;
   (void)hipDeviceSynchronize();
   free_temporaries(m);
-  void* bufs[8] = {m->T, m->S, m->SM, m->E, m->S16, m->SM16, m->d_scales, m->d_maxbits};
+  void* bufs[9] = {m->T, m->S, m->SM, m->E, m->S16, m->SM16, m->d_scales, m->d_maxbits, m->Q};
   for (void* b : bufs)
     if (b != nullptr) (void)hipFree(b);
   if (m->d_jobs != nullptr) (void)hipFree(m->d_jobs);
@@ -669,6 +680,24 @@ atmo_status atmo_model_set_option(atmo_model* m, atmo_option option, int value) 
     case ATMO_OPTION_PASS_TIMING:
       m->timing = value != 0;
       return ATMO_OK;
+    case ATMO_OPTION_INTERLEAVED_LOOKUPS: {
+      const bool on = value != 0 && !m->spec.combined;
+      if (on == (m->Q != nullptr)) return ATMO_OK;
+      // The job table holds Q: wait for work that may use it, switch, and
+      // upload the table again.
+      ATMO_HIP(hipDeviceSynchronize());
+      if (on) {
+        ATMO_HIP(hipMalloc(&m->Q, interleaved_bytes(m->sizes, m->spec.half_precision,
+                                                    m->spec.combined)));
+      } else {
+        ATMO_HIP(hipFree(m->Q));
+        m->Q = nullptr;
+      }
+      ATMO_TRY(upload_jobs(m, m->last_orders, nullptr));
+      if (on && m->initialized) ATMO_TRY(build_lookup_copy(m, nullptr));
+      ATMO_HIP(hipDeviceSynchronize());
+      return ATMO_OK;
+    }
   }
   return fail(ATMO_ERROR_INVALID_ARGUMENT, "unknown option");
 }
@@ -678,6 +707,7 @@ atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
   if (orders < 1) return fail(ATMO_ERROR_INVALID_ARGUMENT, "num_scattering_orders must be >= 1");
   hipStream_t s = static_cast<hipStream_t>(stream);
   ATMO_TRY(alloc_temporaries(m));
+  m->last_orders = orders;
   ATMO_TRY(upload_jobs(m, orders, s));
   const int iters = m->iters;
   m->timed.clear();
@@ -700,6 +730,7 @@ atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
     ATMO_TRY(check_launch(atmo::launch_to_half(model_launch(m, iters, 1, kHalf), s),
                           "fp16 conversion"));
   }
+  ATMO_TRY(build_lookup_copy(m, s));
   m->initialized = true;
   if (!m->retain) {
     // Init then returns once the precomputation is complete, as the GL
@@ -1199,17 +1230,19 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
     add_level(kMS, 0, on);
   }
   // Job 'iters' of a model works on its resident LUTs.
-  std::vector<std::pair<int, int>> rgb, half, acc;
+  std::vector<std::pair<int, int>> rgb, half, acc, pack;
   for (int i = 0; i < count; ++i) {
     const int last = models[i]->iters;
     if (models[i]->spec.num_precomputed_wavelengths > 3) rgb.push_back({i, last});
     if (models[i]->spec.half_precision) half.push_back({i, last});
     if (accumulates_fp64(models[i]->spec)) acc.push_back({i, last});
+    if (models[i]->Q != nullptr) pack.push_back({i, last});
   }
   add_level(kAcc, 0, acc);
   add_level(kT, 0, rgb);
   add_level(kClear, 0, half);
   add_level(kHalf, 0, half);
+  add_level(kPack, 0, pack);
 }
 
 atmo_status create_fork(Fork* f) {
@@ -1257,6 +1290,7 @@ atmo_status launch_level(const Level& l, const atmo::Job* d_jobs, const int* d_f
     case kClear: e = atmo::launch_clear_max(pl, s); break;
     case kAcc: e = atmo::launch_acc64_to_f32(pl, s); break;
     case kHalf: e = atmo::launch_to_half(pl, s); break;
+    case kPack: e = atmo::launch_interleave(pl, s); break;
     default: break;
   }
   return check_launch(e, "batched pass");
@@ -1333,6 +1367,7 @@ atmo_status atmo_batch_create(atmo_model* const* models, const unsigned* orders,
   // Temporaries and per-model job tables (the lookups use the latter).
   for (int i = 0; i < count; ++i) {
     ATMO_TRY(alloc_temporaries(models[i]));
+    models[i]->last_orders = orders[i];
     ATMO_TRY(upload_jobs(models[i], orders[i], nullptr));
   }
   ATMO_HIP(hipStreamSynchronize(nullptr));

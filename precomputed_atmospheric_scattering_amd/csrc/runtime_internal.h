@@ -37,7 +37,7 @@ struct Sizes {
 };
 
 
-enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kAcc, kHalf, kClear, kPasses };
+enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kAcc, kHalf, kClear, kPack, kPasses };
 
 
 // One launch of a batched pass sequence: pass kind, order argument, and the
@@ -85,6 +85,11 @@ struct atmo_model {
   // FP16 storage: resident half LUTs; S and SM above are then FP32
   // accumulators that live only during Init, like the deltas.
   uint2 *S16 = nullptr, *SM16 = nullptr;
+  // ATMO_OPTION_INTERLEAVED_LOOKUPS: S and SM interleaved texel by texel
+  // (float4 pairs, or one uint4 of two FP16 texels), for the lookups.
+  bool interleave = true;
+  void* Q = nullptr;
+  unsigned last_orders = 4;
   // FP64 accumulators of S and SM in illuminance mode (Init temporaries),
   // one per iteration: iteration k's at S64 + 4 * k * (scattering texels).
   double *S64 = nullptr, *SM64 = nullptr;
@@ -127,6 +132,11 @@ atmo_status validate_sizes(const atmo_lut_sizes& z);
 int num_iterations(const Spec& s);
 // Whether Init accumulates S and SM in FP64 (illuminance mode).
 inline bool accumulates_fp64(const Spec& s) { return s.num_precomputed_wavelengths > 3; }
+// Bytes of the interleaved lookup copy of a model with these sizes and
+// storage (0 with combined textures, which have no S_Mie).
+inline size_t interleaved_bytes(const Sizes& z, bool half, bool combined) {
+  return combined ? 0 : 2 * z.s_texels() * (half ? sizeof(uint2) : sizeof(float4));
+}
 int pass_blocks(Pass p, const atmo::Job& j);
 // Tile shapes for a model's LUT sizes: the autotuner's choice, or the
 // default shapes for sizes it has not tuned (autotune.cpp).
@@ -137,6 +147,8 @@ Tiles tiles_for(const Sizes& z);
 // Rebuild a model's job table for 'orders' and copy it to the device on
 // 'stream' (asynchronous with respect to other streams).
 atmo_status upload_jobs(atmo_model* m, unsigned orders, hipStream_t stream);
+// Fills the interleaved lookup copy from S and SM (if the model has one).
+atmo_status build_lookup_copy(atmo_model* m, hipStream_t stream);
 // Bytes of Init temporaries of a model (or of a model of these sizes and
 // storage with 'iters' Precompute iterations), and carving them out of
 // caller-owned memory (the model then does not free them).

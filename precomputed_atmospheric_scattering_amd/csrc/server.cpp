@@ -125,11 +125,13 @@ void footprint(const atmo_lut_sizes& z, bool half, bool combined, int iters,
   const size_t ns = static_cast<size_t>(z.scattering_r) * z.scattering_mu *
                     z.scattering_mu_s * z.scattering_nu;
   const size_t texel = half ? 8 : 16;
-  *resident = round256(nt * 16) + round256(ne * 16) +
-              (combined ? 1 : 2) * round256(ns * texel);
   const Sizes sz{z.transmittance_width, z.transmittance_height, z.scattering_r,
                  z.scattering_mu,       z.scattering_mu_s,      z.scattering_nu,
                  z.irradiance_width,    z.irradiance_height};
+  // The models keep the default interleaved lookup copy.
+  *resident = round256(nt * 16) + round256(ne * 16) +
+              (combined ? 1 : 2) * round256(ns * texel) +
+              round256(interleaved_bytes(sz, half, combined));
   *scratch = temporaries_bytes_for(sz, half, combined, iters);
 }
 

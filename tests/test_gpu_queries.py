@@ -278,3 +278,36 @@ def test_query_generator_matches_host_twin():
         # FP64 on both sides, rounded to FP32; sin/cos/sqrt may differ in the
         # last bit of FP64 before that rounding.
         assert np.allclose(g, host[k], rtol=2e-7, atol=2e-7), k
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_interleaved_lookups_equal_separate(half):
+    """The interleaved lookup copy of S and S_Mie (one fetch per tap) gives
+    bit-identical lookups to reading the two LUTs, in FP32 and FP16 storage,
+    for every lookup; turning the option off and on again on an initialized
+    model rebuilds the copy."""
+    cfg = configs.earth(half_precision=half, lut_sizes=dict(
+        transmittance_width=64, transmittance_height=16, scattering_r=8, scattering_mu=32,
+        scattering_mu_s=8, scattering_nu=8, irradiance_width=16, irradiance_height=4))
+    kw, orders = configs.model_kwargs(cfg)
+    rb, rt = kw["bottom_radius"] / 1000.0, kw["top_radius"] / 1000.0
+    q = configs.sky_queries(11, 4096, rb, rt)
+    shadow = np.where(np.arange(4096) % 3 == 0, 2.0, 0.0).astype(np.float32)
+    def lookups(m):
+        return (m.sky_radiance(q["camera"], q["view_ray"], shadow, q["sun_direction"]) +
+                m.sky_radiance_to_point(q["camera"], q["point"], shadow, q["sun_direction"]) +
+                m.sun_and_sky_irradiance(q["point"], q["normal"], q["sun_direction"]))
+    a = atmo.Model(**kw)  # interleaved copy (the default)
+    a.init(orders)
+    b = atmo.Model(interleaved_lookups=False, **kw)
+    b.init(orders)
+    ra, rb_ = lookups(a), lookups(b)
+    for x, y in zip(ra, rb_):
+        assert np.isfinite(x).all() and np.array_equal(x, y)
+    assert a.footprint()[0] > b.footprint()[0]
+    a.set_interleaved_lookups(False)
+    a.set_interleaved_lookups(True)
+    for x, y in zip(lookups(a), rb_):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
