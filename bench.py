@@ -65,6 +65,30 @@ def sd_flops(n_s, order):
     return 2.0 * n_s * SAMPLES["scattering_density"] * ops
 
 
+PROFILE_TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "hbm_traffic_spectral_workload.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of a kernel of the headline workload, from the
+    FETCH_SIZE and WRITE_SIZE passes summarized in the committed profile
+    (calibrated against a copy of known size): (bytes, source) or (None,
+    None) if the profile has no row for it."""
+    try:
+        rows = open(PROFILE_TRAFFIC).read().splitlines()
+    except OSError:
+        return None, None
+    for r in rows:
+        cells = [c.strip() for c in r.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == kernel:
+            try:
+                return round((float(cells[2]) + float(cells[3])) * 1e6), \
+                    "profiles/hbm_traffic_spectral_workload.md"
+            except ValueError:
+                return None, None
+    return None, None
+
+
 def cpu_baseline(cfg, gpu_luts, budget_s, threads):
     """FP64 oracle (this repository's CPU restatement of the reference) timed
     on a bounded sample of one precompute of the same atmosphere: the whole
@@ -436,8 +460,14 @@ def run_high_resolution(args):
     cfg = configs.high_resolution(half_precision=True)
     kw, orders = configs.model_kwargs(cfg)
     n_t, n_s, n_e = texel_counts(kw["lut_sizes"])
-    m = atmo.Model(retain_intermediates=True, pass_timing=True, **kw)
     stream = atmo.Stream()
+    # Tile shapes of this size class from the autotuner, before timing.
+    atmo.reset_tiles()
+    t0 = time.perf_counter()
+    atmo.autotune([kw["lut_sizes"]], stream=stream)
+    tune_s = time.perf_counter() - t0
+    tiles = atmo.get_tiles(kw["lut_sizes"])
+    m = atmo.Model(retain_intermediates=True, pass_timing=True, **kw)
     steps = max(1, min(args.steps, 5))
     for _ in range(max(1, min(args.warmup, 2))):
         m.init(orders, stream)
@@ -481,6 +511,11 @@ def run_high_resolution(args):
         "p50_ms": round(percentile(ms, 0.5), 3), "p99_ms": round(percentile(ms, 0.99), 3),
         "pass_ms_per_atmosphere": {k: round(v / steps, 3) for k, v in pass_ms.items()},
         "compulsory_gb_per_s": gbs,
+        "tiles": {"scattering_density": "%d nu x %d threads" % (
+                      tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
+                  "multiple_scattering": "%d texels per thread" % tiles[
+                      "ms_texels_per_thread"],
+                  "autotune_s": round(tune_s, 3)},
         "footprint_bytes": dict(zip(("resident", "scratch"), m.footprint())),
     })
     m.close()
@@ -540,9 +575,9 @@ def main():
     sizes = kw["lut_sizes"]
     n_t, n_s, n_e = texel_counts(sizes)
     stream = atmo.Stream()
-    t0 = time.perf_counter()
-    atmo.autotune([sizes], stream=stream)
-    tune_s = time.perf_counter() - t0
+    # Default tile shapes (no tuning launches in this run, so that a kernel
+    # profile of it holds the workload's launches only).
+    atmo.reset_tiles()
     tiles = atmo.get_tiles(sizes)
     model = atmo.Model(retain_intermediates=True, pass_timing=True, **kw)
     n_rays = WIDTH * HEIGHT
@@ -646,6 +681,7 @@ def main():
 
     steps = args.steps
     sd_avg_ms = sd_ms / sd_launches
+    sd_traffic, traffic_src = profiled_traffic("k_scattering_density")
     achieved_tflops = sd_flops_total / (sd_ms * 1e-3) / 1e12
     ms_per_atmo = sum(pre_ms) / steps
     query_s = sum(sum(l) for l in lookup_ms) * 1e-3
@@ -697,8 +733,7 @@ def main():
         "tiles": {"scattering_density": "%d nu x %d threads" % (
                       tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
                   "multiple_scattering": "%d texels per thread" % tiles[
-                      "ms_texels_per_thread"],
-                  "autotune_s": round(tune_s, 3)},
+                      "ms_texels_per_thread"]},
         "roofline": {
             "kernel": "k_scattering_density",
             "bound": "valu",
@@ -706,7 +741,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": sd_traffic,
+            "traffic_source": traffic_src,
             "avg_launch_ms": round(sd_avg_ms, 5),
             "algorithmic_flops_per_launch": round(sd_flops_total / sd_launches),
         },

@@ -203,3 +203,35 @@ def test_latencies_are_ordered():
         assert c["status"] == 0
         assert 0.0 <= c["submit_ms"] <= c["start_ms"] <= c["end_ms"]
     srv.close()
+
+
+def test_illuminance_requests_in_mix_equal_alone():
+    """Illuminance-mode requests (15 wavelengths: 5 Precompute iterations
+    run as parallel jobs with their own accumulators), FP32 and FP16, batched
+    by the server with 3-wavelength requests, give the LUTs of the same
+    models initialised alone."""
+    small = dict(transmittance_width=64, transmittance_height=16, scattering_r=8,
+                 scattering_mu=32, scattering_mu_s=8, scattering_nu=4,
+                 irradiance_width=16, irradiance_height=4)
+    cfgs = [configs.spectral(orders=3, lut_sizes=small),
+            configs.spectral(half_precision=True, orders=2, lut_sizes=small)]
+    cfgs += mix(77, 2)
+    ref = []
+    for c in cfgs:
+        kw, orders = configs.model_kwargs(c)
+        m = atmo.Model(**kw)
+        m.init(orders)
+        ref.append({k: m.lut(k) for k in LUTS})
+        m.close()
+    srv = atmo.Server(max_inflight_batches=1, max_batch_models=4)
+    ids = []
+    for c in cfgs:
+        kw, orders = configs.model_kwargs(c)
+        ids.append(srv.submit_precompute(num_scattering_orders=orders, **kw))
+    done = srv.wait(ids)
+    assert all(done[i]["status"] == 0 for i in ids), done
+    for i, mid in enumerate(ids):
+        m = srv.model(mid)
+        for k in LUTS:
+            assert np.array_equal(m.lut(k), ref[i][k]), (i, k)
+    srv.close()
