@@ -295,17 +295,6 @@ ATMO_HD double smoothstep_d(double e0, double e1, double x) {
   return t * t * (3.0 - 2.0 * t);
 }
 
-// functions.glsl:552-563
-ATMO_HD float3 transmittance_to_sun(const Job& j, const float4* T, double r,
-                                    double mu_s) {
-  double sin_h = j.rb / r;
-  double cos_h = -sqrt(1.0 - sin_h * sin_h > 0.0 ? 1.0 - sin_h * sin_h : 0.0);
-  float f = static_cast<float>(smoothstep_d(-sin_h * j.sun_angular_radius,
-                                            sin_h * j.sun_angular_radius,
-                                            mu_s - cos_h));
-  float3 t = t_lookup(j, T, r, mu_s);
-  return make_float3(t.x * f, t.y * f, t.z * f);
-}
 
 // ---------------------------------------------------------------------------
 // Scattering LUT (functions.glsl:773-976).
@@ -369,46 +358,6 @@ ATMO_HD SCoords s_coords(const Job& j, double r, double mu, double mu_s,
   return c;
 }
 
-// GetScattering (functions.glsl:958-976) with the GL texture() of each slice:
-// the two nu slices are read with their own trilinear (mu_s, mu, r) filter.
-// A slice index past the last one only ever gets weight 0 (the reference's
-// x > 1 coordinate is clamped by the sampler); it is clamped here too.
-ATMO_HD float4 s_lookup(const Job& j, const float4* lut, const SCoords& c) {
-  double cnu = c.nu;
-  if (!(cnu >= -1.0)) cnu = -1.0;
-  if (cnu > static_cast<double>(j.nnu)) cnu = static_cast<double>(j.nnu);
-  double f = floor(cnu);
-  float ws = static_cast<float>(cnu - f);
-  int s0 = static_cast<int>(f);
-  int s1 = s0 + 1;
-  s0 = s0 < 0 ? 0 : (s0 > j.nnu - 1 ? j.nnu - 1 : s0);
-  s1 = s1 < 0 ? 0 : (s1 > j.nnu - 1 ? j.nnu - 1 : s1);
-  Taps tz = axis_taps(c.z, j.nr), ty = axis_taps(c.y, j.nmu),
-       tx = axis_taps(c.x_in, j.nmus);
-  const int w = j.nnu * j.nmus;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int zs[2] = {tz.i0, tz.i1};
-  const int ys[2] = {ty.i0, ty.i1};
-  const int xs[2] = {tx.i0, tx.i1};
-  const float wz[2] = {1.0f - tz.w, tz.w};
-  const float wy[2] = {1.0f - ty.w, ty.w};
-  const float wx[2] = {1.0f - tx.w, tx.w};
-  const int ss[2] = {s0, s1};
-  const float wsl[2] = {1.0f - ws, ws};
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int e = 0; e < 2; ++e)
-        for (int q = 0; q < 2; ++q) {
-          float wt = wz[a] * wy[b] * wx[e] * wsl[q];
-          float4 v = gload(lut, (static_cast<int64_t>(zs[a]) * j.nmu + ys[b]) * w +
-                                    ss[q] * j.nmus + xs[e]);
-          acc.x += wt * v.x;
-          acc.y += wt * v.y;
-          acc.z += wt * v.z;
-          acc.w += wt * v.w;
-        }
-  return acc;
-}
 
 // FP64-accumulating variants for the lookups: texels are FP32, weights and
 // sums FP64, so that differences of two nearby lookups (GetSkyRadianceToPoint)
@@ -433,7 +382,10 @@ struct F16Texels {
 };
 
 // The 16 taps of GetScattering (functions.glsl:958-976) with the GL
-// texture() of each nu slice: f(texel index, FP64 weight) in a fixed order.
+// texture() of each nu slice, a trilinear (mu_s, mu, r) filter: f(texel
+// index, FP64 weight) in a fixed order. A slice index past the last one only
+// ever gets weight 0 (the reference's x > 1 coordinate is clamped by the
+// sampler); it is clamped here too.
 template <class F>
 ATMO_HD void s_taps(const Job& j, const SCoords& c, F&& f) {
   double cnu = c.nu;
@@ -623,15 +575,6 @@ ATMO_HD double s_slice_nu(const Job& j, int k, double mu, double mu_s) {
   return clamp_d(n, mu * mu_s - half_range, mu * mu_s + half_range);
 }
 
-ATMO_HD void s_texel_params(const Job& j, int x, int y, int z, double* r,
-                            double* mu, double* mu_s, double* nu,
-                            bool* ground) {
-  double rho;
-  s_layer_r(j, z, r, &rho);
-  s_row_mu(j, y, *r, rho, mu, ground);
-  *mu_s = s_col_mus(j, x % j.nmus);
-  *nu = s_slice_nu(j, x / j.nmus, *mu, *mu_s);
-}
 
 // ---------------------------------------------------------------------------
 // Irradiance LUT (functions.glsl:1524-1601).
@@ -645,12 +588,6 @@ ATMO_HD void e_rmus_from_texel(const Job& j, int i, int k, double* r,
   *mu_s = clamp_cos(2.0 * x_mu_s - 1.0);
 }
 
-ATMO_HD float3 e_lookup(const Job& j, const float4* E, double r, double mu_s) {
-  double x_r = (r - j.rb) / (j.rt - j.rb);
-  double x_mu_s = mu_s * 0.5 + 0.5;
-  return bilinear(E, j.ew, j.eh, unit_to_texel(x_mu_s, j.ew),
-                  unit_to_texel(x_r, j.eh));
-}
 
 ATMO_HD D4 e_lookup_d(const Job& j, const float4* E, double r, double mu_s) {
   double x_r = (r - j.rb) / (j.rt - j.rb);
@@ -667,14 +604,6 @@ ATMO_HD double mie_phase_d(double g, double nu) {
   const double k = 3.0 / (8.0 * kPi) * (1.0 - g * g) / (2.0 + g * g);
   const double y = 1.0 + g * g - 2.0 * g * nu;
   return k * (1.0 + nu * nu) / (y * sqrt(y));
-}
-ATMO_HD float rayleigh_phase(float nu) {
-  return 3.0f / (16.0f * 3.14159265358979f) * (1.0f + nu * nu);
-}
-ATMO_HD float mie_phase(float g, float nu) {
-  float k = 3.0f / (8.0f * 3.14159265358979f) * (1.0f - g * g) / (2.0f + g * g);
-  float y = 1.0f + g * g - 2.0f * g * nu;
-  return k * (1.0f + nu * nu) / (y * sqrtf(y));
 }
 
 // D4 texels of the FP64 accumulators (4 doubles per texel, 32 bytes).
