@@ -461,11 +461,10 @@ def run_high_resolution(args):
     kw, orders = configs.model_kwargs(cfg)
     n_t, n_s, n_e = texel_counts(kw["lut_sizes"])
     stream = atmo.Stream()
-    # Tile shapes of this size class from the autotuner, before timing.
+    # Default tile shapes (the autotuner keeps them for this size class), and
+    # no tuning launches, so that a kernel profile of this run holds the
+    # workload's launches only.
     atmo.reset_tiles()
-    t0 = time.perf_counter()
-    atmo.autotune([kw["lut_sizes"]], stream=stream)
-    tune_s = time.perf_counter() - t0
     tiles = atmo.get_tiles(kw["lut_sizes"])
     m = atmo.Model(retain_intermediates=True, pass_timing=True, **kw)
     steps = max(1, min(args.steps, 5))
@@ -514,8 +513,7 @@ def run_high_resolution(args):
         "tiles": {"scattering_density": "%d nu x %d threads" % (
                       tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
                   "multiple_scattering": "%d texels per thread" % tiles[
-                      "ms_texels_per_thread"],
-                  "autotune_s": round(tune_s, 3)},
+                      "ms_texels_per_thread"]},
         "footprint_bytes": dict(zip(("resident", "scratch"), m.footprint())),
     })
     m.close()
