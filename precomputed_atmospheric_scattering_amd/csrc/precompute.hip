@@ -1262,26 +1262,50 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
+// Launched over blocks_half_conversion blocks per job (one per 256 texels,
+// the grid of k_to_half); block b reduces texels [b * kMaxSpan, (b + 1) *
+// kMaxSpan) and the rest exit at once. Each thread keeps 64 texels' maxima,
+// the block combines its waves in LDS and one lane per channel updates the
+// job's maxima: 7 atomics per 16K texels.
+constexpr int kMaxPerThread = 64;
+constexpr int kMaxSpan = kBlock * kMaxPerThread;
 __global__ __launch_bounds__(kBlock) void k_lut_max(const Job* __restrict__ jobs,
                                                     const int* __restrict__ first_block,
                                                     int njobs) {
+  __shared__ float part[kBlock / 64][8];
   const int ji = find_job(first_block, njobs, blockIdx.x);
   const Job& J = jobs[ji];
   const int64_t n = static_cast<int64_t>(J.nr) * J.nmu * J.nmus * J.nnu;
-  const int64_t t = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
-  float4 s = make_float4(0.f, 0.f, 0.f, 0.f), m = s;
-  if (t < n) {
-    s = gload(J.S, t);
-    if (!J.combined) m = gload(J.SM, t);
-  }
+  const int64_t base = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kMaxSpan;
+  if (base >= n) return;  // uniform per block
   // |value| as non-negative float bits compare like unsigned integers.
-  const float v[8] = {fabsf(s.x), fabsf(s.y), fabsf(s.z), fabsf(s.w),
-                      fabsf(m.x), fabsf(m.y), fabsf(m.z), 0.0f};
-  const bool lane0 = (threadIdx.x & 63) == 0;
+  float v[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int k = 0; k < kMaxPerThread; ++k) {
+    const int64_t t = base + k * kBlock + threadIdx.x;
+    if (t >= n) break;
+    const float4 s = gload(J.S, t);
+    v[0] = fmaxf(v[0], fabsf(s.x));
+    v[1] = fmaxf(v[1], fabsf(s.y));
+    v[2] = fmaxf(v[2], fabsf(s.z));
+    v[3] = fmaxf(v[3], fabsf(s.w));
+    if (!J.combined) {
+      const float4 m = gload(J.SM, t);
+      v[4] = fmaxf(v[4], fabsf(m.x));
+      v[5] = fmaxf(v[5], fabsf(m.y));
+      v[6] = fmaxf(v[6], fabsf(m.z));
+    }
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
     const float w = wave_max(v[c]);
-    if (lane0 && w > 0.0f) atomicMax(&J.maxbits[c], __float_as_uint(w));
+    if (lane == 0) part[wave][c] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    float w = 0.0f;
+    for (int q = 0; q < kBlock / 64; ++q) w = fmaxf(w, part[q][threadIdx.x]);
+    if (w > 0.0f) atomicMax(&J.maxbits[threadIdx.x], __float_as_uint(w));
   }
 }
 
