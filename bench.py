@@ -267,8 +267,9 @@ def run_batched(args):
     shapes = {}
     for z in sizes:
         t = atmo.get_tiles(z)
-        key = "sd %dx%d, ms %d" % (t["sd_nu_per_thread"], t["sd_block_threads"],
-                                  t["ms_texels_per_thread"])
+        key = "sd %dx%d, ms %d%s" % (t["sd_nu_per_thread"], t["sd_block_threads"],
+                                    t["ms_texels_per_thread"],
+                                    " mu-major" if t["ms_mu_major"] else "")
         shapes[key] = shapes.get(key, 0) + 1
 
     def timed_batch(use_graph, steps, warmup):
@@ -512,8 +513,9 @@ def run_high_resolution(args):
         "compulsory_gb_per_s": gbs,
         "tiles": {"scattering_density": "%d nu x %d threads" % (
                       tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
-                  "multiple_scattering": "%d texels per thread" % tiles[
-                      "ms_texels_per_thread"]},
+                  "multiple_scattering": "%d texels per thread, %s rows" % (
+                      tiles["ms_texels_per_thread"],
+                      "mu-major" if tiles["ms_mu_major"] else "r-major")},
         "footprint_bytes": dict(zip(("resident", "scratch"), m.footprint())),
     })
     m.close()
@@ -730,8 +732,9 @@ def main():
         "pass_ms_per_atmosphere": {k: round(v / steps, 4) for k, v in pass_ms.items()},
         "tiles": {"scattering_density": "%d nu x %d threads" % (
                       tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
-                  "multiple_scattering": "%d texels per thread" % tiles[
-                      "ms_texels_per_thread"]},
+                  "multiple_scattering": "%d texels per thread, %s rows" % (
+                      tiles["ms_texels_per_thread"],
+                      "mu-major" if tiles["ms_mu_major"] else "r-major")},
         "roofline": {
             "kernel": "k_scattering_density",
             "bound": "valu",

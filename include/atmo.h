@@ -193,17 +193,22 @@ void atmo_batch_destroy(atmo_batch* batch);
 /* Tile shapes and their autotuner (no counterpart in the reference, whose
  * passes are fragment-shader draws). The scattering-density pass runs with
  * sd_nu_per_thread nu outputs per thread in blocks of sd_block_threads; the
- * multiple-scattering pass with ms_texels_per_thread texels per thread.
+ * multiple-scattering pass with ms_texels_per_thread texels per thread,
+ * its blocks walking the (r, mu) rows of the LUT r-major (ms_mu_major 0, the
+ * LUT's own order) or mu-major (1).
  * Every shape computes each texel identically: the LUTs do not depend on
  * the shapes. Shapes are chosen per LUT size class, (MU, NU) for the
- * density pass and (MU, MU_S, NU) for multiple scattering, in a table shared
- * by the process; a model's Init, batch or serving request reads it when its
- * job table is built. Classes never tuned or set use the defaults (4 nu per
- * thread, 256 threads; the fewest texels per thread that hold a row). */
+ * density pass and (R, MU, MU_S, NU) for multiple scattering, in a table
+ * shared by the process; a model's Init, batch or serving request reads it
+ * when its job table is built. Classes never tuned or set use the defaults
+ * (4 nu per thread, 256 threads; the fewest texels per thread that hold a
+ * row, mu-major rows once the R x MU x MU_S x NU float4 LUT reaches
+ * 64 MiB). */
 typedef struct atmo_tiles {
   int sd_nu_per_thread;
   int sd_block_threads;
   int ms_texels_per_thread;
+  int ms_mu_major;
   int tuned; /* atmo_get_tiles: bit 0 density class, bit 1 multiple class set */
 } atmo_tiles;
 /* Candidate shapes (returns the count; fills up to capacity entries). */
@@ -219,12 +224,14 @@ atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
                           void* stream);
 atmo_status atmo_get_tiles(const atmo_lut_sizes* sizes, atmo_tiles* out);
 /* Pins the shapes of the size classes of 'sizes' (the tuned field is
- * ignored); an error if a shape is not a candidate or if NU x MU_S texels
+ * ignored); an error if a shape is not a candidate, ms_mu_major is not 0 or
+ * 1, or if NU x MU_S texels
  * per row exceed 256 x ms_texels_per_thread (rows of at most 1024 texels;
  * longer rows take a gather kernel whatever the shape). */
 atmo_status atmo_set_tiles(const atmo_lut_sizes* sizes, const atmo_tiles* tiles);
 /* The table as text ("sd MU NU nu_per_thread block_threads",
- * "ms MU MU_S NU texels_per_thread"), merged back by atmo_tune_load. */
+ * "ms R MU MU_S NU texels_per_thread mu_major"), merged back by
+ * atmo_tune_load. */
 atmo_status atmo_tune_save(const char* path);
 atmo_status atmo_tune_load(const char* path);
 void atmo_tune_reset(void);

@@ -631,7 +631,8 @@ def synchronize():
 # ---------------------------------------------------------------------------
 # Tile shapes of the scattering-density and multiple-scattering passes, and
 # their autotuner (atmo_autotune and friends). A shape is
-# {"sd_nu_per_thread", "sd_block_threads", "ms_texels_per_thread"}; sizes are
+# {"sd_nu_per_thread", "sd_block_threads", "ms_texels_per_thread",
+# "ms_mu_major"}; sizes are
 # LUT-size dicts (DEFAULT_SIZES keys; missing keys take the defaults).
 # ---------------------------------------------------------------------------
 
@@ -669,11 +670,17 @@ def get_tiles(sizes=None):
     check(lib().atmo_get_tiles(C.byref(z), C.byref(t)), "atmo_get_tiles")
     return dict(sd_nu_per_thread=t.sd_nu_per_thread, sd_block_threads=t.sd_block_threads,
                 ms_texels_per_thread=t.ms_texels_per_thread,
-                sd_tuned=bool(t.tuned & 1), ms_tuned=bool(t.tuned & 2))
+                ms_mu_major=bool(t.ms_mu_major), sd_tuned=bool(t.tuned & 1), ms_tuned=bool(t.tuned & 2))
 
 
-def set_tiles(sizes, sd_nu_per_thread, sd_block_threads, ms_texels_per_thread):
-    t = _capi.Tiles(sd_nu_per_thread, sd_block_threads, ms_texels_per_thread, 0)
+def set_tiles(sizes, sd_nu_per_thread, sd_block_threads, ms_texels_per_thread,
+              ms_mu_major=None):
+    """Pins the shapes of the size classes of 'sizes'; ms_mu_major None
+    keeps the class's current row order."""
+    if ms_mu_major is None:
+        ms_mu_major = get_tiles(sizes)["ms_mu_major"]
+    t = _capi.Tiles(sd_nu_per_thread, sd_block_threads, ms_texels_per_thread,
+                    int(bool(ms_mu_major)), 0)
     z = _sizes(sizes)
     check(lib().atmo_set_tiles(C.byref(z), C.byref(t)), "atmo_set_tiles")
 

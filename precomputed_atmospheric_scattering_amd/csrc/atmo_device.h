@@ -88,6 +88,9 @@ struct Job {
   // Tile shapes (indices into kSdTile and kMsTpt of kernels.h): host-side
   // block counts and launch grouping; the kernels are instantiated per shape.
   int sd_tile, ms_tile;
+  // Order in which the multiple-scattering blocks walk the packed (r, mu)
+  // rows: 0 r-major (the LUT's own order), 1 mu-major (ms_row).
+  int ms_mu_major;
   // LUT sizes.
   int tw, th;
   int nr, nmu, nmus, nnu;

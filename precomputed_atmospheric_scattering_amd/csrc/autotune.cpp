@@ -8,7 +8,8 @@
 // shape on a batch of Earth-like atmospheres of the size in question, enough
 // copies to fill the GPU, and records the fastest per size class:
 //   scattering density:   (MU, NU)
-//   multiple scattering:  (MU, MU_S, NU)
+//   multiple scattering:  (R, MU, MU_S, NU), texels per thread and the
+//                         order of the blocks over the (r, mu) rows
 // The choice is read when a model's job table is built (Init, batches, the
 // serving loop); sizes never tuned get the default shapes.
 #include <hip/hip_runtime.h>
@@ -33,14 +34,24 @@ namespace atmo_rt {
 namespace {
 
 using SdKey = std::array<int, 2>;  // (MU, NU)
-using MsKey = std::array<int, 3>;  // (MU, MU_S, NU)
+using MsKey = std::array<int, 4>;  // (R, MU, MU_S, NU)
+
+// A multiple-scattering shape: tile (index into kMsTpt) and row order.
+struct MsShape {
+  int tile, mu_major;
+};
 
 std::mutex g_tiles_mu;
 std::map<SdKey, int> g_sd;
-std::map<MsKey, int> g_ms;
+std::map<MsKey, MsShape> g_ms;
 
 SdKey sd_key(const Sizes& z) { return {z.nmu, z.nnu}; }
-MsKey ms_key(const Sizes& z) { return {z.nmu, z.nmus, z.nnu}; }
+MsKey ms_key(const Sizes& z) { return {z.nr, z.nmu, z.nmus, z.nnu}; }
+
+MsShape ms_default(const Sizes& z) {
+  const int w = z.nnu * z.nmus;
+  return {atmo::ms_default_tile(w), atmo::ms_default_mu_major(z.nr, z.nmu, w)};
+}
 
 Sizes sizes_of(const atmo_lut_sizes& z) {
   return Sizes{z.transmittance_width, z.transmittance_height, z.scattering_r,
@@ -140,7 +151,7 @@ struct DeviceArrays {
 // Times every shape of the passes asked for on k copies of the tuning
 // atmosphere at sizes z; writes the fastest shapes.
 atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStream_t s,
-                      int* sd_out, int* ms_out) {
+                      int* sd_out, MsShape* ms_out) {
   const Sizes z = sizes_of(lz);
   const double texels = static_cast<double>(z.s_texels());
   const int k = std::max(1, std::min(kMaxCopies, static_cast<int>(std::ceil(kTuneTexels /
@@ -167,13 +178,15 @@ atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStrea
   TUNE_HIP(hipEventCreate(&d.start));
   TUNE_HIP(hipEventCreate(&d.stop));
   const int w = z.nnu * z.nmus;
-  auto time_pass = [&](Pass pass, int order, int tile, float* best) -> atmo_status {
+  auto time_pass = [&](Pass pass, int order, int tile, int mu_major,
+                       float* best) -> atmo_status {
     std::vector<atmo::Job> js(k);
     std::vector<int> fb(k + 1);
     int total = 0;
     for (int i = 0; i < k; ++i) {
       js[i] = models[i]->jobs[0];
       (pass == kSD ? js[i].sd_tile : js[i].ms_tile) = tile;
+      if (pass == kMS) js[i].ms_mu_major = mu_major;
       fb[i] = total;
       total += pass_blocks(pass, js[i]);
     }
@@ -222,23 +235,28 @@ atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStrea
     std::vector<double> t(atmo::kSdTiles);
     for (int tile = 0; tile < atmo::kSdTiles; ++tile) {
       float t2 = 0.0f, t3 = 0.0f;
-      ATMO_TRY(time_pass(kSD, 2, tile, &t2));
-      ATMO_TRY(time_pass(kSD, 3, tile, &t3));
+      ATMO_TRY(time_pass(kSD, 2, tile, 0, &t2));
+      ATMO_TRY(time_pass(kSD, 3, tile, 0, &t3));
       t[tile] = t2 + 2.0 * t3;
     }
     *sd_out = pick(t, 0);
   }
-  // Rows too long for LDS take the gather kernel whatever the tile.
-  if (do_ms && w <= atmo::kMsMaxRow) {
-    const int dflt = atmo::ms_default_tile(w);
-    std::vector<double> t(atmo::kMsTiles, std::numeric_limits<double>::max());
+  // Candidates 2 * tile + mu_major. Rows too long for LDS take the gather
+  // kernel whatever the tile: only the row order is timed for them.
+  if (do_ms) {
+    const MsShape dflt = ms_default(z);
+    std::vector<double> t(2 * atmo::kMsTiles, std::numeric_limits<double>::max());
     for (int tile = 0; tile < atmo::kMsTiles; ++tile) {
       if (!ms_tile_fits(z, tile)) continue;
-      float tm = 0.0f;
-      ATMO_TRY(time_pass(kMS, 0, tile, &tm));
-      t[tile] = tm;
+      if (w > atmo::kMsMaxRow && tile != dflt.tile) continue;
+      for (int mm = 0; mm < 2; ++mm) {
+        float tm = 0.0f;
+        ATMO_TRY(time_pass(kMS, 0, tile, mm, &tm));
+        t[2 * tile + mm] = tm;
+      }
     }
-    *ms_out = pick(t, dflt);
+    const int best = pick(t, 2 * dflt.tile + dflt.mu_major);
+    *ms_out = MsShape{best / 2, best % 2};
   }
   TUNE_HIP(hipStreamSynchronize(s));
   return ATMO_OK;
@@ -248,11 +266,12 @@ atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStrea
 
 Tiles tiles_for(const Sizes& z) {
   std::lock_guard<std::mutex> lock(g_tiles_mu);
-  Tiles t{0, atmo::ms_default_tile(z.nnu * z.nmus)};
+  MsShape ms = ms_default(z);
+  const auto b = g_ms.find(ms_key(z));
+  if (b != g_ms.end() && ms_tile_fits(z, b->second.tile)) ms = b->second;
+  Tiles t{0, ms.tile, ms.mu_major};
   const auto a = g_sd.find(sd_key(z));
   if (a != g_sd.end()) t.sd = a->second;
-  const auto b = g_ms.find(ms_key(z));
-  if (b != g_ms.end() && ms_tile_fits(z, b->second)) t.ms = b->second;
   return t;
 }
 
@@ -285,6 +304,7 @@ atmo_status atmo_get_tiles(const atmo_lut_sizes* sizes, atmo_tiles* out) {
   out->sd_nu_per_thread = atmo::kSdTile[t.sd].chunk;
   out->sd_block_threads = atmo::kSdTile[t.sd].block;
   out->ms_texels_per_thread = atmo::kMsTpt[t.ms];
+  out->ms_mu_major = t.ms_mu_major;
   std::lock_guard<std::mutex> lock(g_tiles_mu);
   out->tuned = (g_sd.count(sd_key(z)) ? 1 : 0) | (g_ms.count(ms_key(z)) ? 2 : 0);
   return ATMO_OK;
@@ -304,9 +324,11 @@ atmo_status atmo_set_tiles(const atmo_lut_sizes* sizes, const atmo_tiles* tiles)
   if (!ms_tile_fits(z, ms))
     return fail(ATMO_ERROR_INVALID_ARGUMENT,
                 "multiple-scattering rows of NU x MU_S texels do not fit this tile");
+  if (tiles->ms_mu_major != 0 && tiles->ms_mu_major != 1)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "ms_mu_major must be 0 or 1");
   std::lock_guard<std::mutex> lock(g_tiles_mu);
   g_sd[sd_key(z)] = sd;
-  g_ms[ms_key(z)] = ms;
+  g_ms[ms_key(z)] = MsShape{ms, tiles->ms_mu_major};
   return ATMO_OK;
 }
 
@@ -337,7 +359,8 @@ atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
     sd_done[sd_key(z)] = true;
     ms_done[ms_key(z)] = true;
     if (!do_sd && !do_ms) continue;
-    int sd = 0, ms = atmo::ms_default_tile(z.nnu * z.nmus);
+    int sd = 0;
+    MsShape ms = ms_default(z);
     ATMO_TRY(tune_size(sizes[i], do_sd, do_ms, s, &sd, &ms));
     std::lock_guard<std::mutex> lock(g_tiles_mu);
     if (do_sd) g_sd[sd_key(z)] = sd;
@@ -348,7 +371,7 @@ atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
 
 // Text file, one size class per line:
 //   sd <MU> <NU> <nu per thread> <block threads>
-//   ms <MU> <MU_S> <NU> <texels per thread>
+//   ms <R> <MU> <MU_S> <NU> <texels per thread> <mu_major>
 atmo_status atmo_tune_save(const char* path) {
   if (path == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "path is NULL");
   FILE* f = std::fopen(path, "w");
@@ -361,8 +384,9 @@ atmo_status atmo_tune_save(const char* path) {
                               atmo::kSdTile[e.second].chunk,
                               atmo::kSdTile[e.second].block) > 0;
     for (const auto& e : g_ms)
-      ok = ok && std::fprintf(f, "ms %d %d %d %d\n", e.first[0], e.first[1], e.first[2],
-                              atmo::kMsTpt[e.second]) > 0;
+      ok = ok && std::fprintf(f, "ms %d %d %d %d %d %d\n", e.first[0], e.first[1],
+                              e.first[2], e.first[3], atmo::kMsTpt[e.second.tile],
+                              e.second.mu_major) > 0;
   }
   ok = (std::fclose(f) == 0) && ok;
   if (!ok) return fail(ATMO_ERROR_INVALID_ARGUMENT, std::string("cannot write ") + path);
@@ -374,16 +398,16 @@ atmo_status atmo_tune_load(const char* path) {
   FILE* f = std::fopen(path, "r");
   if (f == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, std::string("cannot open ") + path);
   std::map<SdKey, int> sd;
-  std::map<MsKey, int> ms;
+  std::map<MsKey, MsShape> ms;
   char line[256];
   int lineno = 0;
   atmo_status st = ATMO_OK;
   while (st == ATMO_OK && std::fgets(line, sizeof(line), f) != nullptr) {
     ++lineno;
     char kind[4] = {};
-    int a = 0, b = 0, c = 0, e = 0;
+    int a = 0, b = 0, c = 0, e = 0, g = 0, h = 0;
     if (line[0] == '#' || line[0] == '\n') continue;
-    const int n = std::sscanf(line, "%3s %d %d %d %d", kind, &a, &b, &c, &e);
+    const int n = std::sscanf(line, "%3s %d %d %d %d %d %d", kind, &a, &b, &c, &e, &g, &h);
     const std::string where = std::string(path) + ":" + std::to_string(lineno);
     if (n == 5 && std::string(kind) == "sd") {
       const int t = sd_tile_of_shape(c, e);
@@ -391,16 +415,18 @@ atmo_status atmo_tune_load(const char* path) {
         st = fail(ATMO_ERROR_INVALID_ARGUMENT, where + ": not a scattering-density entry");
       else
         sd[{a, b}] = t;
-    } else if (n == 5 && std::string(kind) == "ms") {
-      const int t = ms_tile_of_shape(e);
+    } else if (n == 7 && std::string(kind) == "ms") {
+      const int t = ms_tile_of_shape(g);
       Sizes z{};
-      z.nmu = a;
-      z.nmus = b;
-      z.nnu = c;
-      if (t < 0 || a < 2 || b < 2 || c < 2 || !ms_tile_fits(z, t))
+      z.nr = a;
+      z.nmu = b;
+      z.nmus = c;
+      z.nnu = e;
+      if (t < 0 || a < 2 || b < 2 || c < 2 || e < 2 || (h != 0 && h != 1) ||
+          !ms_tile_fits(z, t))
         st = fail(ATMO_ERROR_INVALID_ARGUMENT, where + ": not a multiple-scattering entry");
       else
-        ms[{a, b, c}] = t;
+        ms[{a, b, c, e}] = MsShape{t, h};
     } else {
       st = fail(ATMO_ERROR_INVALID_ARGUMENT, where + ": unrecognized line");
     }

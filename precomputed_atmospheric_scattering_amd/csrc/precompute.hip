@@ -830,6 +830,21 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
 // blocks and each returns at once on the other's jobs, so the path of a job
 // depends on its own row length only.
 // ---------------------------------------------------------------------------
+// Packed row p of a job's block order: r-major (p = z * MU + y, the LUT's
+// own order) or, with ms_mu_major, mu-major (p = y * R + z), where the
+// blocks that run together share mu and step in r. The texels and their
+// values are the same either way; the order only changes which source rows
+// of the scattering density neighbouring blocks read.
+__device__ inline void ms_row(const Job& J, int p, int* z, int* y) {
+  if (J.ms_mu_major) {
+    *y = p / J.nr;
+    *z = p - *y * J.nr;
+  } else {
+    *z = p / J.nmu;
+    *y = p - *z * J.nmu;
+  }
+}
+
 struct MsRowSample {
   double d;      // distance along the view ray
   double inv_r;  // 1 / r_i
@@ -869,7 +884,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     const int slot = e / kMarch, i = e % kMarch;
     const int row = row0 + slot;
     if (row >= nrows) continue;
-    const int y = row % nmu, z = row / nmu;
+    int y, z;
+    ms_row(J, row, &z, &y);
     double r, rho, mu;
     bool ground;
     s_layer_r(J, z, &r, &rho);
@@ -916,11 +932,12 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     ws_of[j] = 0.0f;
     nu_of[j] = rmus_of[j] = 0.0;
     if (slot_of[j] < 0) continue;
-    const int row = row0 + slot;
+    int y, z;
+    ms_row(J, row0 + slot, &z, &y);
     double r, rho, mu;
     bool ground;
-    s_layer_r(J, row / nmu, &r, &rho);
-    s_row_mu(J, row % nmu, r, rho, &mu, &ground);
+    s_layer_r(J, z, &r, &rho);
+    s_row_mu(J, y, r, rho, &mu, &ground);
     const int x = x_of[j];
     const double mu_s = s_col_mus(J, x % nmus);
     const double nu = s_slice_nu(J, x / nmus, mu, mu_s);
@@ -1016,7 +1033,9 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
 #pragma unroll
   for (int j = 0; j < kTpt; ++j) {
     if (slot_of[j] < 0) continue;
-    const int64_t t_idx = static_cast<int64_t>(row0 + slot_of[j]) * W + x_of[j];
+    int y, z;
+    ms_row(J, row0 + slot_of[j], &z, &y);
+    const int64_t t_idx = (static_cast<int64_t>(z) * nmu + y) * W + x_of[j];
     gstore(J.dR, t_idx, make_float4(acc[j][0], acc[j][1], acc[j][2], 0.0f));
     const double nu = nu_of[j];
     if (J.acc64) {
@@ -1061,8 +1080,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
   if (W <= kMsMaxRow) return;  // LDS-path job
   const int tiles = (W + kBlock - 1) / kBlock;
   const int tile = lb % tiles;
-  const int row = lb / tiles;
-  const int y = row % nmu, z = row / nmu;
+  int y, z;
+  ms_row(J, lb / tiles, &z, &y);
   const int tid = threadIdx.x;
 
   double r, rho, mu;
@@ -1160,7 +1179,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
     a1 = fmaf(j1, S.f[1], a1);
     a2 = fmaf(j2, S.f[2], a2);
   }
-  const int64_t t_idx = static_cast<int64_t>(row) * W + x;
+  const int64_t t_idx = (static_cast<int64_t>(z) * nmu + y) * W + x;
   gstore(J.dR, t_idx, make_float4(a0, a1, a2, 0.0f));
   if (J.acc64) {
     const double k = 1.0 / ((3.0 / (16.0 * kPi)) * (1.0 + nu * nu));

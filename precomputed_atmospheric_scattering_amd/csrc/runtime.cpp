@@ -336,6 +336,7 @@ void fill_job_luts(atmo_model* m, atmo::Job* j, int k) {
   const Tiles t = tiles_for(m->sizes);
   j->sd_tile = t.sd;
   j->ms_tile = t.ms;
+  j->ms_mu_major = t.ms_mu_major;
 }
 
 // Rebuild the host job table (LUT pointers may have changed) and upload it.

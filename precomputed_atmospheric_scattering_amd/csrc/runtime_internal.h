@@ -142,6 +142,7 @@ int pass_blocks(Pass p, const atmo::Job& j);
 // default shapes for sizes it has not tuned (autotune.cpp).
 struct Tiles {
   int sd, ms;
+  int ms_mu_major;  // row order of the multiple-scattering blocks
 };
 Tiles tiles_for(const Sizes& z);
 // Rebuild a model's job table for 'orders' and copy it to the device on

@@ -104,3 +104,51 @@ def test_retained_intermediates_match_oracle_passes():
     g_ms = m.lut("delta_rayleigh").reshape(-1, 4)[idx]
     assert parity.rel_err(g_ms, ms)[0] <= parity.TOL_FP32
     m.close()
+
+
+# Packed rows of NU x MU_S = 2048 texels: longer than the 1024 the LDS path
+# of the multiple-scattering pass holds, so that pass takes its per-texel
+# gather kernel.
+LONG_ROW_SIZES = dict(transmittance_width=64, transmittance_height=16, scattering_r=4,
+                      scattering_mu=16, scattering_mu_s=64, scattering_nu=32,
+                      irradiance_width=16, irradiance_height=8)
+
+
+def test_long_rows_multiple_scattering_both_row_orders():
+    """The gather kernel of the multiple-scattering pass, pass-local against
+    the oracle on the GPU's own delta J, in both row orders of its blocks
+    (which must give bit-identical LUTs)."""
+    import ctypes as C
+    import oracle_lib as ol
+    import precomputed_atmospheric_scattering_amd as atmo
+    cfg = configs.earth(lut_sizes=LONG_ROW_SIZES)
+    spec, s, _ = parity.oracle_spec(cfg)
+    a = ol.lower(spec, (680.0, 550.0, 440.0))
+    n_s = s.scattering_r * s.scattering_mu * s.scattering_mu_s * s.scattering_nu
+    rng = np.random.Generator(np.random.PCG64(5))
+    idx = np.ascontiguousarray(rng.integers(0, n_s, 1500), dtype=np.int32)
+    ip = idx.ctypes.data_as(C.POINTER(C.c_int))
+    out = {}
+    try:
+        for mm in (False, True):
+            print("case long rows, mu-major" if mm else "case long rows, r-major", flush=True)
+            atmo.set_tiles(LONG_ROW_SIZES, 4, 256, 1, ms_mu_major=mm)
+            m = parity.gpu_model(cfg, retain_intermediates=True)
+            m.init(2)
+            T = np.ascontiguousarray(m.lut("transmittance"), dtype=np.float64)
+            dJ = np.ascontiguousarray(m.lut("delta_scattering_density"), dtype=np.float64)
+            ms = np.zeros((len(idx), 4))
+            ol.lib().or_texels_multiple_scattering(C.byref(a), s, ol.ptr(T), ol.ptr(dJ),
+                                                   len(idx), ip, ol.ptr(ms))
+            g = m.lut("delta_rayleigh").reshape(-1, 4)[idx]
+            e = parity.rel_err(g, ms)[0]
+            print("long rows, multiple scattering order 2: max rel err %.2e" % e, flush=True)
+            assert e <= parity.TOL_FP32, (mm, e)
+            m.init(4)
+            out[mm] = {k: m.lut(k) for k in ("scattering", "single_mie_scattering",
+                                             "irradiance")}
+            m.close()
+    finally:
+        atmo.reset_tiles()
+    for k in out[False]:
+        assert np.array_equal(out[False][k], out[True][k]), k

@@ -1,5 +1,6 @@
-"""Tile shapes of the scattering-density and multiple-scattering passes:
-every shape gives bit-identical LUTs (so the autotuner's choice never
+"""Tile shapes of the scattering-density and multiple-scattering passes
+(with the row order of the multiple-scattering blocks): every shape gives
+bit-identical LUTs (so the autotuner's choice never
 changes a result), the autotuner records a valid shape per size class, and
 a batch whose jobs have different shapes (one launch per shape) equals each
 atmosphere precomputed alone."""
@@ -46,9 +47,11 @@ def shapes_for(sizes):
     sd, ms = atmo.tile_candidates()
     d = atmo.get_tiles(sizes)
     dsd = (d["sd_nu_per_thread"], d["sd_block_threads"])
-    out = [(s, d["ms_texels_per_thread"]) for s in sd if s != dsd]
+    dmm = d["ms_mu_major"]
+    out = [(s, d["ms_texels_per_thread"], dmm) for s in sd if s != dsd]
     w = sizes["scattering_nu"] * sizes["scattering_mu_s"]
-    out += [(dsd, t) for t in ms if t != d["ms_texels_per_thread"] and w <= 256 * t]
+    out += [(dsd, t, mm) for t in ms for mm in (False, True)
+            if (t, mm) != (d["ms_texels_per_thread"], dmm) and w <= 256 * t]
     return out
 
 
@@ -57,12 +60,12 @@ def test_every_shape_gives_identical_luts(name):
     sizes = SIZES[name]
     cfg = configs.earth(orders=4, lut_sizes=sizes)
     ref = run(cfg)
-    for (chunk, block), tpt in shapes_for(sizes):
-        print("case", name, chunk, block, tpt, flush=True)
-        atmo.set_tiles(sizes, chunk, block, tpt)
+    for (chunk, block), tpt, mm in shapes_for(sizes):
+        print("case", name, chunk, block, tpt, "mu-major" if mm else "r-major", flush=True)
+        atmo.set_tiles(sizes, chunk, block, tpt, ms_mu_major=mm)
         got = run(cfg)
         for n in NAMES:
-            assert np.array_equal(got[n], ref[n]), (name, chunk, block, tpt, n)
+            assert np.array_equal(got[n], ref[n]), (name, chunk, block, tpt, mm, n)
         atmo.reset_tiles()
 
 
@@ -71,9 +74,10 @@ def test_every_shape_identical_illuminance_mode():
     sizes = SIZES["small"]
     cfg = configs.spectral(orders=3, lut_sizes=sizes)
     ref = run(cfg)
-    for (chunk, block), tpt in [((1, 256), 1), ((4, 64), 2), ((2, 128), 4)]:
-        print("case spectral", chunk, block, tpt, flush=True)
-        atmo.set_tiles(sizes, chunk, block, tpt)
+    for (chunk, block), tpt, mm in [((1, 256), 1, True), ((4, 64), 2, False),
+                                     ((2, 128), 4, True)]:
+        print("case spectral", chunk, block, tpt, mm, flush=True)
+        atmo.set_tiles(sizes, chunk, block, tpt, ms_mu_major=mm)
         got = run(cfg)
         for n in NAMES:
             assert np.array_equal(got[n], ref[n]), (chunk, block, tpt, n)
@@ -96,9 +100,9 @@ def test_autotune_then_mixed_shape_batch_equals_alone(use_graph):
     # Force two shapes on the first two size classes so that the batch has
     # jobs of different shapes in one pass level whatever the tuner chose.
     atmo.set_tiles(sizes[0], 1, 256, 4 if sizes[0]["scattering_nu"] *
-                   sizes[0]["scattering_mu_s"] <= 1024 else 1)
+                   sizes[0]["scattering_mu_s"] <= 1024 else 1, ms_mu_major=True)
     atmo.set_tiles(sizes[1], 4, 64, 4 if sizes[1]["scattering_nu"] *
-                   sizes[1]["scattering_mu_s"] <= 1024 else 1)
+                   sizes[1]["scattering_mu_s"] <= 1024 else 1, ms_mu_major=False)
     models = [atmo.Model(**configs.model_kwargs(c)[0]) for c in cfgs]
     b = atmo.Batch(models, [c["orders"] for c in cfgs], use_graph=use_graph)
     # The launches of one level that differ in tile shape run concurrently
