@@ -118,6 +118,11 @@ struct Job {
   // which start at S64, SM64 and E_it, into S, SM and E.
   int acc64;
   int niter;
+  // Optical depths of the transmittance texels (4 doubles per texel: the
+  // Rayleigh, Mie and absorption density integrals), shared by the jobs of
+  // one model: they depend on the geometry and density profiles only, not
+  // on the wavelengths. Null: the transmittance pass integrates them itself.
+  double* OD;
   double* S64;
   double* SM64;
   float4* E_it;

@@ -80,6 +80,7 @@ ATMO_HD int ms_default_mu_major(int nr, int nmu, int w) {
 }
 
 // Blocks one job needs for each pass.
+int blocks_optical_depth(const Job& j);
 int blocks_transmittance(const Job& j);
 int blocks_direct_irradiance(const Job& j);
 int blocks_single_scattering(const Job& j);
@@ -89,6 +90,9 @@ int blocks_multiple_scattering(const Job& j);
 int blocks_half_conversion(const Job& j);
 int blocks_combine(const Job& j);
 
+// The optical depths of the jobs' transmittance texels into their OD (one
+// job per model), read by launch_transmittance for jobs whose OD is set.
+hipError_t launch_optical_depth(const PassLaunch& p, hipStream_t s);
 hipError_t launch_transmittance(const PassLaunch& p, hipStream_t s);
 hipError_t launch_direct_irradiance(const PassLaunch& p, hipStream_t s);
 hipError_t launch_single_scattering(const PassLaunch& p, hipStream_t s);

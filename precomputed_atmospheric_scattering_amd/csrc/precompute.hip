@@ -95,17 +95,16 @@ __device__ constexpr float kSinPhi[16] = {
 // ---------------------------------------------------------------------------
 // Transmittance (functions.glsl:275-320, :454-463). FP64 throughout: the
 // 501-sample trapezoid of each texel is split over the 64 lanes of a wave and
-// reduced with cross-lane shuffles. 4 texels per block.
+// reduced with cross-lane shuffles, 4 texels per block. The three optical
+// depths do not depend on the wavelengths, so a model with several
+// Precompute iterations integrates them once (k_optical_depth, into the
+// job's OD) and each of its transmittance LUTs is then one exp per texel
+// and channel; a model with one iteration integrates them in
+// k_transmittance itself.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_transmittance(
-    const Job* __restrict__ jobs, const int* __restrict__ first_block,
-    int njobs) {
-  const int ji = find_job(first_block, njobs, blockIdx.x);
-  const Job& J = jobs[ji];
-  const int lb = blockIdx.x - first_block[ji];
-  const int lane = threadIdx.x & 63;
-  const int texel = lb * (kBlock / 64) + (threadIdx.x >> 6);
-  if (texel >= J.tw * J.th) return;
+// The optical depths of transmittance texel 'texel' (sums over the wave's
+// lanes, returned on every lane).
+__device__ D4 optical_depths(const Job& J, int texel, int lane) {
   const int i = texel % J.tw, k = texel / J.tw;
   double r, mu;
   t_rmu_from_uv(J, (i + 0.5) / J.tw, (k + 0.5) / J.th, &r, &mu);
@@ -126,20 +125,53 @@ __global__ __launch_bounds__(kBlock) void k_transmittance(
     lm += __shfl_xor(lm, off);
     la += __shfl_xor(la, off);
   }
-  if (lane == 0) {
-    lr *= dx;
-    lm *= dx;
-    la *= dx;
-    float4 t;
-    t.x = static_cast<float>(exp(-(J.rayleigh[0] * lr + J.mie_ext[0] * lm +
-                                   J.absorption[0] * la)));
-    t.y = static_cast<float>(exp(-(J.rayleigh[1] * lr + J.mie_ext[1] * lm +
-                                   J.absorption[1] * la)));
-    t.z = static_cast<float>(exp(-(J.rayleigh[2] * lr + J.mie_ext[2] * lm +
-                                   J.absorption[2] * la)));
-    t.w = 0.0f;
-    gstore(J.T, texel, t);
+  return D4{lr * dx, lm * dx, la * dx, 0.0};
+}
+
+__device__ void store_transmittance(const Job& J, int texel, const D4& od) {
+  float4 t;
+  t.x = static_cast<float>(exp(-(J.rayleigh[0] * od.x + J.mie_ext[0] * od.y +
+                                 J.absorption[0] * od.z)));
+  t.y = static_cast<float>(exp(-(J.rayleigh[1] * od.x + J.mie_ext[1] * od.y +
+                                 J.absorption[1] * od.z)));
+  t.z = static_cast<float>(exp(-(J.rayleigh[2] * od.x + J.mie_ext[2] * od.y +
+                                 J.absorption[2] * od.z)));
+  t.w = 0.0f;
+  gstore(J.T, texel, t);
+}
+
+// One wave per texel (4 texels per block), into the job's OD.
+__global__ __launch_bounds__(kBlock) void k_optical_depth(
+    const Job* __restrict__ jobs, const int* __restrict__ first_block,
+    int njobs) {
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int lb = blockIdx.x - first_block[ji];
+  const int lane = threadIdx.x & 63;
+  const int texel = lb * (kBlock / 64) + (threadIdx.x >> 6);
+  if (texel >= J.tw * J.th || J.OD == nullptr) return;
+  const D4 od = optical_depths(J, texel, lane);
+  if (lane == 0) store_d4(J.OD, texel, od);
+}
+
+// Jobs with OD: one thread per texel. Others: one wave per texel (4 texels
+// per block), integrating the optical depths.
+__global__ __launch_bounds__(kBlock) void k_transmittance(
+    const Job* __restrict__ jobs, const int* __restrict__ first_block,
+    int njobs) {
+  const int ji = find_job(first_block, njobs, blockIdx.x);
+  const Job& J = jobs[ji];
+  const int lb = blockIdx.x - first_block[ji];
+  if (J.OD != nullptr) {
+    const int texel = lb * kBlock + threadIdx.x;
+    if (texel < J.tw * J.th) store_transmittance(J, texel, load_d4(J.OD, texel));
+    return;
   }
+  const int lane = threadIdx.x & 63;
+  const int texel = lb * (kBlock / 64) + (threadIdx.x >> 6);
+  if (texel >= J.tw * J.th) return;
+  const D4 od = optical_depths(J, texel, lane);
+  if (lane == 0) store_transmittance(J, texel, od);
 }
 
 // ---------------------------------------------------------------------------
@@ -228,11 +260,14 @@ __device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<float>&
   o[2] = fmaf(wy1, fmaf(w, d.z - c.z, c.z), wy0 * fmaf(w, b.z - a.z, a.z));
 }
 
+// In FP64 the distance needs no cancellation-free form: sq - tt loses at most
+// about 1e-16 sq = 6e-13 km, 1e-13 texel of the LUT coordinate, and saves a
+// division and a divergent branch per sample.
 __device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<double>& S,
                                             double mus, double o[3]) {
   const double tt = S.r * mus;
   const double sq = sqrt(fma(tt, tt, S.c));
-  const double dd = tt > 0.0 ? S.c / (sq + tt) : sq - tt;
+  const double dd = sq - tt;
   double ut = (dd - S.dmin) * S.inv_range * (J.tw - 1);
   ut = fmin(fmax(ut, 0.0), static_cast<double>(J.tw - 1));
   const double fl = floor(ut);
@@ -1363,8 +1398,12 @@ __global__ __launch_bounds__(kBlock) void k_to_half(const Job* __restrict__ jobs
 // Launch geometry and launchers.
 // ---------------------------------------------------------------------------
 
-int blocks_transmittance(const Job& j) {
+int blocks_optical_depth(const Job& j) {
   return (j.tw * j.th + kBlock / 64 - 1) / (kBlock / 64);
+}
+int blocks_transmittance(const Job& j) {
+  if (j.OD != nullptr) return (j.tw * j.th + kBlock - 1) / kBlock;
+  return blocks_optical_depth(j);
 }
 int blocks_direct_irradiance(const Job& j) {
   return (j.ew * j.eh + kBlock - 1) / kBlock;
@@ -1397,6 +1436,11 @@ int blocks_half_conversion(const Job& j) {
   return static_cast<int>((n + kBlock - 1) / kBlock);
 }
 
+hipError_t launch_optical_depth(const PassLaunch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_optical_depth, dim3(p.total_blocks), dim3(kBlock), 0, s,
+                     p.jobs, p.first_block, p.njobs);
+  return hipGetLastError();
+}
 hipError_t launch_transmittance(const PassLaunch& p, hipStream_t s) {
   hipLaunchKernelGGL(k_transmittance, dim3(p.total_blocks), dim3(kBlock), 0, s,
                      p.jobs, p.first_block, p.njobs);

@@ -187,6 +187,7 @@ int pass_blocks(Pass p, const atmo::Job& j) {
     case kAcc: return atmo::blocks_combine(j);
     case kClear: return 1;
     case kPack: return atmo::blocks_half_conversion(j);
+    case kOD: return atmo::blocks_optical_depth(j);
     default: return 0;
   }
 }
@@ -205,6 +206,7 @@ void clear_temporary_pointers(atmo_model* m) {
   m->dE = m->dR = m->dM = m->dJ = nullptr;
   if (m->spec.half_precision) m->S = m->SM = nullptr;
   m->S64 = m->SM64 = nullptr;
+  m->OD = nullptr;
 }
 
 size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
@@ -212,8 +214,8 @@ size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 // The layout of the temporaries block; with 'base' null, only its size.
 // Illuminance mode (iters > 1): per iteration a transmittance LUT, an
 // irradiance accumulator, the four delta LUTs and the FP64 accumulators of S
-// and SM, each array of 'iters' copies contiguous. FP16 storage adds the
-// FP32 accumulators of S and SM.
+// and SM, each array of 'iters' copies contiguous, and one array of optical
+// depths. FP16 storage adds the FP32 accumulators of S and SM.
 size_t carve(const Sizes& z, bool half, bool combined, int iters, char* base,
              atmo_model* m) {
   const bool par = iters > 1;
@@ -234,6 +236,7 @@ size_t carve(const Sizes& z, bool half, bool combined, int iters, char* base,
   char* sm32 = half && !combined ? take(z.s_texels() * sizeof(float4)) : nullptr;
   char* s64 = par ? take(P * z.s_texels() * 4 * sizeof(double)) : nullptr;
   char* sm64 = par && !combined ? take(P * z.s_texels() * 4 * sizeof(double)) : nullptr;
+  char* od = par ? take(z.t_texels() * 4 * sizeof(double)) : nullptr;
   if (m != nullptr) {
     m->T_it = reinterpret_cast<float4*>(t_it);
     m->E_it = reinterpret_cast<float4*>(e_it);
@@ -252,6 +255,7 @@ size_t carve(const Sizes& z, bool half, bool combined, int iters, char* base,
     }
     m->S64 = reinterpret_cast<double*>(s64);
     m->SM64 = reinterpret_cast<double*>(sm64);
+    m->OD = reinterpret_cast<double*>(od);
   }
   return off;
 }
@@ -318,6 +322,7 @@ void fill_job_luts(atmo_model* m, atmo::Job* j, int k) {
   j->dM = temps ? m->dM_it + kk * z.s_texels() : nullptr;
   j->dJ = temps ? m->dJ_it + kk * z.s_texels() : nullptr;
   j->niter = m->iters;
+  j->OD = m->OD;  // every job of the model, the final transmittance's too
   set_sizes(j, m->sizes);
   j->combined = m->spec.combined ? 1 : 0;
   j->half_luts = m->spec.half_precision ? 1 : 0;
@@ -448,6 +453,11 @@ struct LaunchTimer {
 atmo_status enqueue_precompute(atmo_model* m, unsigned orders, LaunchTimer& t) {
   hipStream_t s = t.s;
   const int n = m->iters;
+  if (m->OD != nullptr) {
+    ATMO_TRY(t.run(kT, 0, "optical depth", [&] {
+      return atmo::launch_optical_depth(model_launch(m, 0, 1, kOD), s);
+    }));
+  }
   ATMO_TRY(t.run(kT, 0, "transmittance", [&] {
     return atmo::launch_transmittance(model_launch(m, 0, n, kT), s);
   }));
@@ -1215,10 +1225,15 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
   // iterations of one model have their own temporaries and accumulators.
   unsigned max_orders = 0;
   std::vector<std::pair<int, int>> sel;
+  std::vector<std::pair<int, int>> od;
   for (int i = 0; i < count; ++i) {
     max_orders = std::max(max_orders, orders[i]);
     for (int it = 0; it < models[i]->iters; ++it) sel.push_back({i, it});
+    if (models[i]->jobs[0].OD != nullptr) od.push_back({i, 0});
   }
+  // The optical depths, once per model that shares them among its
+  // transmittance LUTs.
+  add_level(kOD, 0, od);
   add_level(kT, 0, sel);
   add_level(kE0, 0, sel);
   add_level(kSS, 0, sel);
@@ -1292,6 +1307,7 @@ atmo_status launch_level(const Level& l, const atmo::Job* d_jobs, const int* d_f
     case kAcc: e = atmo::launch_acc64_to_f32(pl, s); break;
     case kHalf: e = atmo::launch_to_half(pl, s); break;
     case kPack: e = atmo::launch_interleave(pl, s); break;
+    case kOD: e = atmo::launch_optical_depth(pl, s); break;
     default: break;
   }
   return check_launch(e, "batched pass");

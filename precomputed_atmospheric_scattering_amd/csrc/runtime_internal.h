@@ -37,7 +37,7 @@ struct Sizes {
 };
 
 
-enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kAcc, kHalf, kClear, kPack, kPasses };
+enum Pass { kT = 0, kE0, kSS, kSD, kII, kMS, kAcc, kHalf, kClear, kPack, kOD, kPasses };
 
 
 // One launch of a batched pass sequence: pass kind, order argument, and the
@@ -93,6 +93,9 @@ struct atmo_model {
   // FP64 accumulators of S and SM in illuminance mode (Init temporaries),
   // one per iteration: iteration k's at S64 + 4 * k * (scattering texels).
   double *S64 = nullptr, *SM64 = nullptr;
+  // Optical depths of the transmittance texels, shared by the iterations'
+  // transmittance LUTs and the final one (Init temporary, illuminance mode).
+  double* OD = nullptr;
   float* d_scales = nullptr;
   unsigned* d_maxbits = nullptr;
   // Device job table: [iterations..., rgb transmittance job, query job].
