@@ -215,6 +215,15 @@ __global__ __launch_bounds__(kBlock) void k_direct_irradiance(
 // ---------------------------------------------------------------------------
 constexpr int kMarch = 51;  // 50 trapezoid intervals
 
+// Integer part and fraction of a texel coordinate already clamped to
+// [0, hi]: for t >= 0 truncation is floor, and v_fract_f32 returns
+// t - floor(t) exactly (a difference of two floats within a factor 2, or t
+// itself below 1), one instruction fewer than floor and subtract.
+__device__ __forceinline__ int split_coord(float t, float* frac) {
+  *frac = __builtin_amdgcn_fractf(t);
+  return static_cast<int>(t);
+}
+
 // FP32 path (radiance mode) and FP64 path (illuminance mode, whose S and SM
 // accumulate in FP64): per-sample geometry in Real, the r mu_s + d nu term
 // and the horizon term of every texel in FP64 in both.
@@ -246,9 +255,8 @@ __device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<float>&
   const float xm = (dd - S.dmin) * S.inv_range;
   const float ut = clamp_index_coord(xm * static_cast<float>(J.tw - 1),
                                      static_cast<float>(J.tw - 1));
-  const float fl = floorf(ut);
-  const float w = ut - fl;
-  const int i0 = static_cast<int>(fl);
+  float w;
+  const int i0 = split_coord(ut, &w);
   const int i1 = min(i0 + 1, J.tw - 1);
   const float4* __restrict__ T = J.T;
   const uint32_t r0 = static_cast<uint32_t>(S.t0), r1 = static_cast<uint32_t>(S.t1);
@@ -425,21 +433,30 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
 // ---------------------------------------------------------------------------
 constexpr int kTheta = 16;
 
-// Table entry i: (v.x, v.y, v.z, dv.x), (dv.y, dv.z, -, -) with
-// dv = v[i + 1] - v[i] (0 for the last entry).
-__device__ __forceinline__ float3 table_lerp(const float4* __restrict__ tab,
-                                             float t_max, float t) {
-  // Median of (t, 0, t_max): one instruction. The coordinates come from
-  // finite dot products of unit vectors; were t NaN, the float -> int
-  // conversion below gives 0, still in bounds.
-  t = __builtin_amdgcn_fmed3f(t, 0.0f, t_max);
-  const float fl = floorf(t);
-  const float w = t - fl;
-  const int i = static_cast<int>(fl);
+// A table of n values v[0..n-1] has n + 1 entries of two float4:
+// (v.x, v.y, v.z, dv.x), (dv.y, dv.z, -, -). Entry 0 is a guard holding v[0]
+// with dv = 0; entry k >= 1 holds v[k - 1] with dv = v[k] - v[k - 1], and
+// dv = 0 for the last. It is read at the coordinate t' = t + 1 of the
+// texel-space coordinate t in [0, n - 1]: this reproduces GL's clamp to
+// edge for every t in (-2, n) without a clamp instruction. Below 0 the
+// guard gives v[0] whatever the weight; above n - 1 the last entry gives
+// v[n - 1]. The coordinates are finite dot products of unit vectors mapped
+// to [0, n - 1], off by rounding at most; were t' NaN, the float -> int
+// conversion gives 0, still in bounds.
+__device__ __forceinline__ float3 table_lerp(const float4* __restrict__ tab, float t1) {
+  float w;
+  const int i = split_coord(t1, &w);
   const float4 e0 = tab[2 * i];
   const float4 e1 = tab[2 * i + 1];
   return make_float3(fmaf(w, e0.w, e0.x), fmaf(w, e1.x, e0.y),
                      fmaf(w, e1.y, e0.z));
+}
+
+// Writes table entry k (k >= 1: v[k - 1] and its successor vn; k = 0: the
+// guard, vn = v).
+__device__ __forceinline__ void put_entry(float4* tab, int k, float3 v, float3 vn) {
+  tab[2 * k] = make_float4(v.x, v.y, v.z, vn.x - v.x);
+  tab[2 * k + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
 }
 
 struct SdTheta {
@@ -466,7 +483,8 @@ __device__ __forceinline__ void sd_theta(
   // table coordinate (mu_g + 1) / 2 * (E_W - 1) with mu_g = ga + gb nu1.
   const float hn = 0.5f * nu_max;
   const float inv_hn = 1.0f / hn;
-  const float tA = fmaf(A, hn, hn);
+  // t' = t + 1 (table_lerp): the 1 is folded into the constant term.
+  const float tA = fmaf(A, hn, hn + 1.0f);
   float tB[kChunk], tC[kChunk];
 #pragma unroll
   for (int j = 0; j < kChunk; ++j) {
@@ -475,7 +493,9 @@ __device__ __forceinline__ void sd_theta(
   }
   const float he = 0.5f * e_max;
   const float eB = th.gb * inv_hn * he;
-  const float eA = fmaf(th.ga - th.gb, he, he);
+  // The ground-irradiance coordinate plus 1, as a function of t' = t + 1.
+  const float eA = fmaf(th.ga - th.gb, he, he) - eB + 1.0f;
+  const float nu1_0 = -1.0f - inv_hn;  // nu1 = t' / hn + nu1_0
 #pragma unroll
   for (int m = 0; m < 16; ++m) {
     const float c = kCosPhi[m], s = kSinPhi[m];
@@ -495,13 +515,13 @@ __device__ __forceinline__ void sd_theta(
 #pragma unroll
       for (int sgn = 0; sgn < 2; ++sgn) {
         const float t = sgn == 0 ? fmaf(tC[j], s, tb) : fmaf(-tC[j], s, tb);
-        float3 v = table_lerp(tab_a, nu_max, t);
+        float3 v = table_lerp(tab_a, t);
         float x1 = 1.0f;  // order 2: the Rayleigh phase factor 1 + nu1^2
         if (kOrder2) {
           // Order-1 incident radiance: Rayleigh and Mie single scattering
           // times their phase functions of nu1 (constants folded in tables).
-          const float nu1 = fmaf(t, inv_hn, -1.0f);
-          const float3 vm = table_lerp(tab_b, nu_max, t);
+          const float nu1 = fmaf(t, inv_hn, nu1_0);
+          const float3 vm = table_lerp(tab_b, t);
           const float r1 = rsq_f(fmaf(mg2, nu1, g1));
           const float q = r1 * r1 * r1;
           x1 = fmaf(nu1, nu1, 1.0f);
@@ -516,7 +536,7 @@ __device__ __forceinline__ void sd_theta(
             v.z = x1 * v.z;
             x1 = 1.0f;
           }
-          const float3 e = table_lerp(etab, e_max, fmaf(eB, t, eA));
+          const float3 e = table_lerp(etab, fmaf(eB, t, eA));
           v.x = fmaf(th.gf[0], e.x, v.x);
           v.y = fmaf(th.gf[1], e.y, v.y);
           v.z = fmaf(th.gf[2], e.z, v.z);
@@ -543,8 +563,12 @@ __device__ __forceinline__ void sd_theta(
   }
 }
 
+// Five waves per SIMD: the 4-output shapes then fit 96 VGPRs (101 unbounded,
+// four waves), and the pass, which issues a VALU instruction in about 90% of
+// the SIMD cycles, runs 2-3% faster on MI355X.
 template <int kChunk, int kB>
-__global__ __launch_bounds__(kB) void k_scattering_density(
+__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(5))) void
+k_scattering_density(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
     int njobs, int order) {
   extern __shared__ float4 lds[];
@@ -563,9 +587,12 @@ __global__ __launch_bounds__(kB) void k_scattering_density(
   const int imus = pl % nmus, k = pl / nmus;
   const int W = nnu * nmus;
   const bool two = order == 2;
+  // Per theta a table of NU + 1 entries (table_lerp); then the
+  // ground-irradiance table of E_W + 1 entries.
+  const int stride = 2 * (nnu + 1);
   float4* tab_a = lds;
-  float4* tab_b = lds + 2 * kTheta * nnu;
-  float4* etab = lds + (two ? 4 : 2) * kTheta * nnu;
+  float4* tab_b = lds + kTheta * stride;
+  float4* etab = lds + (two ? 2 : 1) * kTheta * stride;
   const int tid = threadIdx.x;
 
   double r, rho;
@@ -623,8 +650,9 @@ __global__ __launch_bounds__(kB) void k_scattering_density(
                            (wy0 * a0.z + wy * a1.z) * fa);
     float3 vn = make_float3((wy0 * b0.x + wy * b1.x) * fa, (wy0 * b0.y + wy * b1.y) * fa,
                             (wy0 * b0.z + wy * b1.z) * fa);
-    tab_a[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
-    tab_a[2 * e + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
+    float4* ta = tab_a + l * stride;
+    put_entry(ta, n + 1, v, vn);
+    if (n == 0) put_entry(ta, 0, v, v);
     if (two) {
       const float fb = s_dw[l] * kM;
       src = J.dM;
@@ -636,17 +664,18 @@ __global__ __launch_bounds__(kB) void k_scattering_density(
                       (wy0 * a0.z + wy * a1.z) * fb);
       vn = make_float3((wy0 * b0.x + wy * b1.x) * fb, (wy0 * b0.y + wy * b1.y) * fb,
                        (wy0 * b0.z + wy * b1.z) * fb);
-      tab_b[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
-      tab_b[2 * e + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
+      float4* tb = tab_b + l * stride;
+      put_entry(tb, n + 1, v, vn);
+      if (n == 0) put_entry(tb, 0, v, v);
     }
   }
   // Row 0 of delta irradiance (r = rb): the ground-irradiance lookup of the
   // reference at bottom_radius reads exactly this row.
   for (int e = tid; e < J.ew; e += kB) {
-    const float4 v = gload(J.dE, e);
-    const float4 vn = gload(J.dE, e + 1 < J.ew ? e + 1 : e);
-    etab[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
-    etab[2 * e + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
+    const float3 v = make3(gload(J.dE, e));
+    const float3 vn = make3(gload(J.dE, e + 1 < J.ew ? e + 1 : e));
+    put_entry(etab, e + 1, v, vn);
+    if (e == 0) put_entry(etab, 0, v, v);
   }
   __syncthreads();
 
@@ -698,8 +727,8 @@ __global__ __launch_bounds__(kB) void k_scattering_density(
       Bv[jn] = fsx[jn] * t.sin_t;
       Cv[jn] = fsy[jn] * t.sin_t;
     }
-    const float4* ta = tab_a + 2 * l * nnu;
-    const float4* tb = tab_b + 2 * l * nnu;
+    const float4* ta = tab_a + l * stride;
+    const float4* tb = tab_b + l * stride;
     const bool g = __builtin_amdgcn_readfirstlane(t.ground) != 0;
     if (two) {
       if (g)
@@ -732,7 +761,8 @@ __global__ __launch_bounds__(kB) void k_scattering_density(
 // builds, for each of the 16 zenith angles, the nu table of the scattering at
 // (r, mu = cos theta, mu_s) (trilinear in r, mu, mu_s), then each lane takes
 // one of the 64 azimuths, sums over theta, and the wave reduces. One wave per
-// block, so the tables of a texel need at most 4 * 16 * NU float4 of LDS.
+// block, so the tables of a texel need at most 4 * 16 * (NU + 1) float4 of
+// LDS (table_lerp's layout).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_indirect_irradiance(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
@@ -744,7 +774,8 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
   const int texel = blockIdx.x - first_block[ji];
   const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
   const bool one = order == 1;
-  const int per_table = 2 * kTheta * nnu;
+  const int stride = 2 * (nnu + 1);
+  const int per_table = kTheta * stride;
   float4* tab_a = lds;
   float4* tab_b = tab_a + per_table;
   const bool active = texel < J.ew * J.eh;
@@ -791,9 +822,9 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
         vn.x *= fk;
         vn.y *= fk;
         vn.z *= fk;
-        float4* tab = which == 0 ? tab_a : tab_b;
-        tab[2 * e] = make_float4(v.x, v.y, v.z, vn.x - v.x);
-        tab[2 * e + 1] = make_float4(vn.y - v.y, vn.z - v.z, 0.f, 0.f);
+        float4* tab = (which == 0 ? tab_a : tab_b) + jt * stride;
+        put_entry(tab, n + 1, v, vn);
+        if (n == 0) put_entry(tab, 0, v, v);
       }
     }
   }
@@ -804,6 +835,7 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
   const float fmus = static_cast<float>(mu_s);
   const float nu_max = static_cast<float>(nnu - 1);
   const float hn = 0.5f * nu_max;
+  const float hn1 = hn + 1.0f;  // table_lerp reads at t + 1
   const float g1 = static_cast<float>(1.0 + J.g * J.g);
   const float mg2 = static_cast<float>(-2.0 * J.g);
   float a0 = 0.f, a1 = 0.f, a2 = 0.f;
@@ -811,9 +843,9 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
     const double theta = (jt + 0.5) * dtheta;
     const float ct = static_cast<float>(cos(theta)), st = static_cast<float>(sin(theta));
     const float nu = fmaf(st, wsx, ct * fmus);  // omega . omega_s
-    float3 v = table_lerp(tab_a + 2 * jt * nnu, nu_max, fmaf(nu, hn, hn));
+    float3 v = table_lerp(tab_a + jt * stride, fmaf(nu, hn, hn1));
     if (one) {
-      const float3 vm = table_lerp(tab_b + 2 * jt * nnu, nu_max, fmaf(nu, hn, hn));
+      const float3 vm = table_lerp(tab_b + jt * stride, fmaf(nu, hn, hn1));
       const float r1 = rsq_f(fmaf(mg2, nu, g1));
       const float q = r1 * r1 * r1;
       const float x1 = fmaf(nu, nu, 1.0f);
@@ -870,14 +902,14 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
 // blocks that run together share mu and step in r. The texels and their
 // values are the same either way; the order only changes which source rows
 // of the scattering density neighbouring blocks read.
-__device__ inline void ms_row(const Job& J, int p, int* z, int* y) {
-  if (J.ms_mu_major) {
-    *y = p / J.nr;
-    *z = p - *y * J.nr;
-  } else {
-    *z = p / J.nmu;
-    *y = p - *z * J.nmu;
-  }
+// Returns (z, y). One division, and selects of values: writing the two
+// results through pointers in two branches compiles to stores through a
+// selected address, which puts them in scratch memory.
+__device__ inline int2 ms_row(const Job& J, int p) {
+  const bool mm = J.ms_mu_major != 0;
+  const int d = mm ? J.nr : J.nmu;
+  const int q = p / d, rem = p - q * d;
+  return mm ? make_int2(rem, q) : make_int2(q, rem);
 }
 
 struct MsRowSample {
@@ -919,8 +951,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     const int slot = e / kMarch, i = e % kMarch;
     const int row = row0 + slot;
     if (row >= nrows) continue;
-    int y, z;
-    ms_row(J, row, &z, &y);
+    const int2 zy = ms_row(J, row);
+    const int z = zy.x, y = zy.y;
     double r, rho, mu;
     bool ground;
     s_layer_r(J, z, &r, &rho);
@@ -967,8 +999,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     ws_of[j] = 0.0f;
     nu_of[j] = rmus_of[j] = 0.0;
     if (slot_of[j] < 0) continue;
-    int y, z;
-    ms_row(J, row0 + slot, &z, &y);
+    const int2 zy = ms_row(J, row0 + slot);
+    const int z = zy.x, y = zy.y;
     double r, rho, mu;
     bool ground;
     s_layer_r(J, z, &r, &rho);
@@ -1049,9 +1081,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
       const float aa = (dd - dmin0) * inv_range0;
       const float xm = fmaxf(fmaf(-aa, invA, 1.0f), 0.0f) * rcp_f(1.0f + aa);
       const float xin = clamp_index_coord(xm * xmax, xmax);
-      const float fx = floorf(xin);
-      const float wx = xin - fx;
-      const int i0 = static_cast<int>(fx);
+      float wx;
+      const int i0 = split_coord(xin, &wx);
       const int i1 = min(i0 + 1, nmus - 1);
       const float4 p = B[c0_of[j] + i0], q = B[c0_of[j] + i1];
       const float4 u = B[c1_of[j] + i0], v = B[c1_of[j] + i1];
@@ -1068,8 +1099,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
 #pragma unroll
   for (int j = 0; j < kTpt; ++j) {
     if (slot_of[j] < 0) continue;
-    int y, z;
-    ms_row(J, row0 + slot_of[j], &z, &y);
+    const int2 zy = ms_row(J, row0 + slot_of[j]);
+    const int z = zy.x, y = zy.y;
     const int64_t t_idx = (static_cast<int64_t>(z) * nmu + y) * W + x_of[j];
     gstore(J.dR, t_idx, make_float4(acc[j][0], acc[j][1], acc[j][2], 0.0f));
     const double nu = nu_of[j];
@@ -1115,8 +1146,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
   if (W <= kMsMaxRow) return;  // LDS-path job
   const int tiles = (W + kBlock - 1) / kBlock;
   const int tile = lb % tiles;
-  int y, z;
-  ms_row(J, lb / tiles, &z, &y);
+  const int2 zy = ms_row(J, lb / tiles);
+  const int z = zy.x, y = zy.y;
   const int tid = threadIdx.x;
 
   double r, rho, mu;
@@ -1182,9 +1213,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
     const float aa = (dd - dmin0) * inv_range0;
     const float xm = fmaxf(fmaf(-aa, invA, 1.0f), 0.0f) * rcp_f(1.0f + aa);
     const float xin = clamp_index_coord(xm * xmax, xmax);
-    const float fx = floorf(xin);
-    const float wx = xin - fx;
-    const int i0 = static_cast<int>(fx);
+    float wx;
+    const int i0 = split_coord(xin, &wx);
     const int i1 = min(i0 + 1, nmus - 1);
     const float wz1 = S.wz, wz0 = 1.0f - S.wz, wy1 = S.wy, wy0 = 1.0f - S.wy;
     const float wrow[4] = {wz0 * wy0, wz0 * wy1, wz1 * wy0, wz1 * wy1};
@@ -1468,9 +1498,10 @@ void launch_sd_tile(const PassLaunch& p, int order, size_t lds, hipStream_t s) {
 
 hipError_t launch_scattering_density(const PassLaunch& p, int order,
                                      hipStream_t s) {
+  // Tables of NU + 1 and E_W + 1 entries of two float4 (table_lerp).
   const size_t lds = sizeof(float4) *
-                     (static_cast<size_t>((order == 2 ? 4 : 2) * kTheta * p.max_nnu) +
-                      2 * static_cast<size_t>(p.max_ew));
+                     (static_cast<size_t>((order == 2 ? 4 : 2) * kTheta * (p.max_nnu + 1)) +
+                      2 * static_cast<size_t>(p.max_ew + 1));
   static_assert(kSdTiles == 6, "one case per tile shape");
   switch (p.tile) {
     case 0: launch_sd_tile<kSdTile[0].chunk, kSdTile[0].block>(p, order, lds, s); break;
@@ -1486,7 +1517,7 @@ hipError_t launch_scattering_density(const PassLaunch& p, int order,
 hipError_t launch_indirect_irradiance(const PassLaunch& p, int order,
                                       hipStream_t s) {
   const size_t lds =
-      sizeof(float4) * static_cast<size_t>((order == 1 ? 4 : 2) * kTheta * p.max_nnu);
+      sizeof(float4) * static_cast<size_t>((order == 1 ? 4 : 2) * kTheta * (p.max_nnu + 1));
   hipLaunchKernelGGL(k_indirect_irradiance, dim3(p.total_blocks), dim3(64), lds, s,
                      p.jobs, p.first_block, p.njobs, order);
   return hipGetLastError();
