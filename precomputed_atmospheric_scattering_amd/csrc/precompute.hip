@@ -278,9 +278,10 @@ __device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<double>
   const double dd = sq - tt;
   double ut = (dd - S.dmin) * S.inv_range * (J.tw - 1);
   ut = fmin(fmax(ut, 0.0), static_cast<double>(J.tw - 1));
-  const double fl = floor(ut);
-  const double w = ut - fl;
-  const int i0 = static_cast<int>(fl);
+  // ut >= 0: truncation is floor, and v_fract_f64 gives ut - floor(ut)
+  // exactly (as split_coord in FP32).
+  const double w = __builtin_amdgcn_fract(ut);
+  const int i0 = static_cast<int>(ut);
   const int i1 = min(i0 + 1, J.tw - 1);
   const float4* __restrict__ T = J.T;
   const uint32_t r0 = static_cast<uint32_t>(S.t0), r1 = static_cast<uint32_t>(S.t1);
@@ -377,10 +378,10 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
     const SsSample<Real>& S = smp[s];
     const double t = fma(S.d, nu, rmus);  // r * mu_s + d * nu = r_d * mu_s_d
     const Real xs = static_cast<Real>((t + S.rho) * S.inv_r);  // mu_s_d - cos(theta_h)
-    Real mus = static_cast<Real>(t * S.inv_r);
-    mus = mus < Real(-1) ? Real(-1) : (mus > Real(1) ? Real(1) : mus);
-    Real u = (xs + S.e) * S.inv_span;
-    u = u < Real(0) ? Real(0) : (u > Real(1) ? Real(1) : u);
+    // Clamps as max/min, 2 instructions each (the operands are finite; the
+    // ternary form, which also keeps NaNs, compiles to compares and selects).
+    const Real mus = fmin(fmax(static_cast<Real>(t * S.inv_r), Real(-1)), Real(1));
+    const Real u = fmin(fmax((xs + S.e) * S.inv_span, Real(0)), Real(1));
     const Real vis = u * u * fma(Real(-2), u, Real(3));
     Real ts[3];
     tsun_lookup(J, S, mus, ts);
@@ -897,6 +898,35 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
 // blocks and each returns at once on the other's jobs, so the path of a job
 // depends on its own row length only.
 // ---------------------------------------------------------------------------
+// The mu_s texel coordinate of the scattering LUT (functions.glsl:814-827)
+// in FP32, clamped to [0, MU_S - 1]: with d the distance to the top boundary
+// from the ground at mu_s, a = (d - d_min) / (d_max - d_min) and
+// x = max(1 - a / A, 0) / (1 + a). d = N / D, with (N, D) = (H^2, sq + t)
+// for t = r_b mu_s > 0 (cancellation-free) and (sq - t, 1) otherwise, so
+// multiplying the fraction by D > 0 gives x = max(D - u / A, 0) / (D + u)
+// with u = a D = (N - d_min D) / (d_max - d_min): one reciprocal instead of
+// two.
+struct MusMap {
+  float rb, H2, dmin, inv_range, invA, xmax;
+  __device__ explicit MusMap(const Job& J)
+      : rb(static_cast<float>(J.rb)),
+        H2(static_cast<float>(J.H * J.H)),
+        dmin(static_cast<float>(J.rt - J.rb)),
+        inv_range(static_cast<float>(1.0 / (J.H - (J.rt - J.rb)))),
+        invA(static_cast<float>(1.0 / J.A)),
+        xmax(static_cast<float>(J.nmus - 1)) {}
+  __device__ __forceinline__ float texel(float mus) const {
+    const float t = rb * mus;
+    const float sq = sqrt_f(fmaf(t, t, H2));
+    const bool up = t > 0.0f;
+    const float N = up ? H2 : sq - t;
+    const float D = up ? sq + t : 1.0f;
+    const float u = fmaf(-dmin, D, N) * inv_range;
+    const float x = fmaxf(fmaf(-u, invA, D), 0.0f) * rcp_f(D + u);
+    return clamp_index_coord(x * xmax, xmax);
+  }
+};
+
 // Packed row p of a job's block order: r-major (p = z * MU + y, the LUT's
 // own order) or, with ms_mu_major, mu-major (p = y * R + z), where the
 // blocks that run together share mu and step in r. The texels and their
@@ -1024,13 +1054,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   }
   __syncthreads();
 
-  // mu_s mapping constants (functions.glsl:814-827).
-  const float rbf = static_cast<float>(J.rb);
-  const float H2 = static_cast<float>(J.H * J.H);
-  const float dmin0 = static_cast<float>(J.rt - J.rb);
-  const float inv_range0 = static_cast<float>(1.0 / (J.H - (J.rt - J.rb)));
-  const float invA = static_cast<float>(1.0 / J.A);
-  const float xmax = static_cast<float>(nmus - 1);
+  const MusMap mus_map(J);
   const float4* __restrict__ D = J.dJ;
   // Source texels of the next sample, loaded one sample ahead so that the
   // loads are in flight during the barrier and the gather of the current one.
@@ -1075,12 +1099,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
       // mu_s at the sample, (r mu_s + d nu) / r_i (functions.glsl:1311).
       const float mus =
           clampf(static_cast<float>(fma(S.d, nu_of[j], rmus_of[j]) * S.inv_r), -1.0f, 1.0f);
-      const float tt = rbf * mus;
-      const float sq = sqrt_f(fmaf(tt, tt, H2));
-      const float dd = tt > 0.0f ? H2 * rcp_f(sq + tt) : sq - tt;
-      const float aa = (dd - dmin0) * inv_range0;
-      const float xm = fmaxf(fmaf(-aa, invA, 1.0f), 0.0f) * rcp_f(1.0f + aa);
-      const float xin = clamp_index_coord(xm * xmax, xmax);
+      const float xin = mus_map.texel(mus);
       float wx;
       const int i0 = split_coord(xin, &wx);
       const int i1 = min(i0 + 1, nmus - 1);
@@ -1195,24 +1214,13 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
   s1 = s1 < 0 ? 0 : (s1 > nnu - 1 ? nnu - 1 : s1);
   const int cs0 = s0 * nmus, cs1 = s1 * nmus;
   const double rmus = r * mu_s;
-  // mu_s mapping constants (functions.glsl:814-827).
-  const float rbf = static_cast<float>(J.rb);
-  const float H2 = static_cast<float>(J.H * J.H);
-  const float dmin0 = static_cast<float>(J.rt - J.rb);
-  const float inv_range0 = static_cast<float>(1.0 / (J.H - (J.rt - J.rb)));
-  const float invA = static_cast<float>(1.0 / J.A);
-  const float xmax = static_cast<float>(nmus - 1);
+  const MusMap mus_map(J);
   const float4* __restrict__ D = J.dJ;
   float a0 = 0.f, a1 = 0.f, a2 = 0.f;
   for (int s = 0; s < kMarch; ++s) {
     const MsSample& S = smp[s];
     const float mus = clampf(static_cast<float>(fma(S.d, nu, rmus) * S.inv_r), -1.0f, 1.0f);
-    const float tt = rbf * mus;
-    const float sq = sqrt_f(fmaf(tt, tt, H2));
-    const float dd = tt > 0.0f ? H2 * rcp_f(sq + tt) : sq - tt;
-    const float aa = (dd - dmin0) * inv_range0;
-    const float xm = fmaxf(fmaf(-aa, invA, 1.0f), 0.0f) * rcp_f(1.0f + aa);
-    const float xin = clamp_index_coord(xm * xmax, xmax);
+    const float xin = mus_map.texel(mus);
     float wx;
     const int i0 = split_coord(xin, &wx);
     const int i1 = min(i0 + 1, nmus - 1);
