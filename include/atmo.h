@@ -79,8 +79,11 @@ typedef struct atmo_model_params {
 
 /* LUT resolution. The reference fixes these at compile time
  * (atmosphere/constants.h:47-61); here they are per model. NULL selects the
- * reference values: 256x64, (32, 128, 32, 8), 64x16. scattering_mu must be
- * even and every size >= 2. */
+ * reference values: 256x64, (32, 128, 32, 8), 64x16. Every size is in
+ * [2, 4096], scattering_mu is even and >= 4 (two halves of at least 2
+ * texels), scattering_nu <= 32, irradiance_width
+ * <= 512 (the density pass keeps nu tables and an irradiance row in LDS),
+ * and R x MU x MU_S x NU <= 2^28. */
 typedef struct atmo_lut_sizes {
   int transmittance_width;
   int transmittance_height;

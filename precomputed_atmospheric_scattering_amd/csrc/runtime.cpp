@@ -497,12 +497,17 @@ atmo_status validate_sizes(const atmo_lut_sizes& z) {
   for (int x : v)
     if (x < 2 || x > 4096)
       return fail(ATMO_ERROR_INVALID_ARGUMENT, "LUT sizes must be in [2, 4096]");
-  if (z.scattering_mu % 2 != 0)
-    return fail(ATMO_ERROR_INVALID_ARGUMENT, "scattering_mu must be even");
+  // Each half of the mu axis (ground and sky rays) is a texture axis of MU / 2
+  // texels whose coordinate mapping divides by 1 - 2 / MU
+  // (functions.glsl:346-349, :859-860): at least 2 texels per half.
+  if (z.scattering_mu % 2 != 0 || z.scattering_mu < 4)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "scattering_mu must be even and >= 4");
   if (z.scattering_nu > 32)
     return fail(ATMO_ERROR_INVALID_ARGUMENT, "scattering_nu must be <= 32");
-  if (z.irradiance_width > 4096)
-    return fail(ATMO_ERROR_INVALID_ARGUMENT, "irradiance_width too large");
+  // The scattering-density pass keeps a row of the irradiance LUT in LDS
+  // next to its nu tables: at NU = 32 and E_W = 512 a block needs 50 KB.
+  if (z.irradiance_width > 512)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "irradiance_width must be <= 512");
   const double texels = static_cast<double>(z.scattering_r) * z.scattering_mu *
                         z.scattering_mu_s * z.scattering_nu;
   // At most 2^28 texels (4 GiB in FP32 RGBA): kernels index texels with

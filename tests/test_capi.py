@@ -68,8 +68,9 @@ def test_invalid_parameters_rejected(bad):
 
 
 @pytest.mark.parametrize("sizes", [
-    dict(scattering_mu=127), dict(scattering_nu=1), dict(scattering_nu=64),
-    dict(irradiance_height=0),
+    dict(scattering_mu=127), dict(scattering_mu=2), dict(scattering_nu=1),
+    dict(scattering_nu=64),
+    dict(irradiance_height=0), dict(irradiance_width=513),
 ])
 def test_invalid_sizes_rejected(sizes):
     z = dict(atmo.DEFAULT_SIZES)

@@ -152,3 +152,26 @@ def test_long_rows_multiple_scattering_both_row_orders():
         atmo.reset_tiles()
     for k in out[False]:
         assert np.array_equal(out[False][k], out[True][k]), k
+
+
+# The ends of the table sizes of the scattering-density and indirect-
+# irradiance passes: 32 nu slices and a 512-texel irradiance row (the
+# largest LDS tables), and every LUT dimension at its minimum (2; 4 for
+# MU, two halves of 2).
+EDGE_SIZES = {
+    "largest tables": dict(transmittance_width=32, transmittance_height=8, scattering_r=2,
+                           scattering_mu=8, scattering_mu_s=4, scattering_nu=32,
+                           irradiance_width=512, irradiance_height=4),
+    "smallest": dict(transmittance_width=2, transmittance_height=2, scattering_r=2,
+                     scattering_mu=4, scattering_mu_s=2, scattering_nu=2,
+                     irradiance_width=2, irradiance_height=2),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE_SIZES))
+def test_table_size_limits(name):
+    print("case", name, flush=True)
+    cfg = configs.earth(lut_sizes=EDGE_SIZES[name])
+    errs = parity.compare_init(cfg, 3)
+    for tex, e in errs.items():
+        assert e <= parity.TOL_FP32, (name, tex, e)
