@@ -1062,9 +1062,11 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   auto fetch = [&](int i) {
 #pragma unroll
     for (int j = 0; j < kTpt; ++j) {
-      if (slot_of[j] < 0) continue;
-      const MsRowSample& S = smp[slot_of[j]][i];
-      const int x = x_of[j];
+      // Idle lanes (slot -1) load row slot 0's texel 0, in bounds, so that
+      // the loads need no branch.
+      const bool on = slot_of[j] >= 0;
+      const MsRowSample& S = smp[on ? slot_of[j] : 0][i];
+      const int x = on ? x_of[j] : 0;
       src[j][0] = make3(gload32(D, static_cast<uint32_t>(S.o00 + x)));
       src[j][1] = make3(gload32(D, static_cast<uint32_t>(S.o01 + x)));
       src[j][2] = make3(gload32(D, static_cast<uint32_t>(S.o10 + x)));
