@@ -679,6 +679,22 @@ def main():
         for d in list(sky.values()) + (list(hd.values()) if hd else []) + hout:
             d.free()
 
+    # The same atmosphere with FP16 LUT storage (FP32 accumulation, then the
+    # per-channel maxima and the FP16 conversion), after the headline steps
+    # and outside them: Init time only.
+    kw16, _ = configs.model_kwargs(configs.spectral(half_precision=True))
+    m16 = atmo.Model(**kw16)
+    for _ in range(args.warmup):
+        m16.init(orders, stream)
+    stream.synchronize()
+    half_ms = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        m16.init(orders, stream)
+        stream.synchronize()
+        half_ms.append((time.perf_counter() - t0) * 1e3)
+    m16.close()
+
     steps = args.steps
     sd_avg_ms = sd_ms / sd_launches
     sd_traffic, traffic_src = profiled_traffic("k_scattering_density")
@@ -730,6 +746,8 @@ def main():
             "lookups_per_frame": {str(v): n for v, n in frame_rays.items()},
         },
         "pass_ms_per_atmosphere": {k: round(v / steps, 4) for k, v in pass_ms.items()},
+        "fp16_storage": {"ms_per_atmosphere": round(sum(half_ms) / len(half_ms), 4),
+                         "precompute_p99_ms": round(percentile(half_ms, 0.99), 4)},
         "tiles": {"scattering_density": "%d nu x %d threads" % (
                       tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
                   "multiple_scattering": "%d texels per thread, %s rows" % (
