@@ -204,9 +204,9 @@ void atmo_batch_destroy(atmo_batch* batch);
  * density pass and (R, MU, MU_S, NU) for multiple scattering, in a table
  * shared by the process; a model's Init, batch or serving request reads it
  * when its job table is built. Classes never tuned or set use the defaults
- * (4 nu per thread, 256 threads; the fewest texels per thread that hold a
- * row, mu-major rows once the R x MU x MU_S x NU float4 LUT reaches
- * 64 MiB). */
+ * (4 nu per thread in blocks of 128 threads when MU x ceil(NU / 4) <= 256,
+ * else 256; the fewest texels per thread that hold a row; mu-major rows
+ * once the R x MU x MU_S x NU float4 LUT reaches 64 MiB). */
 typedef struct atmo_tiles {
   int sd_nu_per_thread;
   int sd_block_threads;

@@ -239,7 +239,7 @@ atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStrea
       ATMO_TRY(time_pass(kSD, 3, tile, 0, &t3));
       t[tile] = t2 + 2.0 * t3;
     }
-    *sd_out = pick(t, 0);
+    *sd_out = pick(t, atmo::sd_default_tile(z.nmu, z.nnu));
   }
   // Candidates 2 * tile + mu_major. Rows too long for LDS take the gather
   // kernel whatever the tile: only the row order is timed for them.
@@ -269,7 +269,7 @@ Tiles tiles_for(const Sizes& z) {
   MsShape ms = ms_default(z);
   const auto b = g_ms.find(ms_key(z));
   if (b != g_ms.end() && ms_tile_fits(z, b->second.tile)) ms = b->second;
-  Tiles t{0, ms.tile, ms.mu_major};
+  Tiles t{atmo::sd_default_tile(z.nmu, z.nnu), ms.tile, ms.mu_major};
   const auto a = g_sd.find(sd_key(z));
   if (a != g_sd.end()) t.sd = a->second;
   return t;
@@ -359,7 +359,7 @@ atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
     sd_done[sd_key(z)] = true;
     ms_done[ms_key(z)] = true;
     if (!do_sd && !do_ms) continue;
-    int sd = 0;
+    int sd = atmo::sd_default_tile(z.nmu, z.nnu);
     MsShape ms = ms_default(z);
     ATMO_TRY(tune_size(sizes[i], do_sd, do_ms, s, &sd, &ms));
     std::lock_guard<std::mutex> lock(g_tiles_mu);

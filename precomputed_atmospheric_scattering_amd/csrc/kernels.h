@@ -45,6 +45,15 @@ struct SdTile {
 constexpr int kSdTiles = 6;
 constexpr SdTile kSdTile[kSdTiles] = {{4, 256}, {2, 256}, {1, 256},
                                       {4, 128}, {2, 128}, {4, 64}};
+// Default shape: 4 nu per thread, in blocks of 128 threads when a plane has
+// at most 256 threads (MU x ceil(NU / 4)), else 256. Measured on MI355X
+// with the other shapes: at MU x NU = 128 x 8 (a 256-thread plane) the
+// 128-thread blocks are 4% faster at every order; at 256 x 16 (1024) the
+// 256-thread blocks are, the blocks then rebuilding a plane's tables from
+// a LUT beyond the caches 4 times instead of 8.
+ATMO_HD int sd_default_tile(int nmu, int nnu) {
+  return nmu * ((nnu + 3) / 4) <= 256 ? 3 : 0;
+}
 
 // Multiple scattering: packed rows of at most kMsMaxRow texels are processed
 // through LDS by blocks of kBlock threads, each thread holding kMsTpt[tile]

@@ -19,14 +19,18 @@ def clean_table():
 
 def test_candidates():
     sd, ms = atmo.tile_candidates()
-    assert sd[0] == (4, 256)  # the default shape comes first
+    assert sd[0] == (4, 256) and (4, 128) in sd
     assert len(set(sd)) == len(sd) and all(b % 64 == 0 for _, b in sd)
     assert ms == [1, 2, 4]
 
 
 def test_defaults_follow_row_length():
     t = atmo.get_tiles()
-    assert (t["sd_nu_per_thread"], t["sd_block_threads"]) == (4, 256)
+    # 4 nu per thread; 128-thread blocks up to 256 threads per (r, mu_s)
+    # plane (MU x ceil(NU / 4) = 128 x 2 here), 256 above.
+    assert (t["sd_nu_per_thread"], t["sd_block_threads"]) == (4, 128)
+    big = atmo.get_tiles(dict(scattering_mu=256, scattering_nu=16))
+    assert (big["sd_nu_per_thread"], big["sd_block_threads"]) == (4, 256)
     assert t["ms_texels_per_thread"] == 1  # 8 x 32 = 256 texels per row
     assert not t["sd_tuned"] and not t["ms_tuned"]
     assert atmo.get_tiles(dict(scattering_mu_s=64))["ms_texels_per_thread"] == 2
