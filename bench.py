@@ -70,7 +70,7 @@ PROFILE_TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
 
 
 def profiled_traffic(kernel):
-    """HBM bytes per dispatch of a kernel of the headline workload, from the
+    """HBM bytes per dispatch of a kernel of the spectral workload, from the
     FETCH_SIZE and WRITE_SIZE passes summarized in the committed profile
     (calibrated against a copy of known size): (bytes, source) or (None,
     None) if the profile has no row for it."""
@@ -525,7 +525,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--workload", default="spectral",
                     choices=("spectral", "batched", "serving", "high-resolution"),
-                    help="spectral (default): the headline line; batched: 64 seeded "
+                    help="spectral (default): the spectral atmosphere's line; batched: 64 seeded "
                          "random atmospheres, batched vs one at a time; serving: Poisson "
                          "precompute requests + query batches through the serving loop; "
                          "high-resolution: 64x256x64x16 LUT, FP16, 8 orders")
@@ -546,7 +546,7 @@ def main():
                          "HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
@@ -680,9 +680,9 @@ def main():
             d.free()
 
     # The same atmosphere with FP16 LUT storage (FP32 accumulation, then the
-    # per-channel maxima and the FP16 conversion), after the headline steps
-    # and outside them: Init time only, with the temporaries kept between
-    # calls as for the headline model.
+    # per-channel maxima and the FP16 conversion), after the timed steps and
+    # outside them: Init time only, with the temporaries kept between calls
+    # as for the FP32 model above.
     kw16, _ = configs.model_kwargs(configs.spectral(half_precision=True))
     m16 = atmo.Model(retain_intermediates=True, **kw16)
     for _ in range(args.warmup):

@@ -1,6 +1,7 @@
 // FP64 CPU oracle for the atmosphere LUT precomputation and sky lookups.
 //
-// TEST INFRASTRUCTURE ONLY: the checker the HIP product is compared against.
+// TEST INFRASTRUCTURE ONLY: the reference results the HIP product is
+// compared against.
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
 // it. See oracle.h for the scope, layout and conventions.
 //
@@ -99,7 +100,8 @@ int ThreadCount() {
   if (g_threads > 0) return g_threads;
   const char* env = std::getenv("ORACLE_THREADS");
   if (env != nullptr && std::atoi(env) > 0) return std::atoi(env);
-  // Default: the host's threads, at most 16 (a GPU box's CPU share).
+  // Default: the host's threads, at most 16, so that a run shares its host
+  // with the GPU work beside it.
   unsigned hc = std::thread::hardware_concurrency();
   return hc == 0 ? 1 : static_cast<int>(hc < 16 ? hc : 16);
 }

@@ -1,8 +1,8 @@
 /*
  * FP64 CPU oracle for the atmosphere LUT precomputation and sky lookups.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker that the HIP product
- * is compared against; it is loaded by tests/, by __graft_entry__.smoke() and
+ * TEST INFRASTRUCTURE ONLY. This library gives the reference results that the
+ * HIP product is compared against; it is loaded by tests/, by __graft_entry__.smoke() and
  * by the cpu_baseline leg of bench.py, and by nothing else. It is never on the
  * product path.
  *

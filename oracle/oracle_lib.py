@@ -1,6 +1,7 @@
 """ctypes bindings of the FP64 oracle (oracle/liboracle.so).
 
-Test infrastructure only: the oracle is the checker. Only tests/,
+Test infrastructure only: the oracle gives the reference results the
+product is compared against. Only tests/,
 __graft_entry__.smoke() and the cpu_baseline leg of bench.py load it; the
 product (precomputed_atmospheric_scattering_amd) never does.
 """

@@ -4,7 +4,7 @@ the full Init chain, by facts that hold independently of the implementation:
 published values of the CIE and sRGB tables, identities between functions,
 symmetries and linearity of the radiative transfer, and closed forms for
 degenerate atmospheres. The reference holds no golden vectors for these
-(SURVEY.md §8(c)); each check below is chosen so that a wrong term (a missing
+(its model tests need an OpenGL context); each check below is chosen so that a wrong term (a missing
 blend, a wrong iteration wavelength, a transposed matrix, a dropped factor)
 changes the result."""
 import ctypes as C
