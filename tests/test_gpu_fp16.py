@@ -75,3 +75,19 @@ def test_fp16_batch_equals_alone():
     for m, ref in zip(models, alone):
         assert np.array_equal(m.lut("scattering"), ref)
         m.close()
+
+
+@pytest.mark.parametrize("spectral,half", [(True, False), (True, True), (False, True)])
+def test_combined_textures_with_other_options(spectral, half):
+    """COMBINED_SCATTERING_TEXTURES (the Mie red channel in the alpha of the
+    scattering LUT, no single-Mie LUT) together with illuminance mode (FP64
+    accumulators over 5 iterations) and with FP16 storage."""
+    make = configs.spectral if spectral else configs.earth
+    cfg = make(lut_sizes=parity.SMALL_SIZES, half_precision=half)
+    cfg["combine_scattering_textures"] = True
+    print("case", "spectral" if spectral else "earth", "fp16" if half else "fp32", flush=True)
+    errs = parity.compare_init(cfg, 3)
+    assert "single_mie_scattering" not in errs
+    for name, e in errs.items():
+        tol = parity.TOL_FP16 if half and name in HALF else parity.TOL_FP32
+        assert e <= tol, (name, e)
