@@ -237,6 +237,42 @@ def test_illuminance_mode_has_no_radiance_api():
     m.close()
 
 
+def test_illuminance_mode_lookups_match_oracle():
+    """The lookups of the spectral workload: GetSkyLuminance,
+    GetSkyLuminanceToPoint and GetSunAndSkyIlluminance on a 15-wavelength
+    model, whose LUTs hold luminance. Each equals the radiance lookup
+    arithmetic on those LUTs at 680/550/440 nm times the luminance factors
+    (MAX_LUMINOUS_EFFICACY for the sky in this mode, model.cc:735-742);
+    the oracle evaluates that arithmetic in FP64 on the GPU's own LUTs."""
+    cfg = configs.spectral(lut_sizes=parity.MID_SIZES)
+    kw, _ = configs.model_kwargs(cfg)
+    m = atmo.Model(**kw)
+    m.init(3)
+    spec, s, _ = parity.oracle_spec(cfg)
+    a = ol.lower(spec, (680.0, 550.0, 440.0))
+    luts = {k: f64(m.lut(k)) for k in ("transmittance", "scattering",
+                                       "single_mie_scattering", "irradiance")}
+    rb, rt = kw["bottom_radius"] / 1000.0, kw["top_radius"] / 1000.0
+    q = configs.sky_queries(17, 4096, rb, rt)
+    sky_k, sun_k = (np.asarray(k) for k in m.luminance_factors())
+    lum, tr = m.sky_luminance(q["camera"], q["view_ray"], q["shadow_length"],
+                              q["sun_direction"])
+    r, t = oracle_sky(a, s, luts, q)
+    e1 = (err(lum, r * sky_k), err(tr, t))
+    lum, tr = m.sky_luminance_to_point(q["camera"], q["point"], q["shadow_length"],
+                                       q["sun_direction"])
+    r, t = oracle_to_point(a, s, luts, q)
+    e2 = (err(lum, r * sky_k), err(tr, t))
+    sun, sky = m.sun_and_sky_illuminance(q["point"], q["normal"], q["sun_direction"])
+    rs, rk = oracle_irradiance(a, s, luts, q)
+    e3 = (err(sun, rs * sun_k), err(sky, rk * sky_k))
+    m.close()
+    print("illuminance-mode lookups: sky %.2e/%.2e, to point %.2e/%.2e, sun and sky "
+          "%.2e/%.2e" % (e1 + e2 + e3), flush=True)
+    for e in e1 + e2 + e3:
+        assert e <= TOL
+
+
 def test_combined_textures_lookups():
     """COMBINED_SCATTERING_TEXTURES lookups extrapolate the single Mie
     scattering from the alpha channel (functions.glsl:1633-1645)."""
