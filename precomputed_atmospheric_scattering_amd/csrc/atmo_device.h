@@ -394,7 +394,9 @@ struct F16Texels {
 // index, FP64 weight) in a fixed order. A slice index past the last one only
 // ever gets weight 0 (the reference's x > 1 coordinate is clamped by the
 // sampler); it is clamped here too.
-template <class F>
+// Real: the type of the tap weights (double: exact FP64 weights; float:
+// the FP64 axis weights rounded, products in FP32).
+template <class Real = double, class F>
 ATMO_HD void s_taps(const Job& j, const SCoords& c, F&& f) {
   double cnu = c.nu;
   if (!(cnu >= -1.0)) cnu = -1.0;
@@ -418,11 +420,11 @@ ATMO_HD void s_taps(const Job& j, const SCoords& c, F&& f) {
   const int zs[2] = {tz.i0, tz.i1};
   const int ys[2] = {ty.i0, ty.i1};
   const int xs[2] = {tx.i0, tx.i1};
-  const double wz[2] = {1.0 - fz, fz};
-  const double wy[2] = {1.0 - fy, fy};
-  const double wx[2] = {1.0 - fx, fx};
+  const Real wz[2] = {static_cast<Real>(1.0 - fz), static_cast<Real>(fz)};
+  const Real wy[2] = {static_cast<Real>(1.0 - fy), static_cast<Real>(fy)};
+  const Real wx[2] = {static_cast<Real>(1.0 - fx), static_cast<Real>(fx)};
   const int ss[2] = {s0, s1};
-  const double wsl[2] = {1.0 - ws, ws};
+  const Real wsl[2] = {static_cast<Real>(1.0 - ws), static_cast<Real>(ws)};
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int e = 0; e < 2; ++e)
@@ -481,6 +483,38 @@ ATMO_HD void s_lookup_pair_d(const Job& j, const Pairs& lut, const SCoords& c, D
   });
   *s = as;
   *m = am;
+}
+
+// The same lookups with FP32 weights and sums, for lookups whose result is
+// not a difference of two lookups (GetSkyRadiance): relative error about
+// 1e-7 instead of 1e-16, and half the instructions.
+ATMO_HD void acc_f4(float4* acc, float wt, float4 v) {
+  acc->x = fmaf(wt, v.x, acc->x);
+  acc->y = fmaf(wt, v.y, acc->y);
+  acc->z = fmaf(wt, v.z, acc->z);
+  acc->w = fmaf(wt, v.w, acc->w);
+}
+ATMO_HD D4 to_d4(float4 v) { return D4{v.x, v.y, v.z, v.w}; }
+
+template <class Texels>
+ATMO_HD D4 s_lookup_f(const Job& j, const Texels& lut, const SCoords& c) {
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  s_taps<float>(j, c, [&](int64_t i, float wt) { acc_f4(&acc, wt, lut[i]); });
+  return to_d4(acc);
+}
+
+template <class Pairs>
+ATMO_HD void s_lookup_pair_f(const Job& j, const Pairs& lut, const SCoords& c, D4* s,
+                             D4* m) {
+  float4 as = make_float4(0.0f, 0.0f, 0.0f, 0.0f), am = as;
+  s_taps<float>(j, c, [&](int64_t i, float wt) {
+    float4 vs, vm;
+    lut.load(i, &vs, &vm);
+    acc_f4(&as, wt, vs);
+    acc_f4(&am, wt, vm);
+  });
+  *s = to_d4(as);
+  *m = to_d4(am);
 }
 
 ATMO_HD D4 bilinear_d(const float4* lut, int w, int h, double x, double y) {

@@ -47,10 +47,21 @@ __device__ __forceinline__ D4 extrapolated_mie(const Job& J, D4 s) {
             s.z * k * (J.mie_sca[2] / J.rayleigh[2]), 0.0};
 }
 
-// functions.glsl:1658-1690
+// functions.glsl:1658-1690. kF32: FP32 tap weights and sums (lookups whose
+// result is not a difference of two lookups); otherwise FP64.
+template <bool kF32>
 __device__ D4 combined_scattering(const Job& J, double r, double mu, double mu_s,
                                   double nu, bool ground, D4* single_mie) {
   const SCoords c = s_coords(J, r, mu, mu_s, nu, ground);
+  auto lookup = [&](const auto& lut) {
+    return kF32 ? s_lookup_f(J, lut, c) : s_lookup_d(J, lut, c);
+  };
+  auto lookup_pair = [&](const auto& lut, D4* s, D4* m) {
+    if (kF32)
+      s_lookup_pair_f(J, lut, c, s, m);
+    else
+      s_lookup_pair_d(J, lut, c, s, m);
+  };
   D4 s;
   if (J.Q != nullptr && !J.combined) {
     // Interleaved copy: the S and SM texels of a tap in one fetch.
@@ -59,9 +70,9 @@ __device__ D4 combined_scattering(const Job& J, double r, double mu, double mu_s
       const F16Pairs pt{static_cast<const uint4*>(J.Q),
                         make_float4(1.0f / sc[0], 1.0f / sc[1], 1.0f / sc[2], 1.0f / sc[3]),
                         make_float4(1.0f / sc[4], 1.0f / sc[5], 1.0f / sc[6], 0.0f)};
-      s_lookup_pair_d(J, pt, c, &s, single_mie);
+      lookup_pair(pt, &s, single_mie);
     } else {
-      s_lookup_pair_d(J, F32Pairs{static_cast<const float4*>(J.Q)}, c, &s, single_mie);
+      lookup_pair(F32Pairs{static_cast<const float4*>(J.Q)}, &s, single_mie);
     }
     *single_mie = rgb(*single_mie);
     return s;
@@ -70,15 +81,15 @@ __device__ D4 combined_scattering(const Job& J, double r, double mu, double mu_s
     const float* sc = J.scales;
     const F16Texels st{J.S16, make_float4(1.0f / sc[0], 1.0f / sc[1], 1.0f / sc[2],
                                           1.0f / sc[3])};
-    s = s_lookup_d(J, st, c);
+    s = lookup(st);
     if (!J.combined) {
       const F16Texels mt{J.SM16, make_float4(1.0f / sc[4], 1.0f / sc[5], 1.0f / sc[6],
                                              0.0f)};
-      *single_mie = rgb(s_lookup_d(J, mt, c));
+      *single_mie = rgb(lookup(mt));
     }
   } else {
-    s = s_lookup_d(J, F32Texels{J.S}, c);
-    if (!J.combined) *single_mie = rgb(s_lookup_d(J, F32Texels{J.SM}, c));
+    s = lookup(F32Texels{J.S});
+    if (!J.combined) *single_mie = rgb(lookup(F32Texels{J.SM}));
   }
   if (J.combined) *single_mie = extrapolated_mie(J, s);
   return s;
@@ -128,14 +139,14 @@ __global__ __launch_bounds__(kBlock) void k_sky_radiance(
   const D4 t = ground ? D4{0.0, 0.0, 0.0, 0.0} : t_lookup_d(J, J.T, r, mu);
   D4 mie, scat;
   if (shadow == 0.0) {
-    scat = rgb(combined_scattering(J, r, mu, mu_s, nu, ground, &mie));
+    scat = rgb(combined_scattering<true>(J, r, mu, mu_s, nu, ground, &mie));
   } else {
     // Light shafts (functions.glsl:1746-1766).
     const double d = shadow;
     const double r_p = clamp_radius(J, sqrt(d * d + 2.0 * r * mu * d + r * r));
     const double mu_p = (r * mu + d) / r_p;
     const double mu_s_p = (r * mu_s + d * nu) / r_p;
-    scat = rgb(combined_scattering(J, r_p, mu_p, mu_s_p, nu, ground, &mie));
+    scat = rgb(combined_scattering<true>(J, r_p, mu_p, mu_s_p, nu, ground, &mie));
     const D4 st = transmittance_d(J, J.T, r, mu, shadow, ground);
     scat = mul(scat, st);
     mie = mul(mie, st);
@@ -176,14 +187,14 @@ __global__ __launch_bounds__(kBlock) void k_sky_radiance_to_point(
   const bool ground = hits_ground(J, r, mu);
   const D4 t = transmittance_d(J, J.T, r, mu, d, ground);
   D4 mie;
-  D4 scat = rgb(combined_scattering(J, r, mu, mu_s, nu, ground, &mie));
+  D4 scat = rgb(combined_scattering<false>(J, r, mu, mu_s, nu, ground, &mie));
   // functions.glsl:1831-1840
   d = d - shadow > 0.0 ? d - shadow : 0.0;
   const double r_p = clamp_radius(J, sqrt(d * d + 2.0 * r * mu * d + r * r));
   const double mu_p = (r * mu + d) / r_p;
   const double mu_s_p = (r * mu_s + d * nu) / r_p;
   D4 mie_p;
-  const D4 scat_p = rgb(combined_scattering(J, r_p, mu_p, mu_s_p, nu, ground, &mie_p));
+  const D4 scat_p = rgb(combined_scattering<false>(J, r_p, mu_p, mu_s_p, nu, ground, &mie_p));
   D4 st = t;
   if (shadow > 0.0) st = transmittance_d(J, J.T, r, mu, d, ground);
   scat = sub(scat, mul(st, scat_p));
