@@ -517,7 +517,10 @@ ATMO_HD void s_lookup_pair_f(const Job& j, const Pairs& lut, const SCoords& c, D
   *m = to_d4(am);
 }
 
-ATMO_HD D4 bilinear_d(const float4* lut, int w, int h, double x, double y) {
+// Real: the type of the weights and sums (double: exact FP64 weights;
+// float: for lookups whose result is not a difference of two lookups).
+template <class Real = double>
+ATMO_HD D4 bilinear_r(const float4* lut, int w, int h, double x, double y) {
   Taps tx = axis_taps(x, w), ty = axis_taps(y, h);
   if (!(x >= -1.0)) x = -1.0;
   if (x > static_cast<double>(w)) x = static_cast<double>(w);
@@ -526,7 +529,8 @@ ATMO_HD D4 bilinear_d(const float4* lut, int w, int h, double x, double y) {
   const double fx = x - floor(x), fy = y - floor(y);
   const float4 a = gload(lut, ty.i0 * w + tx.i0), b = gload(lut, ty.i0 * w + tx.i1);
   const float4 c = gload(lut, ty.i1 * w + tx.i0), d = gload(lut, ty.i1 * w + tx.i1);
-  const double wx1 = fx, wx0 = 1.0 - fx, wy1 = fy, wy0 = 1.0 - fy;
+  const Real wx1 = static_cast<Real>(fx), wx0 = static_cast<Real>(1.0 - fx);
+  const Real wy1 = static_cast<Real>(fy), wy0 = static_cast<Real>(1.0 - fy);
   D4 o;
   o.x = wy0 * (wx0 * a.x + wx1 * b.x) + wy1 * (wx0 * c.x + wx1 * d.x);
   o.y = wy0 * (wx0 * a.y + wx1 * b.y) + wy1 * (wx0 * c.y + wx1 * d.y);
@@ -534,11 +538,18 @@ ATMO_HD D4 bilinear_d(const float4* lut, int w, int h, double x, double y) {
   o.w = 0.0;
   return o;
 }
+ATMO_HD D4 bilinear_d(const float4* lut, int w, int h, double x, double y) {
+  return bilinear_r<double>(lut, w, h, x, y);
+}
 
-ATMO_HD D4 t_lookup_d(const Job& j, const float4* T, double r, double mu) {
+template <class Real = double>
+ATMO_HD D4 t_lookup_r(const Job& j, const float4* T, double r, double mu) {
   double x, y;
   t_texel_coords(j, r, mu, &x, &y);
-  return bilinear_d(T, j.tw, j.th, x, y);
+  return bilinear_r<Real>(T, j.tw, j.th, x, y);
+}
+ATMO_HD D4 t_lookup_d(const Job& j, const float4* T, double r, double mu) {
+  return t_lookup_r<double>(j, T, r, mu);
 }
 
 // functions.glsl:493-519 in FP64 arithmetic.
@@ -558,14 +569,16 @@ ATMO_HD D4 transmittance_d(const Job& j, const float4* T, double r, double mu,
   return D4{m1(num.x / den.x), m1(num.y / den.y), m1(num.z / den.z), 0.0};
 }
 
-// functions.glsl:552-563 in FP64 arithmetic.
+// functions.glsl:552-563 in FP64 arithmetic (Real: the lookup's weights and
+// sums).
+template <class Real = double>
 ATMO_HD D4 transmittance_to_sun_d(const Job& j, const float4* T, double r,
                                   double mu_s) {
   const double sin_h = j.rb / r;
   const double cos_h = -sqrt(1.0 - sin_h * sin_h > 0.0 ? 1.0 - sin_h * sin_h : 0.0);
   const double f = smoothstep_d(-sin_h * j.sun_angular_radius,
                                 sin_h * j.sun_angular_radius, mu_s - cos_h);
-  D4 t = t_lookup_d(j, T, r, mu_s);
+  D4 t = t_lookup_r<Real>(j, T, r, mu_s);
   return D4{t.x * f, t.y * f, t.z * f, 0.0};
 }
 
@@ -631,11 +644,12 @@ ATMO_HD void e_rmus_from_texel(const Job& j, int i, int k, double* r,
 }
 
 
+template <class Real = double>
 ATMO_HD D4 e_lookup_d(const Job& j, const float4* E, double r, double mu_s) {
   double x_r = (r - j.rb) / (j.rt - j.rb);
   double x_mu_s = mu_s * 0.5 + 0.5;
-  return bilinear_d(E, j.ew, j.eh, unit_to_texel(x_mu_s, j.ew),
-                    unit_to_texel(x_r, j.eh));
+  return bilinear_r<Real>(E, j.ew, j.eh, unit_to_texel(x_mu_s, j.ew),
+                          unit_to_texel(x_r, j.eh));
 }
 
 // functions.glsl:739-747

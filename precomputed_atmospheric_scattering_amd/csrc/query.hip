@@ -218,8 +218,10 @@ __global__ __launch_bounds__(kBlock) void k_sun_and_sky_irradiance(
   const D3 sd = load3(sun_direction, i);
   const double r = sqrt(dot3(p, p));
   const double mu_s = dot3(p, sd) / r;
-  D4 sky = scale(e_lookup_d(J, J.E, r, mu_s), (1.0 + dot3(nrm, p) / r) * 0.5);
-  const D4 ts = transmittance_to_sun_d(J, J.T, r, mu_s);
+  // Two lookups, neither a difference: FP32 weights and sums (as in
+  // GetSkyRadiance).
+  D4 sky = scale(e_lookup_d<float>(J, J.E, r, mu_s), (1.0 + dot3(nrm, p) / r) * 0.5);
+  const D4 ts = transmittance_to_sun_d<float>(J, J.T, r, mu_s);
   const double c = dot3(nrm, sd);
   const double cf = c > 0.0 ? c : 0.0;
   D4 sun{J.solar[0] * ts.x * cf, J.solar[1] * ts.y * cf, J.solar[2] * ts.z * cf, 0.0};
