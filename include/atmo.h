@@ -159,8 +159,10 @@ typedef struct atmo_model atmo_model;
  * copies the parameters, allocates the result LUTs on the current device. */
 atmo_status atmo_model_create(const atmo_model_params* params,
                               const atmo_lut_sizes* sizes, atmo_model** out);
-/* Model::~Model (atmosphere/model.cc:801-811). */
-void atmo_model_destroy(atmo_model* model);
+/* Model::~Model (atmosphere/model.cc:801-811). NULL is a no-op. A model
+ * held by a live atmo_batch is not destroyed (ATMO_ERROR_INVALID_ARGUMENT):
+ * destroy the batch first. */
+atmo_status atmo_model_destroy(atmo_model* model);
 atmo_status atmo_model_set_option(atmo_model* model, atmo_option option,
                                   int value);
 /* Launch timings of the last Init of a model created with
@@ -182,9 +184,14 @@ atmo_status atmo_model_init_batch(atmo_model* const* models,
 /* A reusable batched precompute plan (no counterpart in the reference): the
  * pass sequence of several models, one launch per pass level over all of
  * them, recorded once. With use_graph the launches are captured in a HIP
- * graph and atmo_batch_launch replays the graph. The models keep their
- * temporaries until the batch is destroyed; a model must not be in two live
- * batches. atmo_model_init_batch is create + launch + wait + destroy. */
+ * graph and atmo_batch_launch replays the graph. The batch's launches use
+ * the models' temporaries and lookup copies, so while it lives: a model
+ * cannot join a second batch (atmo_batch_create fails), atmo_model_init
+ * keeps the temporaries, turning ATMO_OPTION_INTERLEAVED_LOOKUPS on or off
+ * and atmo_model_destroy fail. The temporaries are freed when the batch is
+ * destroyed (unless the model retains them). Models owned by a server
+ * cannot be batched. atmo_model_init_batch is create + launch + wait +
+ * destroy. */
 typedef struct atmo_batch atmo_batch;
 atmo_status atmo_batch_create(atmo_model* const* models,
                               const unsigned* orders, int count, int use_graph,
@@ -265,6 +272,15 @@ atmo_status atmo_model_export_luts(const atmo_model* model,
  * model's; FP16 models re-encode with the same per-channel scale rule as
  * Init). The model is then initialised, as after Init. */
 atmo_status atmo_model_import_luts(atmo_model* model, const char* directory);
+/* The per-channel scale of FP16 LUT storage (half_precision; no
+ * counterpart in the reference, whose half-float textures are unscaled):
+ * for a channel whose largest magnitude is max_abs, the power of two
+ * 2^min(14 - e, 126), where max_abs = f 2^e with f in [0.5, 1). It maps
+ * max_abs into [2^13, 2^14), well inside the FP16 range, and stays finite
+ * (with a normal inverse) for tiny and denormal maxima. Zero, negative, NaN
+ * and infinite max_abs give 1. Init and atmo_model_import_luts use it;
+ * atmo_lut_view.scale reports it. Host-only. */
+float atmo_fp16_channel_scale(float max_abs);
 /* reference::Model::Init with a cache directory
  * (atmosphere/reference/model.cc:74-84): loads the LUTs if
  * directory/transmittance.dat exists, else runs Init and saves them. */

@@ -101,6 +101,42 @@ def _stream_handle(stream):
     return stream.handle if isinstance(stream, Stream) else stream
 
 
+# Lookup argument layouts: 3 = n x 3 floats, 1 = n floats (shadow_length).
+_SKY_INPUTS = (3, 3, 1, 3)        # camera, view_ray/point, shadow_length, sun
+_SUN_SKY_INPUTS = (3, 3, 3)       # point, normal, sun_direction
+
+
+def _check_device_arrays(n, arrays, widths, what):
+    """Raises ValueError unless n >= 0 and every device array is float32 and
+    holds at least n x width floats. The kernels index the arrays with n
+    only, so a short array would be read or written out of bounds."""
+    if isinstance(n, bool) or int(n) != n or n < 0:
+        raise ValueError("%s: n must be a non-negative integer" % what)
+    n = int(n)
+    for i, (a, w) in enumerate(zip(arrays, widths)):
+        if not isinstance(a, DeviceArray) or a.ptr is None:
+            raise ValueError("%s: argument %d is not a live DeviceArray" % (what, i))
+        if a.dtype != np.float32:
+            raise ValueError("%s: argument %d must be float32" % (what, i))
+        if a.nbytes < 4 * n * w:
+            raise ValueError("%s: argument %d holds %d bytes, %d x %d floats need %d"
+                             % (what, i, a.nbytes, n, w, 4 * n * w))
+    return n
+
+
+def _check_host_arrays(arrays, widths, what):
+    """Host inputs of a lookup as float32 arrays of one leading length n
+    (n x 3, or n for shadow_length); returns (n, arrays)."""
+    out = [np.ascontiguousarray(a, dtype=np.float32) for a in arrays]
+    n = len(out[0])
+    for i, (a, w) in enumerate(zip(out, widths)):
+        want = (n, 3) if w == 3 else (n,)
+        if a.shape != want:
+            raise ValueError("%s: argument %d has shape %s, expected %s"
+                             % (what, i, a.shape, want))
+    return n, out
+
+
 class DeviceArray:
     """A float32 device buffer allocated through libatmo."""
 
@@ -275,9 +311,11 @@ class Model:
         return self._handle
 
     def close(self):
+        """Frees the model (~Model). Raises AtmoError, keeping the model, if
+        a live Batch holds it."""
         if self._handle:
             if not getattr(self, "_borrowed", False):
-                lib().atmo_model_destroy(self._handle)
+                check(lib().atmo_model_destroy(self._handle), "Model.close")
             self._handle = None
 
     def __del__(self):
@@ -341,6 +379,9 @@ class Model:
     # ---- lookups on device arrays ----
     def sky_radiance_device(self, n, camera, view_ray, shadow_length, sun_direction,
                             radiance, transmittance, stream=None):
+        n = _check_device_arrays(
+            n, (camera, view_ray, shadow_length, sun_direction, radiance, transmittance),
+            _SKY_INPUTS + (3, 3), "GetSkyRadiance")
         check(lib().atmo_get_sky_radiance(self._handle, n, camera.ptr, view_ray.ptr,
                                           shadow_length.ptr, sun_direction.ptr,
                                           radiance.ptr, transmittance.ptr,
@@ -348,6 +389,9 @@ class Model:
 
     def sky_radiance_to_point_device(self, n, camera, point, shadow_length, sun_direction,
                                      radiance, transmittance, stream=None):
+        n = _check_device_arrays(
+            n, (camera, point, shadow_length, sun_direction, radiance, transmittance),
+            _SKY_INPUTS + (3, 3), "GetSkyRadianceToPoint")
         check(lib().atmo_get_sky_radiance_to_point(self._handle, n, camera.ptr, point.ptr,
                                                    shadow_length.ptr, sun_direction.ptr,
                                                    radiance.ptr, transmittance.ptr,
@@ -356,6 +400,9 @@ class Model:
 
     def sun_and_sky_irradiance_device(self, n, point, normal, sun_direction,
                                       sun_irradiance, sky_irradiance, stream=None):
+        n = _check_device_arrays(
+            n, (point, normal, sun_direction, sun_irradiance, sky_irradiance),
+            _SUN_SKY_INPUTS + (3, 3), "GetSunAndSkyIrradiance")
         check(lib().atmo_get_sun_and_sky_irradiance(self._handle, n, point.ptr, normal.ptr,
                                                     sun_direction.ptr, sun_irradiance.ptr,
                                                     sky_irradiance.ptr,
@@ -365,6 +412,9 @@ class Model:
     # ---- luminance API (atmosphere/model.cc:252-281) on device arrays ----
     def sky_luminance_device(self, n, camera, view_ray, shadow_length, sun_direction,
                              luminance, transmittance, stream=None):
+        n = _check_device_arrays(
+            n, (camera, view_ray, shadow_length, sun_direction, luminance, transmittance),
+            _SKY_INPUTS + (3, 3), "GetSkyLuminance")
         check(lib().atmo_get_sky_luminance(self._handle, n, camera.ptr, view_ray.ptr,
                                            shadow_length.ptr, sun_direction.ptr,
                                            luminance.ptr, transmittance.ptr,
@@ -372,6 +422,9 @@ class Model:
 
     def sky_luminance_to_point_device(self, n, camera, point, shadow_length,
                                       sun_direction, luminance, transmittance, stream=None):
+        n = _check_device_arrays(
+            n, (camera, point, shadow_length, sun_direction, luminance, transmittance),
+            _SKY_INPUTS + (3, 3), "GetSkyLuminanceToPoint")
         check(lib().atmo_get_sky_luminance_to_point(self._handle, n, camera.ptr, point.ptr,
                                                     shadow_length.ptr, sun_direction.ptr,
                                                     luminance.ptr, transmittance.ptr,
@@ -380,6 +433,9 @@ class Model:
 
     def sun_and_sky_illuminance_device(self, n, point, normal, sun_direction,
                                        sun_illuminance, sky_illuminance, stream=None):
+        n = _check_device_arrays(
+            n, (point, normal, sun_direction, sun_illuminance, sky_illuminance),
+            _SUN_SKY_INPUTS + (3, 3), "GetSunAndSkyIlluminance")
         check(lib().atmo_get_sun_and_sky_illuminance(self._handle, n, point.ptr,
                                                      normal.ptr, sun_direction.ptr,
                                                      sun_illuminance.ptr,
@@ -398,49 +454,43 @@ class Model:
         check(lib().atmo_get_luminance_factors(self._handle, sky, sun), "luminance_factors")
         return np.array(list(sky)), np.array(list(sun))
 
-    def _host_call(self, fn, arrays, n_out=2):
-        n = len(arrays[0])
-        d = [DeviceArray.from_host(np.asarray(a, np.float32)) for a in arrays]
-        outs = [DeviceArray((n, 3)) for _ in range(n_out)]
+    # ---- host-array conveniences (copy in, run, copy out) ----
+    def _host_call(self, fn, arrays, widths, what):
+        n, host = _check_host_arrays(arrays, widths, what)
+        d = [DeviceArray.from_host(a) for a in host]
+        outs = [DeviceArray((n, 3)) for _ in range(2)]
         fn(n, *d, *outs)
         return tuple(o.to_host() for o in outs)
 
     def sky_luminance(self, camera, view_ray, shadow_length, sun_direction):
         return self._host_call(self.sky_luminance_device,
-                               (camera, view_ray, shadow_length, sun_direction))
+                               (camera, view_ray, shadow_length, sun_direction),
+                               _SKY_INPUTS, "GetSkyLuminance")
 
     def sky_luminance_to_point(self, camera, point, shadow_length, sun_direction):
         return self._host_call(self.sky_luminance_to_point_device,
-                               (camera, point, shadow_length, sun_direction))
+                               (camera, point, shadow_length, sun_direction),
+                               _SKY_INPUTS, "GetSkyLuminanceToPoint")
 
     def sun_and_sky_illuminance(self, point, normal, sun_direction):
         return self._host_call(self.sun_and_sky_illuminance_device,
-                               (point, normal, sun_direction))
+                               (point, normal, sun_direction), _SUN_SKY_INPUTS,
+                               "GetSunAndSkyIlluminance")
 
-    # ---- host-array conveniences (copy in, run, copy out) ----
     def sky_radiance(self, camera, view_ray, shadow_length, sun_direction):
-        n = len(camera)
-        d = [DeviceArray.from_host(np.asarray(a, np.float32))
-             for a in (camera, view_ray, shadow_length, sun_direction)]
-        rad, tr = DeviceArray((n, 3)), DeviceArray((n, 3))
-        self.sky_radiance_device(n, *d, rad, tr)
-        return rad.to_host(), tr.to_host()
+        return self._host_call(self.sky_radiance_device,
+                               (camera, view_ray, shadow_length, sun_direction),
+                               _SKY_INPUTS, "GetSkyRadiance")
 
     def sky_radiance_to_point(self, camera, point, shadow_length, sun_direction):
-        n = len(camera)
-        d = [DeviceArray.from_host(np.asarray(a, np.float32))
-             for a in (camera, point, shadow_length, sun_direction)]
-        rad, tr = DeviceArray((n, 3)), DeviceArray((n, 3))
-        self.sky_radiance_to_point_device(n, *d, rad, tr)
-        return rad.to_host(), tr.to_host()
+        return self._host_call(self.sky_radiance_to_point_device,
+                               (camera, point, shadow_length, sun_direction),
+                               _SKY_INPUTS, "GetSkyRadianceToPoint")
 
     def sun_and_sky_irradiance(self, point, normal, sun_direction):
-        n = len(point)
-        d = [DeviceArray.from_host(np.asarray(a, np.float32))
-             for a in (point, normal, sun_direction)]
-        sun, sky = DeviceArray((n, 3)), DeviceArray((n, 3))
-        self.sun_and_sky_irradiance_device(n, *d, sun, sky)
-        return sun.to_host(), sky.to_host()
+        return self._host_call(self.sun_and_sky_irradiance_device,
+                               (point, normal, sun_direction), _SUN_SKY_INPUTS,
+                               "GetSunAndSkyIrradiance")
 
     # atmosphere/model.h:301-304
     @staticmethod
@@ -524,7 +574,13 @@ class Server:
         they are kept alive until the request completes."""
         kind = {"sky": _capi.LOOKUP_SKY, "sky_to_point": _capi.LOOKUP_SKY_TO_POINT,
                 "sun_and_sky": _capi.LOOKUP_SUN_AND_SKY}[lookup]
-        ins = [a.ptr if a is not None else None for a in inputs] + [None] * (4 - len(inputs))
+        widths = _SUN_SKY_INPUTS if lookup == "sun_and_sky" else _SKY_INPUTS
+        if len(inputs) != len(widths) or len(outputs) != 2:
+            raise ValueError("submit_lookup: %s takes %d inputs and 2 outputs"
+                             % (lookup, len(widths)))
+        n = _check_device_arrays(n, list(inputs) + list(outputs), widths + (3, 3),
+                                 "submit_lookup")
+        ins = [a.ptr for a in inputs] + [None] * (4 - len(inputs))
         outs = [a.ptr for a in outputs]
         rid = C.c_uint64()
         check(lib().atmo_server_submit_lookup(self.handle, int(model_id), kind,

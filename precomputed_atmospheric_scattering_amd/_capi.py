@@ -125,7 +125,8 @@ VP, FP, I64, SZ = C.c_void_p, C.POINTER(C.c_float), C.c_int64, C.c_size_t
 ST = C.c_int
 SIGNATURES = {
     "atmo_model_create": (ST, [C.POINTER(ModelParams), C.POINTER(LutSizes), C.POINTER(VP)]),
-    "atmo_model_destroy": (None, [VP]),
+    "atmo_model_destroy": (ST, [VP]),
+    "atmo_fp16_channel_scale": (C.c_float, [C.c_float]),
     "atmo_model_set_option": (ST, [VP, C.c_int, C.c_int]),
     "atmo_model_pass_timings": (ST, [VP, C.POINTER(PassTiming), C.c_int,
                                      C.POINTER(C.c_int)]),

@@ -163,7 +163,7 @@ atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStrea
   for (int i = 0; i < k; ++i) {
     atmo_model* m = nullptr;
     ATMO_TRY(atmo_model_create(&p, &lz, &m));
-    owned.emplace_back(m, atmo_model_destroy);
+    owned.emplace_back(m, [](atmo_model* x) { (void)atmo_model_destroy(x); });
     models.push_back(m);
     m->retain = true;  // keep the deltas: the timed launches read them
   }

@@ -4,6 +4,7 @@
 // reference CPU model's cache (atmosphere/reference/model.cc:74-84, :240-244).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -46,12 +47,13 @@ bool exported(const atmo_model* m, atmo_lut_id id) {
 }
 
 // The FP16 scale of a channel whose largest magnitude is mx: a power of two
-// that maps mx into [2^13, 2^14) (the rule of k_to_half).
+// that maps mx into [2^13, 2^14), with the exponent capped at 126 (the rule
+// of k_to_half).
 float pow2_scale(float mx) {
   if (!(mx > 0.0f) || std::isinf(mx)) return 1.0f;
   int e;
   std::frexp(mx, &e);
-  return std::ldexp(1.0f, 14 - e);
+  return std::ldexp(1.0f, std::min(14 - e, 126));
 }
 
 uint2 pack_half4(const float* v) {
@@ -179,5 +181,7 @@ atmo_status atmo_model_init_cached(atmo_model* m, unsigned orders, const char* d
     return fail(ATMO_ERROR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(err));
   return atmo_model_export_luts(m, directory);
 }
+
+float atmo_fp16_channel_scale(float max_abs) { return pow2_scale(max_abs); }
 
 }  // extern "C"

@@ -1403,12 +1403,15 @@ __global__ __launch_bounds__(kBlock) void k_lut_max(const Job* __restrict__ jobs
   }
 }
 
+// The scale maps the channel maximum mx into [2^13, 2^14). The exponent is
+// capped at 126, so that channels whose maximum is tiny (below 2^-112,
+// denormals included) keep a finite scale whose inverse is a normal float.
 __device__ __forceinline__ float pow2_scale(unsigned bits) {
   const float mx = __uint_as_float(bits);
   if (!(mx > 0.0f) || isinf(mx)) return 1.0f;
   int e;
   frexpf(mx, &e);  // mx = f * 2^e, f in [0.5, 1)
-  return ldexpf(1.0f, 14 - e);
+  return ldexpf(1.0f, min(14 - e, 126));
 }
 
 __global__ __launch_bounds__(kBlock) void k_to_half(const Job* __restrict__ jobs,

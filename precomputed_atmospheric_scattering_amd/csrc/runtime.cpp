@@ -626,8 +626,8 @@ atmo_status atmo_model_create(const atmo_model_params* p,
 
   // On any failure below, the partially built model is destroyed (which frees
   // whatever was allocated).
-  std::unique_ptr<atmo_model, void (*)(atmo_model*)> m(new atmo_model(),
-                                                       atmo_model_destroy);
+  std::unique_ptr<atmo_model, void (*)(atmo_model*)> m(
+      new atmo_model(), [](atmo_model* x) { (void)atmo_model_destroy(x); });
   build_spec(p, &m->spec);
   const Spec& s = m->spec;
   m->sizes = Sizes{z.transmittance_width, z.transmittance_height, z.scattering_r,
@@ -674,8 +674,11 @@ atmo_status atmo_model_create(const atmo_model_params* p,
   return ATMO_OK;
 }
 
-void atmo_model_destroy(atmo_model* m) {
-  if (m == nullptr) return;
+atmo_status atmo_model_destroy(atmo_model* m) {
+  if (m == nullptr) return ATMO_OK;
+  if (m->live_batches > 0)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT,
+                "model is in a live batch: destroy the batch first");
   (void)hipDeviceSynchronize();
   free_temporaries(m);
   void* bufs[9] = {m->T, m->S, m->SM, m->E, m->S16, m->SM16, m->d_scales, m->d_maxbits, m->Q};
@@ -685,6 +688,7 @@ void atmo_model_destroy(atmo_model* m) {
   if (m->d_first_block != nullptr) (void)hipFree(m->d_first_block);
   for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
   delete m;
+  return ATMO_OK;
 }
 
 atmo_status atmo_model_set_option(atmo_model* m, atmo_option option, int value) {
@@ -699,6 +703,9 @@ atmo_status atmo_model_set_option(atmo_model* m, atmo_option option, int value) 
     case ATMO_OPTION_INTERLEAVED_LOOKUPS: {
       const bool on = value != 0 && !m->spec.combined;
       if (on == (m->Q != nullptr)) return ATMO_OK;
+      if (m->live_batches > 0)
+        return fail(ATMO_ERROR_INVALID_ARGUMENT,
+                    "model is in a live batch, whose launches write the lookup copy");
       // The job table holds Q: wait for work that may use it, switch, and
       // upload the table again.
       ATMO_HIP(hipDeviceSynchronize());
@@ -748,7 +755,8 @@ atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
   }
   ATMO_TRY(build_lookup_copy(m, s));
   m->initialized = true;
-  if (!m->retain) {
+  // A live batch holding the model keeps reading and writing its temporaries.
+  if (!m->retain && m->live_batches == 0) {
     // Init then returns once the precomputation is complete, as the GL
     // Init does once its draw calls have been issued and the temporaries
     // deleted (atmosphere/model.cc:966-974).
@@ -1383,6 +1391,10 @@ atmo_status atmo_batch_create(atmo_model* const* models, const unsigned* orders,
     for (int k = 0; k < i; ++k)
       if (models[k] == models[i])
         return fail(ATMO_ERROR_INVALID_ARGUMENT, "a model appears twice in a batch");
+    if (models[i]->live_batches > 0)
+      return fail(ATMO_ERROR_INVALID_ARGUMENT, "a model is already in a live batch");
+    if (models[i]->external_temporaries)
+      return fail(ATMO_ERROR_INVALID_ARGUMENT, "a model is owned by a server");
     b->models.push_back(models[i]);
     b->orders.push_back(orders[i]);
   }
@@ -1416,6 +1428,7 @@ atmo_status atmo_batch_create(atmo_model* const* models, const unsigned* orders,
                                          hipGetErrorString(e != hipSuccess ? e : e2));
     ATMO_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
+  for (atmo_model* m : b->models) ++m->live_batches;
   *out = b.release();
   return ATMO_OK;
 }
@@ -1439,8 +1452,10 @@ int atmo_batch_num_launches(const atmo_batch* b) {
 void atmo_batch_destroy(atmo_batch* b) {
   if (b == nullptr) return;
   (void)hipDeviceSynchronize();
-  for (atmo_model* m : b->models)
-    if (!m->retain) free_temporaries(m);
+  for (atmo_model* m : b->models) {
+    --m->live_batches;
+    if (!m->retain && m->live_batches == 0) free_temporaries(m);
+  }
   destroy_batch(b);
 }
 

@@ -71,6 +71,9 @@ struct atmo_model {
   bool retain = false;
   bool initialized = false;
   bool external_temporaries = false;  // temporaries carved from caller memory
+  // Live atmo_batch objects holding this model (0 or 1): their job tables
+  // point at the model's temporaries and lookup copy, which must then stay.
+  int live_batches = 0;
   float4 *T = nullptr, *S = nullptr, *SM = nullptr, *E = nullptr;
   // Init temporaries, carved from one block (owned, or caller memory with
   // external_temporaries). In illuminance mode each Precompute iteration has

@@ -189,3 +189,22 @@ def test_lut_file_null_arguments():
     assert lib.atmo_model_export_luts(None, b"/tmp") == 1
     assert lib.atmo_model_import_luts(None, b"/tmp") == 1
     assert lib.atmo_model_init_cached(None, 4, b"/tmp", None) == 1
+
+
+def test_fp16_channel_scale_stays_finite():
+    """The FP16 storage scale 2^min(14 - e, 126) (max = f 2^e, f in
+    [0.5, 1)) maps a channel maximum into [2^13, 2^14) and stays finite, with
+    a normal inverse, for tiny and denormal maxima."""
+    import math
+    f = _capi.lib().atmo_fp16_channel_scale
+    for mx in (1.0, 0.75, 3.0e4, 1.5e-3, 7.0e-20, 2.0 ** -100):
+        sc = f(mx)
+        assert sc == 2.0 ** round(math.log2(sc))
+        assert 2.0 ** 13 <= np.float32(mx) * sc < 2.0 ** 14, (mx, sc)
+    for mx in (2.0 ** -113, 1e-38, 1e-45, np.finfo(np.float32).tiny):
+        sc = f(mx)
+        assert math.isfinite(sc) and sc == 2.0 ** 126, (mx, sc)
+        assert np.float32(mx) * sc < 2.0 ** 14
+        assert 1.0 / sc >= np.finfo(np.float32).tiny
+    for mx in (0.0, -1.0, float("inf"), float("nan")):
+        assert f(mx) == 1.0

@@ -82,3 +82,38 @@ def test_full_size_batch_equals_alone():
         for n, r in ref.items():
             assert np.array_equal(m.lut(n), r), n
         m.close()
+
+
+def test_batch_holds_its_models():
+    """While a batch lives, its models cannot join another batch, drop their
+    interleaved lookup copy or be destroyed, and Init keeps the temporaries
+    the batch's launches use; after the batch is destroyed all of that works
+    again. The batch's result is unchanged by an Init in between."""
+    cfgs = small_batch(23, 2)
+    orders = [c["orders"] for c in cfgs]
+    models = [atmo.Model(**configs.model_kwargs(c)[0]) for c in cfgs]
+    b = atmo.Batch(models, orders)
+    b.launch()
+    atmo.synchronize()
+    ref = [{n: m.lut(n) for n in NAMES} for m in models]
+    with pytest.raises(atmo.AtmoError, match="live batch"):
+        atmo.Batch([models[0]], [orders[0]])
+    with pytest.raises(atmo.AtmoError, match="live batch"):
+        models[1].set_interleaved_lookups(False)
+    with pytest.raises(atmo.AtmoError, match="live batch"):
+        models[0].close()
+    assert models[0].handle is not None
+    # Init of a batched model (no retained intermediates) must not free the
+    # temporaries: the replay below writes them.
+    models[0].init(orders[0])
+    b.launch()
+    atmo.synchronize()
+    for m, r in zip(models, ref):
+        for n in NAMES:
+            assert np.array_equal(m.lut(n), r[n]), n
+    b.close()
+    models[1].set_interleaved_lookups(False)
+    b2 = atmo.Batch([models[0]], [orders[0]])
+    b2.close()
+    for m in models:
+        m.close()

@@ -51,6 +51,13 @@ def test_fp16_views_and_scales():
     m32 = atmo.Model(**configs.model_kwargs(configs.earth(lut_sizes=parity.SMALL_SIZES))[0])
     res32, _ = m32.footprint()
     assert res < res32
+    # The device rule is the host one, on the FP32 sums it stores (the FP32
+    # model computes the same bits).
+    m32.init(orders)
+    s32 = m32.lut("scattering")
+    scale = atmo.lib().atmo_fp16_channel_scale
+    for c in range(4):
+        assert v.scale[c] == scale(float(np.abs(s32[..., c]).max())), c
     m.close()
     m32.close()
 
