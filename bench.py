@@ -1,20 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the LUT precompute + sky lookups on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-Workload (one step): a full precompute (Model::Init, 4 scattering orders) of
-the spectral atmosphere - 47-band 5778 K solar spectrum, Rayleigh lambda^-4,
-Angstrom Mie, ozone tent, 15 precomputed wavelengths (5 Precompute
-iterations), FP32 LUTs at the reference sizes (transmittance 256x64,
-scattering 32x128x32x8 packed as 256x128x32, irradiance 64x16) - followed by
-one 1920x1080 batch of each sky lookup (GetSkyRadiance, GetSkyRadianceToPoint,
-GetSunAndSkyIrradiance) against the resulting LUTs. Inputs are resident on
-the GPU before the timed region. The printed JSON line reports precomputes
-per second (value), ms per atmosphere, sky rays per second, p50/p99 step
-latency, the roofline of the dominant kernel (scattering density) from HIP
-events around its launches, and the FP64 CPU oracle timed on a bounded sample
-of the same workload.
+Default (--workload headline): one JSON line whose headline steps are the
+high-resolution stress workload - the spectral atmosphere (47-band 5778 K
+solar spectrum, Rayleigh lambda^-4, Angstrom Mie, ozone tent) with 15
+precomputed wavelengths (5 Precompute iterations), 8 scattering orders, FP16
+LUT storage with FP32 accumulation, LUTs 512x128 (transmittance),
+64x256x64x16 (scattering, packed 1024x256x64) and 128x32 (irradiance). One
+step is Model::Init as the reference's API runs it (temporaries allocated and
+freed by the call), then one 1920x1080 batch of each sky lookup
+(GetSkyLuminance, GetSkyLuminanceToPoint, GetSunAndSkyIlluminance) on the
+resulting LUTs. Inputs are resident on the GPU before the timed region. The
+same line carries, measured in the same run after the headline steps:
+
+- "spectral": the same atmosphere at the reference LUT sizes, 4 orders, FP32
+  and FP16 storage;
+- "batched": 64 seeded random atmospheres, one-at-a-time Init against one
+  batched submission replayed from a HIP graph;
+- "serving": Poisson precompute requests at 20/s plus one 1920x1080 query
+  batch per finished model through the serving loop, and the same loop
+  under a saturating burst;
+- "roofline": the dominant precompute kernel's counter-measured VALU rate
+  (instructions from the committed rocprofv3 counter summary, launch times
+  from HIP events in this run) against the FP32 VALU peak; "passes" gives
+  the same per pass, with HBM GB/s;
+- "cpu_baseline": the FP64 oracle on the host cores, on a bounded sample.
+
+The other --workload values run one section alone (for profiling).
 """
 import argparse
 import json
@@ -28,11 +42,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "LUT precompute ms/atmosphere + sky rays/s, p99 latency, at ≤1e-3 rel err"
-PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (FMA = 2 flops)
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (packed FMA = 2 flops per lane)
+# VALU lane-operations per second at the FP32 peak: 256 CUs x 2.4 GHz x 128
+# FMA lanes per clock (two VALU instructions of a wave per 4 clocks per SIMD,
+# or one packed). Every VALU instruction of a wave counts 64 lane-ops.
+PEAK_VALU_TOPS = PEAK_FP32_TFLOPS / 2.0
 PEAK_HBM_GBS = 8000.0
 # Algorithmic work per integration sample (VALU lane-ops, transcendentals
-# weighted 4, loop invariants hoisted); frozen estimates of the reference's
-# per-sample work, used for every 'achieved' figure.
+# weighted 4, loop invariants hoisted): frozen estimates of the reference's
+# per-sample work (a texel evaluated on its own). The kernels here share work
+# across samples and texels, so these give an "algorithmic equivalent" rate,
+# not a roofline fraction.
 LANE_OPS = dict(transmittance=40, single_scattering=178, scattering_density_2=111,
                 scattering_density_n=68, indirect_irradiance=166,
                 multiple_scattering=260)
@@ -41,6 +61,14 @@ SAMPLES = dict(transmittance=501, single_scattering=51, scattering_density=512,
 QUERY_BYTES_PER_RAY = 124
 QUERY_LANE_OPS_PER_RAY = 1500
 WIDTH, HEIGHT = 1920, 1080
+PROFILES = os.path.join(ROOT, "profiles")
+COUNTERS_HIGH_RES = os.path.join(PROFILES, "counters_high_resolution.md")
+# Pass -> kernel of the counter summary (the transmittance pass's launches
+# are two kernels, the optical depths and the LUT, and carry no counters).
+PASS_KERNELS = dict(single_scattering="k_single_scattering",
+                    scattering_density="k_scattering_density",
+                    indirect_irradiance="k_indirect_irradiance",
+                    multiple_scattering="k_multiple_scattering")
 
 
 def percentile(xs, q):
@@ -65,28 +93,63 @@ def sd_flops(n_s, order):
     return 2.0 * n_s * SAMPLES["scattering_density"] * ops
 
 
-PROFILE_TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                               "hbm_traffic_spectral_workload.md")
-
-
-def profiled_traffic(kernel):
-    """HBM bytes per dispatch of a kernel of the spectral workload, from the
-    FETCH_SIZE and WRITE_SIZE passes summarized in the committed profile
-    (calibrated against a copy of known size): (bytes, source) or (None,
-    None) if the profile has no row for it."""
+def profile_table(path):
+    """Rows of the first markdown table of a committed profile summary, as
+    dicts keyed by the header cells ({} if the file is absent)."""
     try:
-        rows = open(PROFILE_TRAFFIC).read().splitlines()
+        lines = open(path).read().splitlines()
     except OSError:
-        return None, None
-    for r in rows:
+        return {}
+    rows, header = {}, None
+    for r in lines:
+        if not r.startswith("|"):
+            if header is not None and rows:
+                break
+            continue
         cells = [c.strip() for c in r.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0] == kernel:
-            try:
-                return round((float(cells[2]) + float(cells[3])) * 1e6), \
-                    "profiles/hbm_traffic_spectral_workload.md"
-            except ValueError:
-                return None, None
-    return None, None
+        if header is None:
+            header = cells
+            continue
+        if set(cells[0]) <= set("-: "):
+            continue
+        rows[cells[0]] = dict(zip(header, cells))
+    return rows
+
+
+def counter_figures(path=COUNTERS_HIGH_RES):
+    """Per kernel: VALU wave-instructions, fetched and written HBM bytes per
+    dispatch, from the committed counter summary of the high-resolution
+    workload (rocprofv3 --pmc runs; HBM bytes calibrated against a copy of
+    known size)."""
+    out = {}
+    for k, row in profile_table(path).items():
+        try:
+            out[k] = dict(valu=float(row["VALU instr / dispatch"]),
+                          fetch=float(row["fetch MB / dispatch"]) * 1e6,
+                          write=float(row["write MB / dispatch"]) * 1e6)
+        except (KeyError, ValueError):
+            continue
+    return out
+
+
+def cpu_threads():
+    """The host threads the CPU baseline uses: the CPUs this process may run
+    on, at most 16 (a one-GPU share of the GPU box's host)."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return max(1, min(16, allowed)), allowed
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(cfg, gpu_luts, budget_s, threads):
@@ -216,7 +279,320 @@ def _json_line(result):
     print(json.dumps(result), flush=True)
 
 
-def run_batched(args):
+def _progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+LOOKUP_NAMES = ("sky_luminance", "sky_luminance_to_point", "sun_and_sky_illuminance")
+
+
+class Lookups:
+    """One 1920x1080 batch of each luminance lookup (GetSkyLuminance,
+    GetSkyLuminanceToPoint, GetSunAndSkyIlluminance) on a model, inputs
+    resident on the device, each timed with HIP events on the lookup
+    stream. Rays: camera altitude 0-100 km, view direction uniform on the
+    sphere, sun zenith 0-100 deg, hit distance 0-50 km (seeded)."""
+
+    def __init__(self, atmo, configs, kw, seed=2024):
+        self.n = WIDTH * HEIGHT
+        self.q = configs.sky_queries(seed, self.n, kw["bottom_radius"] / 1000.0,
+                                     kw["top_radius"] / 1000.0)
+        self.dev = {k: atmo.DeviceArray.from_host(v) for k, v in self.q.items()}
+        self.outs = [atmo.DeviceArray((self.n, 3)) for _ in range(6)]
+        self.ev = [atmo.Event() for _ in range(4)]
+        self.ms = [[], [], []]
+
+    def run(self, model, stream):
+        d, o, ev, n = self.dev, self.outs, self.ev, self.n
+        ev[0].record(stream)
+        model.sky_luminance_device(n, d["camera"], d["view_ray"], d["shadow_length"],
+                                   d["sun_direction"], o[0], o[1], stream)
+        ev[1].record(stream)
+        model.sky_luminance_to_point_device(n, d["camera"], d["point"], d["shadow_length"],
+                                            d["sun_direction"], o[2], o[3], stream)
+        ev[2].record(stream)
+        model.sun_and_sky_illuminance_device(n, d["point"], d["normal"], d["sun_direction"],
+                                             o[4], o[5], stream)
+        ev[3].record(stream)
+
+    def collect(self):
+        for k in range(3):
+            self.ms[k].append(self.ev[k].elapsed_ms(self.ev[k + 1]))
+
+    def summary(self):
+        steps = len(self.ms[0])
+        batch_s = sum(sum(l) for l in self.ms) * 1e-3 / steps
+        ideal = max(QUERY_BYTES_PER_RAY * self.n / (PEAK_HBM_GBS * 1e9),
+                    QUERY_LANE_OPS_PER_RAY * self.n / (PEAK_VALU_TOPS * 1e12))
+        return {
+            "sky_rays_per_s": round(3.0 * self.n / batch_s, 1),
+            "lookup_ms": {k: round(sum(l) / steps, 4) for k, l in zip(LOOKUP_NAMES, self.ms)},
+            "lookup_p99_ms": {k: round(percentile(l, 0.99), 4)
+                              for k, l in zip(LOOKUP_NAMES, self.ms)},
+            # 124 B/ray of compulsory HBM traffic (the inputs and the six
+            # output vectors of the three lookups) and ~1500 VALU lane-ops per
+            # ray for the three, against the time of the three lookups.
+            "lookup_roofline_frac": round(ideal / batch_s, 4),
+        }
+
+    def free(self):
+        for d in list(self.dev.values()) + self.outs:
+            d.free()
+
+
+def pass_key(name, order):
+    if name == "scattering_density":
+        return "scattering_density_2" if order == 2 else "scattering_density_n"
+    return name
+
+
+def timed_inits(atmo, model, orders, stream, steps, lookups=None):
+    """steps x (Init, then the lookups if any); returns (precompute ms,
+    step ms, wall s, {pass: [launch ms]})."""
+    pre_ms, step_ms, launches = [], [], {}
+    atmo.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        model.init(orders, stream)
+        stream.synchronize()
+        t1 = time.perf_counter()
+        if lookups is not None:
+            lookups.run(model, stream)
+            stream.synchronize()
+        t2 = time.perf_counter()
+        pre_ms.append((t1 - t0) * 1e3)
+        step_ms.append((t2 - t0) * 1e3)
+        if lookups is not None:
+            lookups.collect()
+        if model.pass_timing:
+            for name, order, ms in model.pass_timings():
+                launches.setdefault(pass_key(name, order), []).append(ms)
+    atmo.synchronize()
+    return pre_ms, step_ms, time.perf_counter() - t_start, launches
+
+
+def kept_init_ms(atmo, kw, orders, stream, steps):
+    """Init of the same atmosphere with its temporaries kept between calls
+    (no allocation or free inside the call): mean ms over 'steps' calls."""
+    m = atmo.Model(retain_intermediates=True, **kw)
+    m.init(orders, stream)
+    stream.synchronize()
+    ms = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        m.init(orders, stream)
+        stream.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    m.close()
+    return sum(ms) / len(ms)
+
+
+def section_high_resolution(args, steps, warmup, with_lookups=True, keep_luts=False):
+    """The high-resolution stress workload: Init as the reference's API runs
+    it (temporaries allocated and freed by the call; launch times from HIP
+    events), then the three lookups; afterwards the same Init with the
+    temporaries kept between calls."""
+    import precomputed_atmospheric_scattering_amd as atmo
+    from precomputed_atmospheric_scattering_amd import configs
+
+    cfg = configs.high_resolution(half_precision=True)
+    kw, orders = configs.model_kwargs(cfg)
+    n_t, n_s, n_e = texel_counts(kw["lut_sizes"])
+    stream = atmo.Stream()
+    # Default tile shapes (the autotuner keeps them for this size class), and
+    # no tuning launches, so that a kernel profile of this run holds the
+    # workload's launches only.
+    atmo.reset_tiles()
+    tiles = atmo.get_tiles(kw["lut_sizes"])
+    m = atmo.Model(pass_timing=True, **kw)
+    lk = Lookups(atmo, configs, kw) if with_lookups else None
+    for _ in range(warmup):
+        m.init(orders, stream)
+        if lk is not None:
+            lk.run(m, stream)
+    stream.synchronize()
+    _progress("high resolution: warm-up done, %d timed steps" % steps)
+    pre_ms, step_ms, wall, launches = timed_inits(atmo, m, orders, stream, steps, lk)
+    resident, temporaries = m.footprint()
+    luts = None
+    if keep_luts:
+        luts = {k: m.lut(k) for k in ("transmittance", "scattering",
+                                      "single_mie_scattering", "irradiance")}
+    iters = m.params.num_iterations()
+    m.close()
+    kept = kept_init_ms(atmo, kw, orders, stream, max(1, min(steps, 5)))
+    stream.close()
+    res = {
+        "value": round(steps / (sum(pre_ms) * 1e-3), 4),
+        "ms_per_step": round(wall * 1e3 / steps, 3),
+        "ms_per_atmosphere": round(sum(pre_ms) / steps, 3),
+        "precompute_p50_ms": round(percentile(pre_ms, 0.5), 3),
+        "precompute_p99_ms": round(percentile(pre_ms, 0.99), 3),
+        "step_p50_ms": round(percentile(step_ms, 0.5), 3),
+        "step_p99_ms": round(percentile(step_ms, 0.99), 3),
+        "init_kept_temporaries_ms": round(kept, 3),
+        "tiles": {"scattering_density": "%d nu x %d threads" % (
+                      tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
+                  "multiple_scattering": "%d texels per thread, %s rows" % (
+                      tiles["ms_texels_per_thread"], tiles["ms_row_order"])},
+        "footprint_bytes": {"resident": resident, "init_temporaries": temporaries},
+    }
+    if lk is not None:
+        res.update(lk.summary())
+        lk.free()
+    meta = dict(cfg=cfg, steps=steps, n_s=n_s, iters=iters, orders=orders, luts=luts)
+    return res, launches, meta
+
+
+def pass_figures(launches, steps, n_s, iters):
+    """Per pass, from this run's launch times (HIP events on the Init
+    stream) and the committed per-dispatch counters of the same workload:
+    ms per atmosphere, VALU lane-op rate and HBM rate, each as a fraction of
+    its MI355X peak."""
+    cnt = counter_figures()
+    merged = {}
+    for k, v in launches.items():
+        merged.setdefault("scattering_density" if k.startswith("scattering_density") else k,
+                          []).extend(v)
+    out = {}
+    for name, ms_list in merged.items():
+        avg_s = sum(ms_list) / len(ms_list) * 1e-3
+        e = {"ms_per_atmosphere": round(sum(ms_list) / steps, 3),
+             "launches_per_atmosphere": len(ms_list) // steps,
+             "avg_launch_ms": round(avg_s * 1e3, 4)}
+        c = cnt.get(PASS_KERNELS.get(name, ""))
+        if c is not None:
+            e["valu_tops"] = round(c["valu"] * 64.0 / avg_s / 1e12, 3)
+            e["valu_frac"] = round(c["valu"] * 64.0 / avg_s / 1e12 / PEAK_VALU_TOPS, 4)
+            e["hbm_bytes_per_launch"] = round(c["fetch"] + c["write"])
+            e["hbm_gb_per_s"] = round((c["fetch"] + c["write"]) / avg_s / 1e9, 1)
+            e["hbm_frac"] = round((c["fetch"] + c["write"]) / avg_s / 1e9 / PEAK_HBM_GBS, 4)
+        out[name] = e
+    if "scattering_density" in out:
+        # The frozen per-sample estimate of the reference's work, for scale.
+        flops = sum(iters * sd_flops(n_s, 2 if k.endswith("_2") else 3) * len(v)
+                    for k, v in launches.items() if k.startswith("scattering_density"))
+        sd_s = sum(sum(v) for k, v in launches.items()
+                   if k.startswith("scattering_density")) * 1e-3
+        out["scattering_density"]["algorithmic_equivalent_tflops"] = round(
+            flops / sd_s / 1e12, 1)
+    return out
+
+
+def roofline(passes):
+    """The roofline object of the dominant precompute pass (the largest ms
+    per atmosphere among passes with counters): its bound is the larger of
+    its VALU and HBM fractions."""
+    have = {k: v for k, v in passes.items() if "valu_frac" in v}
+    if not have:
+        return None
+    name = max(have, key=lambda k: have[k]["ms_per_atmosphere"])
+    p = have[name]
+    valu = p["valu_frac"] >= p["hbm_frac"]
+    return {
+        "kernel": PASS_KERNELS[name],
+        "bound": "valu" if valu else "hbm",
+        "achieved": p["valu_tops"] if valu else p["hbm_gb_per_s"],
+        "peak": round(PEAK_VALU_TOPS, 2) if valu else PEAK_HBM_GBS,
+        "unit": "TOP/s (VALU lane-ops; FP32 peak 157.3 TFLOP/s = 78.65e12 FMA lanes/s)"
+                if valu else "GB/s",
+        "frac": p["valu_frac"] if valu else p["hbm_frac"],
+        "traffic": p["hbm_bytes_per_launch"],
+        "avg_launch_ms": p["avg_launch_ms"],
+        "valu_frac": p["valu_frac"],
+        "hbm_frac": p["hbm_frac"],
+        "counters": os.path.relpath(COUNTERS_HIGH_RES, ROOT),
+    }
+
+
+def section_spectral(args, steps, warmup):
+    """The spectral atmosphere at the reference LUT sizes (4 orders): FP32
+    and FP16 storage, Init as the API runs it and with kept temporaries, the
+    lookups, and the demo's nine preset views as coherent lookup frames."""
+    import numpy as np
+    import precomputed_atmospheric_scattering_amd as atmo
+    from precomputed_atmospheric_scattering_amd import configs, frames
+
+    out = {}
+    stream = atmo.Stream()
+    atmo.reset_tiles()
+    for half in (False, True):
+        cfg = configs.spectral(half_precision=half)
+        kw, orders = configs.model_kwargs(cfg)
+        m = atmo.Model(pass_timing=not half, **kw)
+        lk = Lookups(atmo, configs, kw) if not half else None
+        for _ in range(warmup):
+            m.init(orders, stream)
+            if lk is not None:
+                lk.run(m, stream)
+        stream.synchronize()
+        pre_ms, _, _, launches = timed_inits(atmo, m, orders, stream, steps, lk)
+        e = {"ms_per_atmosphere": round(sum(pre_ms) / steps, 4),
+             "precompute_p99_ms": round(percentile(pre_ms, 0.99), 4),
+             "init_kept_temporaries_ms": round(
+                 kept_init_ms(atmo, kw, orders, stream, steps), 4)}
+        if launches:
+            e["pass_ms_per_atmosphere"] = {k: round(sum(v) / steps, 4)
+                                           for k, v in launches.items()}
+        if lk is not None:
+            e.update(lk.summary())
+            lk.free()
+            e["demo_frames"] = demo_frames(atmo, frames, np, m, kw, stream)
+        out["fp16" if half else "fp32"] = e
+        m.close()
+    stream.close()
+    out["atmospheres_per_s"] = round(1e3 / out["fp32"]["ms_per_atmosphere"], 2)
+    out["workload"] = ("spectral atmosphere, 15 precomputed wavelengths = 5 iterations, "
+                       "4 orders, LUTs 256x64 / 32x128x32x8 / 64x16")
+    return out
+
+
+def demo_frames(atmo, frames, np, model, kw, stream, reps=3):
+    """The lookups on coherent batches: the frames of the demo's nine preset
+    views at 1920x1080 (a sphere on the planet with light shafts; views
+    from 7 km to 12000 km away). Per frame: GetSkyLuminance for every
+    pixel, GetSunAndSkyIlluminance and GetSkyLuminanceToPoint for the pixels
+    that hit the sphere or the ground."""
+    n_rays = WIDTH * HEIGHT
+    frame_ms, frame_rays = {}, {}
+    fev = [atmo.Event() for _ in range(2)]
+    outs = [atmo.DeviceArray((n_rays, 3)) for _ in range(2)]
+    for view in sorted(frames.DEMO_VIEWS):
+        fr, _ = frames.demo_frame(view, kw["bottom_radius"] / 1000.0,
+                                  kw["sun_angular_radius"], WIDTH, HEIGHT)
+        fq = fr.queries()
+        hit = {k: np.concatenate([fq["sphere"][k], fq["ground"][k]])
+               for k in ("camera", "point", "normal", "shadow", "sun")}
+        nh = len(hit["point"])
+        sky = {k: atmo.DeviceArray.from_host(fq["sky"][k])
+               for k in ("camera", "view", "shadow", "sun")}
+        hd = {k: atmo.DeviceArray.from_host(v) for k, v in hit.items()} if nh else None
+        hout = [atmo.DeviceArray((max(nh, 1), 3)) for _ in range(4)]
+        fev[0].record(stream)
+        for _ in range(reps):
+            model.sky_luminance_device(n_rays, sky["camera"], sky["view"], sky["shadow"],
+                                       sky["sun"], outs[0], outs[1], stream)
+            if nh:
+                model.sun_and_sky_illuminance_device(nh, hd["point"], hd["normal"],
+                                                     hd["sun"], hout[0], hout[1], stream)
+                model.sky_luminance_to_point_device(nh, hd["camera"], hd["point"],
+                                                    hd["shadow"], hd["sun"], hout[2],
+                                                    hout[3], stream)
+        fev[1].record(stream)
+        stream.synchronize()
+        frame_ms[view] = fev[0].elapsed_ms(fev[1]) / reps
+        frame_rays[view] = n_rays + 2 * nh
+        for d in list(sky.values()) + (list(hd.values()) if hd else []) + hout:
+            d.free()
+    for d in outs:
+        d.free()
+    total_s = sum(frame_ms.values()) * 1e-3
+    return {"frames_per_s": round(len(frame_ms) / total_s, 1),
+            "lookups_per_s": round(sum(frame_rays.values()) / total_s, 1)}
+
+
+def section_batched(args, steps, warmup):
     """64 seeded random atmospheres (radii, scale heights, coefficients,
     albedo, per-dimension LUT size factors in {0.5, 1, 2}, 2..8 orders, 3
     wavelengths, FP32): the plain one-at-a-time Init against one batched
@@ -231,18 +607,22 @@ def run_batched(args):
     orders = [o for _, o in kws]
     n_s_total = sum(texel_counts(kw["lut_sizes"])[1] for kw, _ in kws)
     stream = atmo.Stream()
-
     models = [atmo.Model(**kw) for kw, _ in kws]
     atmo.reset_tiles()  # the plain baseline runs at the default tile shapes
 
     # Plain: Model.init one model after the other, as a caller of the
     # reference's API would (temporaries allocated and freed by each Init).
-    models[0].init(orders[0], stream)
-    t0 = time.perf_counter()
     for m, o in zip(models, orders):
         m.init(o, stream)
     stream.synchronize()
-    plain_s = time.perf_counter() - t0
+    plain_runs = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        for m, o in zip(models, orders):
+            m.init(o, stream)
+        stream.synchronize()
+        plain_runs.append(time.perf_counter() - t0)
+    plain_s = min(plain_runs)
     sample = (0, 21, 42, 63)
     ref = {i: models[i].lut("scattering") for i in sample}
     # Plain with the temporaries kept between Inits (no allocation in Init).
@@ -264,22 +644,15 @@ def run_batched(args):
     t0 = time.perf_counter()
     atmo.autotune(sizes, stream=stream)
     tune_s = time.perf_counter() - t0
-    shapes = {}
-    for z in sizes:
-        t = atmo.get_tiles(z)
-        key = "sd %dx%d, ms %d%s" % (t["sd_nu_per_thread"], t["sd_block_threads"],
-                                    t["ms_texels_per_thread"],
-                                    " mu-major" if t["ms_mu_major"] else "")
-        shapes[key] = shapes.get(key, 0) + 1
 
-    def timed_batch(use_graph, steps, warmup):
+    def timed_batch(use_graph, n_steps, n_warmup):
         b = atmo.Batch(models, orders, use_graph=use_graph)
-        for _ in range(warmup):
+        for _ in range(n_warmup):
             b.launch(stream)
         stream.synchronize()
         ms = []
         t_all = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(n_steps):
             t = time.perf_counter()
             b.launch(stream)
             stream.synchronize()
@@ -289,53 +662,46 @@ def run_batched(args):
         b.close()
         return ms, total, launches
 
-    eager_ms, eager_s, _ = timed_batch(False, max(1, args.steps // 2), 1)
-    graph_ms, graph_s, launches = timed_batch(True, args.steps, max(1, args.warmup))
+    eager_ms, eager_s, _ = timed_batch(False, max(1, steps // 2), 1)
+    graph_ms, graph_s, launches = timed_batch(True, steps, max(1, warmup))
     exact = all(np.array_equal(models[i].lut("scattering"), ref[i]) for i in sample)
-    # The same batch with every size class at the default shapes.
     tuned_table = os.path.join(tempfile.mkdtemp(prefix="atmo_tiles_"), "tiles.txt")
     atmo.save_tiles(tuned_table)
     atmo.reset_tiles()
-    _, default_s, default_launches = timed_batch(True, args.steps, max(1, args.warmup))
+    _, default_s, default_launches = timed_batch(True, steps, max(1, warmup))
     exact = exact and all(np.array_equal(models[i].lut("scattering"), ref[i])
                           for i in sample)
     atmo.load_tiles(tuned_table)
-    value = 64 * args.steps / graph_s
+    for m in models:
+        m.close()
+    stream.close()
+    value = 64 * steps / graph_s
     plain = 64 / plain_s
-    _json_line({
-        "metric": "batched LUT precompute throughput (64 random atmospheres)",
-        "value": round(value, 2), "unit": "atmospheres/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(graph_s * 1e3 / args.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "64 random atmospheres from seed %d: bottom radius "
-                               "3000-7000 km, thickness 40-200 km, Rayleigh scale height "
-                               "5-15 km and beta x0.3-3, Mie scale height 0.5-3 km, g "
-                               "0.6-0.9, albedo 0-0.5, LUT sizes x{0.5,1,2} per dimension, "
-                               "2-8 orders, 3 wavelengths, FP32" % args.seed,
-                   "atmospheres": 64, "scattering_texels_total": n_s_total,
-                   "orders": orders},
+    return {
+        "workload": "64 random atmospheres from seed %d: bottom radius 3000-7000 km, "
+                    "thickness 40-200 km, Rayleigh scale height 5-15 km and beta x0.3-3, "
+                    "Mie scale height 0.5-3 km, g 0.6-0.9, albedo 0-0.5, LUT sizes "
+                    "x{0.5,1,2} per dimension, 2-8 orders, 3 wavelengths, FP32" % args.seed,
+        "scattering_texels_total": n_s_total,
+        "atmospheres_per_s": round(value, 2),
         "batch_p50_ms": round(percentile(graph_ms, 0.5), 3),
         "batch_p99_ms": round(percentile(graph_ms, 0.99), 3),
         "launches_per_batch": launches,
         "autotune_s": round(tune_s, 3),
-        "tile_shapes": shapes,
-        "default_tiles_atmospheres_per_s": round(64 * args.steps / default_s, 2),
+        "default_tiles_atmospheres_per_s": round(64 * steps / default_s, 2),
         "default_tiles_launches_per_batch": default_launches,
         "plain_atmospheres_per_s": round(plain, 2),
         "plain_kept_temporaries_atmospheres_per_s": round(64 / plain_kept_s, 2),
         "batched_eager_atmospheres_per_s": round(64 * len(eager_ms) / eager_s, 2),
-        "speedup_vs_plain": round(value / plain, 2),
-        "speedup_vs_plain_kept_temporaries": round(value * plain_kept_s / 64, 2),
+        "speedup_vs_plain": round(value / plain, 3),
+        "speedup_vs_plain_kept_temporaries": round(value * plain_kept_s / 64, 3),
         "bit_exact_vs_plain": {"models_checked": list(sample), "equal": exact},
-    })
-    for m in models:
-        m.close()
+    }
 
 
-def run_serving(args):
+def section_serving(args, rate, duration, seed):
     """Serving loop under a Poisson stream of precompute requests (random
-    atmospheres of the batched workload) at --rate per second for --duration
+    atmospheres of the batched workload) at 'rate' per second for 'duration'
     seconds. Each finished model then serves one 1920x1080 query batch
     (GetSkyRadiance, GetSkyRadianceToPoint, GetSunAndSkyIrradiance; inputs
     made on the device from a seed: camera altitude 0-100 km, view direction
@@ -343,11 +709,11 @@ def run_serving(args):
     import precomputed_atmospheric_scattering_amd as atmo
     from precomputed_atmospheric_scattering_amd import _capi, configs
 
-    rng = configs.SplitMix64(args.seed)
+    rng = configs.SplitMix64(seed)
     arrivals, t = [], 0.0
     while True:
-        t += -math.log(1.0 - rng.uniform()) / args.rate
-        if t > args.duration:
+        t += -math.log(1.0 - rng.uniform()) / rate
+        if t > duration:
             break
         arrivals.append((t, configs.random_atmosphere(rng)))
     n = WIDTH * HEIGHT
@@ -364,14 +730,14 @@ def run_serving(args):
     tune_s = time.perf_counter() - t0
     srv = atmo.Server(max_inflight_batches=3, max_batch_models=16, lookup_streams=2)
 
-    # Warm-up: one request and one query batch, not counted.
-    kw, o = configs.model_kwargs(configs.random_atmosphere(configs.SplitMix64(args.seed + 1)))
+    # Warm-up: one request, not counted.
+    kw, o = configs.model_kwargs(configs.random_atmosphere(configs.SplitMix64(seed + 1)))
     wid = srv.submit_precompute(num_scattering_orders=o, **kw)
     srv.wait([wid])
     srv.release(wid)
 
     info, ready, batches = {}, [], {}
-    pre_lat, qry_lat, batch_sizes = [], [], []
+    pre, qry, batch_sizes = [], [], []
     failed = 0
     i = 0
     start = time.perf_counter()
@@ -390,7 +756,7 @@ def run_serving(args):
                 failed += 1
             if c["kind"] == _capi.REQUEST_PRECOMPUTE:
                 outstanding -= 1
-                pre_lat.append(c["end_ms"] - c["submit_ms"])
+                pre.append(c)
                 batch_sizes.append(c["batch_models"])
                 if c["status"] == 0:
                     ready.append(c["request_id"])
@@ -398,9 +764,10 @@ def run_serving(args):
                 b = batches[c["model_id"]]
                 b["left"] -= 1
                 b["end"] = max(b["end"], c["end_ms"])
+                b["start"] = min(b["start"], c["start_ms"])
                 b["submit"] = min(b["submit"], c["submit_ms"])
                 if b["left"] == 0:
-                    qry_lat.append(b["end"] - b["submit"])
+                    qry.append(b)
                     srv.release(c["model_id"])
                     pool.append(b["buffers"])
                     del batches[c["model_id"]]
@@ -412,123 +779,134 @@ def run_serving(args):
             rb, rt = kw["bottom_radius"] / 1000.0, kw["top_radius"] / 1000.0
             atmo.generate_sky_queries(mid, n, rb, rt, out=q, stream=gen)
             gen.synchronize()
-            ids = [srv.submit_lookup(mid, "sky", n, [q["camera"], q["view_ray"],
-                                                      q["shadow_length"], q["sun_direction"]],
-                                     o[0:2]),
-                   srv.submit_lookup(mid, "sky_to_point", n,
-                                     [q["camera"], q["point"], q["shadow_length"],
-                                      q["sun_direction"]], o[2:4]),
-                   srv.submit_lookup(mid, "sun_and_sky", n,
-                                     [q["point"], q["normal"], q["sun_direction"]], o[4:6])]
-            assert len(ids) == 3
-            batches[mid] = dict(left=3, end=0.0, submit=float("inf"), buffers=(q, o))
+            srv.submit_lookup(mid, "sky", n, [q["camera"], q["view_ray"],
+                                              q["shadow_length"], q["sun_direction"]],
+                              o[0:2])
+            srv.submit_lookup(mid, "sky_to_point", n,
+                              [q["camera"], q["point"], q["shadow_length"],
+                               q["sun_direction"]], o[2:4])
+            srv.submit_lookup(mid, "sun_and_sky", n,
+                              [q["point"], q["normal"], q["sun_direction"]], o[4:6])
+            batches[mid] = dict(left=3, end=0.0, start=float("inf"), submit=float("inf"),
+                                buffers=(q, o))
             outstanding += 1
         peak_resident = max(peak_resident, srv.stats()["resident_bytes"])
     wall = time.perf_counter() - start
     st = srv.stats()
     srv.close()
-    done = len(pre_lat)
-    _json_line({
-        "metric": "served precompute latency p50/p99 and throughput (Poisson arrivals)",
-        "value": round(done / wall, 3), "unit": "atmospheres/s", "n_gpus": 1,
-        "steps": done, "warmup": 1, "ms_per_step": round(wall * 1e3 / max(done, 1), 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "Poisson %.1f req/s for %.1f s of random atmospheres "
-                               "(batched-workload distribution, seed %d), one 1920x1080 "
-                               "query batch of 3 lookups per finished model"
-                               % (args.rate, args.duration, args.seed),
-                   "requests": len(arrivals), "query_rays_per_batch": n},
+    gen.close()
+    for q, o in pool:
+        for d in list(q.values()) + o:
+            d.free()
+    pre_lat = [c["end_ms"] - c["submit_ms"] for c in pre]
+    qry_lat = [b["end"] - b["submit"] for b in qry]
+    # Where the query-batch tail comes from: batches whose device work
+    # overlapped a precompute batch's (they share the GPU, at the same stream
+    # priority) against those that ran alone; and the wait from submission
+    # to the start of the first lookup.
+    busy = [(c["start_ms"], c["end_ms"]) for c in pre]
+    over = [b["end"] - b["submit"] for b in qry
+            if any(s < b["end"] and e > b["start"] for s, e in busy)]
+    alone = [b["end"] - b["submit"] for b in qry
+             if not any(s < b["end"] and e > b["start"] for s, e in busy)]
+    return {
+        "workload": "Poisson %.1f req/s for %.2f s of random atmospheres (batched-workload "
+                    "distribution, seed %d), one 1920x1080 query batch of 3 lookups per "
+                    "finished model" % (rate, duration, seed),
+        "requests": len(arrivals),
+        "atmospheres_per_s": round(len(pre) / wall, 3),
         "precompute_latency_p50_ms": round(percentile(pre_lat, 0.5), 3),
         "precompute_latency_p99_ms": round(percentile(pre_lat, 0.99), 3),
         "query_batch_latency_p50_ms": round(percentile(qry_lat, 0.5), 3),
         "query_batch_latency_p99_ms": round(percentile(qry_lat, 0.99), 3),
-        "sky_rays_per_s": round(3.0 * n * len(qry_lat) / wall, 1),
+        "query_batches_overlapping_precompute": len(over),
+        "query_batch_p99_ms_overlapping_precompute": round(percentile(over, 0.99), 3),
+        "query_batch_p99_ms_alone": round(percentile(alone, 0.99), 3),
+        "query_wait_to_start_p99_ms": round(percentile(
+            [b["start"] - b["submit"] for b in qry], 0.99), 3),
+        "sky_rays_per_s": round(3.0 * n * len(qry) / wall, 1),
         "mean_batch_models": round(sum(batch_sizes) / max(1, len(batch_sizes)), 2),
         "max_batch_models": max(batch_sizes) if batch_sizes else 0,
         "failed": failed, "peak_resident_bytes": peak_resident,
         "budget_bytes": st["budget_bytes"], "autotune_s": round(tune_s, 3),
-    })
-
-
-def run_high_resolution(args):
-    """High-resolution stress: scattering LUT 64x256x64x16, transmittance
-    512x128, irradiance 128x32, the spectral atmosphere with ozone (15
-    wavelengths), 8 scattering orders, FP16 storage with FP32 accumulation."""
-    import precomputed_atmospheric_scattering_amd as atmo
-    from precomputed_atmospheric_scattering_amd import configs
-
-    cfg = configs.high_resolution(half_precision=True)
-    kw, orders = configs.model_kwargs(cfg)
-    n_t, n_s, n_e = texel_counts(kw["lut_sizes"])
-    stream = atmo.Stream()
-    # Default tile shapes (the autotuner keeps them for this size class), and
-    # no tuning launches, so that a kernel profile of this run holds the
-    # workload's launches only.
-    atmo.reset_tiles()
-    tiles = atmo.get_tiles(kw["lut_sizes"])
-    m = atmo.Model(retain_intermediates=True, pass_timing=True, **kw)
-    steps = max(1, min(args.steps, 5))
-    for _ in range(max(1, min(args.warmup, 2))):
-        m.init(orders, stream)
-    stream.synchronize()
-    ms, pass_ms, pass_n = [], {}, {}
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        m.init(orders, stream)
-        stream.synchronize()
-        ms.append((time.perf_counter() - t0) * 1e3)
-        for name, order, t in m.pass_timings():
-            key = name if name not in ("scattering_density",) else (
-                "scattering_density_2" if order == 2 else "scattering_density_n")
-            pass_ms[key] = pass_ms.get(key, 0.0) + t
-            pass_n[key] = pass_n.get(key, 0) + 1
-    # Compulsory bytes per launch (FP32 float4 texels of the delta LUTs),
-    # each launch covering the 5 Precompute iterations: SD order 2 reads
-    # delta Rayleigh + Mie + irradiance and writes the density; SD order >= 3
-    # reads the multiple-scattering delta; MS reads the density and
-    # transmittance, writes its delta and updates the iteration's FP64
-    # scattering accumulator (32 B read + 32 B write per texel).
-    tex = 16.0
-    iters = m.params.num_iterations()
-    bytes_per = {
-        "scattering_density_2": iters * (3 * n_s + n_e) * tex,
-        "scattering_density_n": iters * (2 * n_s + n_e) * tex,
-        "multiple_scattering": iters * ((2 * n_s + n_t) * tex + 64.0 * n_s),
     }
-    gbs = {k: round(bytes_per[k] * pass_n[k] / (pass_ms[k] * 1e-3) / 1e9, 1)
-           for k in bytes_per if k in pass_ms}
-    _json_line({
-        "metric": "high-resolution LUT precompute ms/atmosphere",
-        "value": round(sum(ms) / steps, 3), "unit": "ms/atmosphere", "n_gpus": 1,
-        "steps": steps, "warmup": args.warmup, "ms_per_step": round(sum(ms) / steps, 3),
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+
+
+def run_headline(args):
+    res, launches, meta = section_high_resolution(
+        args, args.steps, args.warmup, keep_luts=not args.no_cpu_baseline)
+    passes = pass_figures(launches, meta["steps"], meta["n_s"], meta["iters"])
+    result = {
+        "metric": METRIC,
+        "value": res.pop("value"),
+        "unit": "atmospheres/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res.pop("ms_per_step"),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "spectral atmosphere with ozone, 15 precomputed "
-                               "wavelengths, 8 scattering orders, FP16 LUT storage with "
-                               "FP32 accumulation, LUTs 512x128 / 64x256x64x16 / 128x32",
-                   "scattering_texels": n_s, "storage": "f16"},
-        "p50_ms": round(percentile(ms, 0.5), 3), "p99_ms": round(percentile(ms, 0.99), 3),
-        "pass_ms_per_atmosphere": {k: round(v / steps, 3) for k, v in pass_ms.items()},
-        "compulsory_gb_per_s": gbs,
-        "tiles": {"scattering_density": "%d nu x %d threads" % (
-                      tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
-                  "multiple_scattering": "%d texels per thread, %s rows" % (
-                      tiles["ms_texels_per_thread"],
-                      "mu-major" if tiles["ms_mu_major"] else "r-major")},
-        "footprint_bytes": dict(zip(("resident", "scratch"), m.footprint())),
-    })
-    m.close()
+        "config": {
+            "workload": "high-resolution stress: spectral atmosphere (47-band 5778 K solar "
+                        "spectrum, ozone), 15 precomputed wavelengths = 5 iterations, 8 "
+                        "scattering orders, FP16 LUT storage with FP32 accumulation, LUTs "
+                        "512x128 / 64x256x64x16 / 128x32; Init as the API runs it "
+                        "(temporaries allocated and freed by the call), then 1920x1080 rays "
+                        "of each of GetSkyLuminance, GetSkyLuminanceToPoint, "
+                        "GetSunAndSkyIlluminance",
+            "precomputed_wavelengths": 15, "scattering_orders": meta["orders"],
+            "scattering_texels": meta["n_s"], "storage": "f16", "query_rays": WIDTH * HEIGHT,
+        },
+    }
+    result.update(res)
+    result["passes"] = passes
+    result["roofline"] = roofline(passes)
+    _progress("spectral section")
+    result["spectral"] = section_spectral(args, max(3, args.steps // 2), max(1, args.warmup))
+    _progress("batched section")
+    result["batched"] = section_batched(args, max(3, args.steps // 2), max(1, args.warmup))
+    _progress("serving section (Poisson 20/s)")
+    result["serving"] = section_serving(args, args.rate, args.duration, args.seed)
+    _progress("serving section (saturating burst)")
+    result["serving_saturated"] = section_serving(args, 2000.0, 0.25, args.seed + 7)
+    if not args.no_cpu_baseline:
+        _progress("cpu baseline")
+        threads, allowed = cpu_threads()
+        base, _ = cpu_baseline(meta["cfg"], meta["luts"], args.cpu_budget, threads)
+        from precomputed_atmospheric_scattering_amd import configs
+        kw, _ = configs.model_kwargs(meta["cfg"])
+        q = configs.sky_queries(2024, 20000, kw["bottom_radius"] / 1000.0,
+                                kw["top_radius"] / 1000.0)
+        base["sky_rays_per_s"] = round(cpu_query_baseline(meta["cfg"], meta["luts"], q,
+                                                          threads), 1)
+        base["value"] = round(base["value"], 8)
+        base["cpu_model"] = cpu_model()
+        base["host_cpus"] = os.cpu_count()
+        base["allowed_cpus"] = allowed
+        result["cpu_baseline"] = base
+        result["speedup_vs_cpu"] = round(result["value"] / base["value"], 1)
+    _json_line(result)
+
+
+def one_line(metric, value, unit, steps, warmup, ms_per_step, higher, workload, extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+         "warmup": warmup, "ms_per_step": ms_per_step, "higher_is_better": higher,
+         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+         "config": {"workload": workload}}
+    d.update(extra)
+    _json_line(d)
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--workload", default="spectral",
-                    choices=("spectral", "batched", "serving", "high-resolution"),
-                    help="spectral (default): the spectral atmosphere's line; batched: 64 seeded "
-                         "random atmospheres, batched vs one at a time; serving: Poisson "
-                         "precompute requests + query batches through the serving loop; "
-                         "high-resolution: 64x256x64x16 LUT, FP16, 8 orders")
+    ap.add_argument("--workload", default="headline",
+                    choices=("headline", "high-resolution", "spectral", "batched", "serving"),
+                    help="headline (default): the high-resolution workload's line with the "
+                         "spectral, batched and serving sections; the others run one section "
+                         "alone")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--duration", type=float, default=10.0,
                     help="serving: seconds of request arrivals")
@@ -537,6 +915,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-lookups", action="store_true",
+                    help="high-resolution: Init only (for counter runs)")
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU oracle work for the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -552,7 +932,6 @@ def main():
 
     import numpy as np
     import precomputed_atmospheric_scattering_amd as atmo
-    from precomputed_atmospheric_scattering_amd import configs, frames
 
     if args.copy_probe > 0:
         nbytes = args.copy_probe << 20
@@ -563,220 +942,34 @@ def main():
         src.free()
         dst.free()
 
-    if args.workload == "batched":
-        return run_batched(args)
-    if args.workload == "serving":
-        return run_serving(args)
+    if args.workload == "headline":
+        return run_headline(args)
     if args.workload == "high-resolution":
-        return run_high_resolution(args)
-
-    cfg = configs.spectral(half_precision=False)
-    kw, orders = configs.model_kwargs(cfg)
-    sizes = kw["lut_sizes"]
-    n_t, n_s, n_e = texel_counts(sizes)
-    stream = atmo.Stream()
-    # Default tile shapes (no tuning launches in this run, so that a kernel
-    # profile of it holds the workload's launches only).
-    atmo.reset_tiles()
-    tiles = atmo.get_tiles(sizes)
-    model = atmo.Model(retain_intermediates=True, pass_timing=True, **kw)
-    n_rays = WIDTH * HEIGHT
-    q = configs.sky_queries(2024, n_rays, kw["bottom_radius"] / 1000.0,
-                            kw["top_radius"] / 1000.0)
-    dev = {k: atmo.DeviceArray.from_host(v) for k, v in q.items()}
-    outs = [atmo.DeviceArray((n_rays, 3)) for _ in range(6)]
-    ev = [atmo.Event() for _ in range(4)]
-
-    # 15 precomputed wavelengths: the LUTs hold luminance, so the lookups are
-    # the luminance API (GetSkyLuminance, GetSkyLuminanceToPoint,
-    # GetSunAndSkyIlluminance), which differ from the radiance ones only by a
-    # per-channel factor.
-    def run_queries():
-        ev[0].record(stream)
-        model.sky_luminance_device(n_rays, dev["camera"], dev["view_ray"],
-                                   dev["shadow_length"], dev["sun_direction"],
-                                   outs[0], outs[1], stream)
-        ev[1].record(stream)
-        model.sky_luminance_to_point_device(n_rays, dev["camera"], dev["point"],
-                                            dev["shadow_length"], dev["sun_direction"],
-                                            outs[2], outs[3], stream)
-        ev[2].record(stream)
-        model.sun_and_sky_illuminance_device(n_rays, dev["point"], dev["normal"],
-                                             dev["sun_direction"], outs[4], outs[5], stream)
-        ev[3].record(stream)
-
-    for _ in range(args.warmup):
-        model.init(orders, stream)
-        run_queries()
-    stream.synchronize()
-
-    pre_ms, qry_ms, step_ms, lookup_ms = [], [], [], [[], [], []]
-    sd_ms, sd_flops_total, pass_ms = 0.0, 0.0, {}
-    sd_launches = 0
-    # One launch of a pass covers every Precompute iteration (each of N_S
-    # texels).
-    iters = model.params.num_iterations()
-    atmo.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        model.init(orders, stream)
-        stream.synchronize()
-        t1 = time.perf_counter()
-        run_queries()
-        stream.synchronize()
-        t2 = time.perf_counter()
-        pre_ms.append((t1 - t0) * 1e3)
-        qry_ms.append((t2 - t1) * 1e3)
-        step_ms.append((t2 - t0) * 1e3)
-        for k in range(3):
-            lookup_ms[k].append(ev[k].elapsed_ms(ev[k + 1]))
-        for name, order, ms in model.pass_timings():
-            pass_ms[name] = pass_ms.get(name, 0.0) + ms
-            if name == "scattering_density":
-                sd_ms += ms
-                sd_flops_total += iters * sd_flops(n_s, order)
-                sd_launches += 1
-    atmo.synchronize()
-    elapsed = time.perf_counter() - t_start
-
-    # The same lookups on coherent batches, outside the timed steps: the
-    # frames of the demo's nine preset views at 1920x1080 (the scene of the
-    # reference's demo, a sphere on the planet with light shafts; views from
-    # 7 km to 12000 km away). Per frame: GetSkyLuminance for every pixel,
-    # GetSunAndSkyIlluminance and GetSkyLuminanceToPoint for the pixels
-    # that hit the sphere or the ground.
-    frame_ms = {}
-    frame_rays = {}
-    fev = [atmo.Event() for _ in range(2)]
-    frame_reps = 5
-    for view in sorted(frames.DEMO_VIEWS):
-        fr, _ = frames.demo_frame(view, kw["bottom_radius"] / 1000.0,
-                                  kw["sun_angular_radius"], WIDTH, HEIGHT)
-        fq = fr.queries()
-        hit = {k: np.concatenate([fq["sphere"][k], fq["ground"][k]])
-               for k in ("camera", "point", "normal", "shadow", "sun")}
-        nh = len(hit["point"])
-        sky = {k: atmo.DeviceArray.from_host(fq["sky"][k])
-               for k in ("camera", "view", "shadow", "sun")}
-        hd = {k: atmo.DeviceArray.from_host(v) for k, v in hit.items()} if nh else None
-        # A pixel can hit both the sphere and the ground behind it.
-        hout = [atmo.DeviceArray((max(nh, 1), 3)) for _ in range(4)]
-        fev[0].record(stream)
-        for _ in range(frame_reps):
-            model.sky_luminance_device(n_rays, sky["camera"], sky["view"], sky["shadow"],
-                                       sky["sun"], outs[0], outs[1], stream)
-            if nh:
-                model.sun_and_sky_illuminance_device(nh, hd["point"], hd["normal"],
-                                                     hd["sun"], hout[0], hout[1], stream)
-                model.sky_luminance_to_point_device(nh, hd["camera"], hd["point"],
-                                                    hd["shadow"], hd["sun"], hout[2],
-                                                    hout[3], stream)
-        fev[1].record(stream)
-        stream.synchronize()
-        frame_ms[view] = fev[0].elapsed_ms(fev[1]) / frame_reps
-        frame_rays[view] = n_rays + 2 * nh
-        for d in list(sky.values()) + (list(hd.values()) if hd else []) + hout:
-            d.free()
-
-    # The same atmosphere with FP16 LUT storage (FP32 accumulation, then the
-    # per-channel maxima and the FP16 conversion), after the timed steps and
-    # outside them: Init time only, with the temporaries kept between calls
-    # as for the FP32 model above.
-    kw16, _ = configs.model_kwargs(configs.spectral(half_precision=True))
-    m16 = atmo.Model(retain_intermediates=True, **kw16)
-    for _ in range(args.warmup):
-        m16.init(orders, stream)
-    stream.synchronize()
-    half_ms = []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        m16.init(orders, stream)
-        stream.synchronize()
-        half_ms.append((time.perf_counter() - t0) * 1e3)
-    m16.close()
-
-    steps = args.steps
-    sd_avg_ms = sd_ms / sd_launches
-    sd_traffic, traffic_src = profiled_traffic("k_scattering_density")
-    achieved_tflops = sd_flops_total / (sd_ms * 1e-3) / 1e12
-    ms_per_atmo = sum(pre_ms) / steps
-    query_s = sum(sum(l) for l in lookup_ms) * 1e-3
-    rays_per_s = 3.0 * n_rays * steps / query_s
-    q_ideal = max(QUERY_BYTES_PER_RAY * n_rays / (PEAK_HBM_GBS * 1e9),
-                  QUERY_LANE_OPS_PER_RAY * n_rays / (PEAK_FP32_TFLOPS / 2 * 1e12))
-    q_frac = q_ideal / (query_s / steps)
-
-    result = {
-        "metric": METRIC,
-        "value": round(steps / (sum(pre_ms) * 1e-3), 3),
-        "unit": "atmospheres/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed * 1e3 / steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": "spectral atmosphere (47-band 5778 K solar spectrum, ozone), "
-                        "15 precomputed wavelengths = 5 iterations, 4 scattering orders, "
-                        "FP32 LUTs 256x64 / 32x128x32x8 / 64x16; then 1920x1080 rays of "
-                        "each of GetSkyLuminance, GetSkyLuminanceToPoint, "
-                        "GetSunAndSkyIlluminance",
-            "precomputed_wavelengths": 15, "scattering_orders": orders,
-            "scattering_texels": n_s, "query_rays": n_rays,
-        },
-        "ms_per_atmosphere": round(ms_per_atmo, 4),
-        "precompute_p50_ms": round(percentile(pre_ms, 0.5), 4),
-        "precompute_p99_ms": round(percentile(pre_ms, 0.99), 4),
-        "step_p50_ms": round(percentile(step_ms, 0.5), 4),
-        "step_p99_ms": round(percentile(step_ms, 0.99), 4),
-        "sky_rays_per_s": round(rays_per_s, 1),
-        "lookup_ms": {n: round(sum(l) / steps, 4) for n, l in zip(
-            ("sky_luminance", "sky_luminance_to_point", "sun_and_sky_illuminance"),
-            lookup_ms)},
-        "lookup_roofline_frac": round(q_frac, 4),
-        "demo_frames": {
-            "frames_per_s": round(len(frame_ms) / (sum(frame_ms.values()) * 1e-3), 1),
-            "lookups_per_s": round(sum(frame_rays.values()) /
-                                   (sum(frame_ms.values()) * 1e-3), 1),
-            "ms_per_frame": {str(v): round(t, 4) for v, t in frame_ms.items()},
-            "lookups_per_frame": {str(v): n for v, n in frame_rays.items()},
-        },
-        "pass_ms_per_atmosphere": {k: round(v / steps, 4) for k, v in pass_ms.items()},
-        "fp16_storage": {"ms_per_atmosphere": round(sum(half_ms) / len(half_ms), 4),
-                         "precompute_p99_ms": round(percentile(half_ms, 0.99), 4)},
-        "tiles": {"scattering_density": "%d nu x %d threads" % (
-                      tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
-                  "multiple_scattering": "%d texels per thread, %s rows" % (
-                      tiles["ms_texels_per_thread"],
-                      "mu-major" if tiles["ms_mu_major"] else "r-major")},
-        "roofline": {
-            "kernel": "k_scattering_density",
-            "bound": "valu",
-            "achieved": round(achieved_tflops, 3),
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-            "traffic": sd_traffic,
-            "traffic_source": traffic_src,
-            "avg_launch_ms": round(sd_avg_ms, 5),
-            "algorithmic_flops_per_launch": round(sd_flops_total / sd_launches),
-        },
-    }
-    if not args.no_cpu_baseline:
-        luts = {k: model.lut(k) for k in ("transmittance", "scattering",
-                                          "single_mie_scattering", "irradiance")}
-        threads = min(16, os.cpu_count() or 1)
-        base, cpu_s = cpu_baseline(cfg, luts, args.cpu_budget, threads)
-        base["sky_rays_per_s"] = round(cpu_query_baseline(cfg, luts, q, threads), 1)
-        base["value"] = round(base["value"], 8)
-        result["cpu_baseline"] = base
-        result["speedup_vs_cpu"] = round(result["value"] / base["value"], 1)
-    print(json.dumps(result), flush=True)
+        res, launches, meta = section_high_resolution(args, args.steps, args.warmup,
+                                                      with_lookups=not args.no_lookups)
+        extra = dict(res)
+        extra.pop("value")
+        extra.pop("ms_per_step")
+        extra["passes"] = pass_figures(launches, meta["steps"], meta["n_s"], meta["iters"])
+        return one_line("high-resolution LUT precompute", res["value"], "atmospheres/s",
+                        args.steps, args.warmup, res["ms_per_step"], True,
+                        "high-resolution stress (64x256x64x16, 15 wavelengths, 8 orders, "
+                        "FP16 storage)", extra)
+    if args.workload == "spectral":
+        res = section_spectral(args, args.steps, args.warmup)
+        return one_line("spectral LUT precompute", res["atmospheres_per_s"], "atmospheres/s",
+                        args.steps, args.warmup, res["fp32"]["ms_per_atmosphere"], True,
+                        res.pop("workload"), res)
+    if args.workload == "batched":
+        res = section_batched(args, args.steps, args.warmup)
+        return one_line("batched LUT precompute throughput (64 random atmospheres)",
+                        res["atmospheres_per_s"], "atmospheres/s", args.steps, args.warmup,
+                        res["batch_p50_ms"], True, res.pop("workload"), res)
+    res = section_serving(args, args.rate, args.duration, args.seed)
+    return one_line("served precompute latency p50/p99 and throughput (Poisson arrivals)",
+                    res["atmospheres_per_s"], "atmospheres/s", res["requests"], 1,
+                    round(1e3 / max(res["atmospheres_per_s"], 1e-9), 3), True,
+                    res.pop("workload"), res)
 
 
 if __name__ == "__main__":

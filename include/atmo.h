@@ -110,8 +110,9 @@ typedef enum atmo_lut_id {
 
 typedef enum atmo_dtype { ATMO_DTYPE_F32 = 0, ATMO_DTYPE_F16 = 1 } atmo_dtype;
 
-/* A LUT in device memory: width x height x depth texels of 4 channels (RGBA),
- * x fastest, in the GL texture layout (the packed scattering LUT has
+/* A LUT in device memory: width x height x depth texels of 'channels'
+ * channels (4, RGBA; 3, RGB, for the delta scattering density), x fastest,
+ * in the GL texture layout (the packed scattering LUT has
  * x = nu_index * MU_S + mu_s_index, y = mu_index, z = r_index). A stored
  * value equals the physical value times scale[c] (scale is 1 for F32). */
 typedef struct atmo_lut_view {
@@ -204,21 +205,24 @@ void atmo_batch_destroy(atmo_batch* batch);
  * passes are fragment-shader draws). The scattering-density pass runs with
  * sd_nu_per_thread nu outputs per thread in blocks of sd_block_threads; the
  * multiple-scattering pass with ms_texels_per_thread texels per thread,
- * its blocks walking the (r, mu) rows of the LUT r-major (ms_mu_major 0, the
- * LUT's own order) or mu-major (1).
+ * its blocks walking the (r, mu) rows of the LUT r-major (ms_row_order 0, the
+ * LUT's own order), mu-major (1) or in ray order (2: rows sorted by the
+ * impact parameter r sqrt(1 - mu^2) of their view ray, so that blocks that
+ * run together read the same source rows).
  * Every shape computes each texel identically: the LUTs do not depend on
  * the shapes. Shapes are chosen per LUT size class, (MU, NU) for the
  * density pass and (R, MU, MU_S, NU) for multiple scattering, in a table
  * shared by the process; a model's Init, batch or serving request reads it
  * when its job table is built. Classes never tuned or set use the defaults
  * (4 nu per thread in blocks of 128 threads when MU x ceil(NU / 4) <= 256,
- * else 256; the fewest texels per thread that hold a row; mu-major rows
- * once the R x MU x MU_S x NU float4 LUT reaches 64 MiB). */
+ * else 256; the fewest texels per thread that hold a row; ray order once
+ * the R x MU x MU_S x NU RGB float density LUT reaches 64 MiB, r-major
+ * below). */
 typedef struct atmo_tiles {
   int sd_nu_per_thread;
   int sd_block_threads;
   int ms_texels_per_thread;
-  int ms_mu_major;
+  int ms_row_order;
   int tuned; /* atmo_get_tiles: bit 0 density class, bit 1 multiple class set */
 } atmo_tiles;
 /* Candidate shapes (returns the count; fills up to capacity entries). */
@@ -234,7 +238,7 @@ atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
                           void* stream);
 atmo_status atmo_get_tiles(const atmo_lut_sizes* sizes, atmo_tiles* out);
 /* Pins the shapes of the size classes of 'sizes' (the tuned field is
- * ignored); an error if a shape is not a candidate, ms_mu_major is not 0 or
+ * ignored); an error if a shape is not a candidate, ms_row_order is not 0 or
  * 1, or if NU x MU_S texels
  * per row exceed 256 x ms_texels_per_thread (rows of at most 1024 texels;
  * longer rows take a gather kernel whatever the shape). */

@@ -283,6 +283,7 @@ class Model:
         if retain_intermediates:
             check(lib().atmo_model_set_option(h, _capi.OPTION_RETAIN_INTERMEDIATES, 1),
                   "set_option")
+        self.pass_timing = bool(pass_timing)
         if pass_timing:
             check(lib().atmo_model_set_option(h, _capi.OPTION_PASS_TIMING, 1),
                   "set_option")
@@ -688,9 +689,13 @@ def synchronize():
 # Tile shapes of the scattering-density and multiple-scattering passes, and
 # their autotuner (atmo_autotune and friends). A shape is
 # {"sd_nu_per_thread", "sd_block_threads", "ms_texels_per_thread",
-# "ms_mu_major"}; sizes are
-# LUT-size dicts (DEFAULT_SIZES keys; missing keys take the defaults).
+# "ms_row_order"} (row order: MS_ROW_ORDERS); sizes are LUT-size dicts
+# (DEFAULT_SIZES keys; missing keys take the defaults).
 # ---------------------------------------------------------------------------
+
+# Multiple-scattering row orders (atmo_tiles.ms_row_order).
+MS_ROW_ORDERS = ("r-major", "mu-major", "ray")
+
 
 def _sizes(sizes):
     z = dict(DEFAULT_SIZES)
@@ -726,17 +731,18 @@ def get_tiles(sizes=None):
     check(lib().atmo_get_tiles(C.byref(z), C.byref(t)), "atmo_get_tiles")
     return dict(sd_nu_per_thread=t.sd_nu_per_thread, sd_block_threads=t.sd_block_threads,
                 ms_texels_per_thread=t.ms_texels_per_thread,
-                ms_mu_major=bool(t.ms_mu_major), sd_tuned=bool(t.tuned & 1), ms_tuned=bool(t.tuned & 2))
+                ms_row_order=MS_ROW_ORDERS[t.ms_row_order], sd_tuned=bool(t.tuned & 1),
+                ms_tuned=bool(t.tuned & 2))
 
 
 def set_tiles(sizes, sd_nu_per_thread, sd_block_threads, ms_texels_per_thread,
-              ms_mu_major=None):
-    """Pins the shapes of the size classes of 'sizes'; ms_mu_major None
-    keeps the class's current row order."""
-    if ms_mu_major is None:
-        ms_mu_major = get_tiles(sizes)["ms_mu_major"]
+              ms_row_order=None):
+    """Pins the shapes of the size classes of 'sizes'; ms_row_order (a name
+    of MS_ROW_ORDERS) None keeps the class's current row order."""
+    if ms_row_order is None:
+        ms_row_order = get_tiles(sizes)["ms_row_order"]
     t = _capi.Tiles(sd_nu_per_thread, sd_block_threads, ms_texels_per_thread,
-                    int(bool(ms_mu_major)), 0)
+                    MS_ROW_ORDERS.index(ms_row_order), 0)
     z = _sizes(sizes)
     check(lib().atmo_set_tiles(C.byref(z), C.byref(t)), "atmo_set_tiles")
 

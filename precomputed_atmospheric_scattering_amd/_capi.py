@@ -112,7 +112,7 @@ class ServerStats(C.Structure):
 
 class Tiles(C.Structure):
     _fields_ = [("sd_nu_per_thread", C.c_int), ("sd_block_threads", C.c_int),
-                ("ms_texels_per_thread", C.c_int), ("ms_mu_major", C.c_int),
+                ("ms_texels_per_thread", C.c_int), ("ms_row_order", C.c_int),
                 ("tuned", C.c_int)]
 
 

@@ -89,8 +89,11 @@ struct Job {
   // block counts and launch grouping; the kernels are instantiated per shape.
   int sd_tile, ms_tile;
   // Order in which the multiple-scattering blocks walk the packed (r, mu)
-  // rows: 0 r-major (the LUT's own order), 1 mu-major (ms_row).
-  int ms_mu_major;
+  // rows (ms_row): 0 r-major (the LUT's own order), 1 mu-major, 2 ray
+  // order, in which row p of the walk is ms_rows[p] = (z << 16) | y (the
+  // model's rows sorted by the impact parameter of their view ray).
+  int ms_row_order;
+  const int* ms_rows;
   // LUT sizes.
   int tw, th;
   int nr, nmu, nmus, nnu;
@@ -103,7 +106,9 @@ struct Job {
   float4* dE;
   float4* dR;  // delta Rayleigh; delta multiple scattering from order 2 on
   float4* dM;
-  float4* dJ;
+  // Delta scattering density: RGB texels (12 bytes; the reference's
+  // GL_RGB delta textures), read only by the multiple-scattering pass.
+  float3* dJ;
   // FP16 storage (half_precision): the scattering LUTs are accumulated in
   // FP32 in S / SM during Init, then stored as 4 x FP16 texels times a
   // per-channel power-of-two scale: S16[t].c = S[t].c * scale[c],

@@ -16,8 +16,11 @@
 
 #include <algorithm>
 #include <array>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <limits>
 #include <map>
 #include <memory>
@@ -38,7 +41,7 @@ using MsKey = std::array<int, 4>;  // (R, MU, MU_S, NU)
 
 // A multiple-scattering shape: tile (index into kMsTpt) and row order.
 struct MsShape {
-  int tile, mu_major;
+  int tile, row_order;
 };
 
 std::mutex g_tiles_mu;
@@ -50,7 +53,7 @@ MsKey ms_key(const Sizes& z) { return {z.nr, z.nmu, z.nmus, z.nnu}; }
 
 MsShape ms_default(const Sizes& z) {
   const int w = z.nnu * z.nmus;
-  return {atmo::ms_default_tile(w), atmo::ms_default_mu_major(z.nr, z.nmu, w)};
+  return {atmo::ms_default_tile(w), atmo::ms_default_row_order(z.nr, z.nmu, w)};
 }
 
 Sizes sizes_of(const atmo_lut_sizes& z) {
@@ -178,7 +181,7 @@ atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStrea
   TUNE_HIP(hipEventCreate(&d.start));
   TUNE_HIP(hipEventCreate(&d.stop));
   const int w = z.nnu * z.nmus;
-  auto time_pass = [&](Pass pass, int order, int tile, int mu_major,
+  auto time_pass = [&](Pass pass, int order, int tile, int row_order,
                        float* best) -> atmo_status {
     std::vector<atmo::Job> js(k);
     std::vector<int> fb(k + 1);
@@ -186,7 +189,7 @@ atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStrea
     for (int i = 0; i < k; ++i) {
       js[i] = models[i]->jobs[0];
       (pass == kSD ? js[i].sd_tile : js[i].ms_tile) = tile;
-      if (pass == kMS) js[i].ms_mu_major = mu_major;
+      if (pass == kMS) js[i].ms_row_order = row_order;
       fb[i] = total;
       total += pass_blocks(pass, js[i]);
     }
@@ -241,22 +244,24 @@ atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStrea
     }
     *sd_out = pick(t, atmo::sd_default_tile(z.nmu, z.nnu));
   }
-  // Candidates 2 * tile + mu_major. Rows too long for LDS take the gather
-  // kernel whatever the tile: only the row order is timed for them.
+  // Candidates kMsRowOrders * tile + row order. Rows too long for LDS take
+  // the gather kernel whatever the tile: only the row order is timed for
+  // them.
   if (do_ms) {
+    constexpr int kO = atmo::kMsRowOrders;
     const MsShape dflt = ms_default(z);
-    std::vector<double> t(2 * atmo::kMsTiles, std::numeric_limits<double>::max());
+    std::vector<double> t(kO * atmo::kMsTiles, std::numeric_limits<double>::max());
     for (int tile = 0; tile < atmo::kMsTiles; ++tile) {
       if (!ms_tile_fits(z, tile)) continue;
       if (w > atmo::kMsMaxRow && tile != dflt.tile) continue;
-      for (int mm = 0; mm < 2; ++mm) {
+      for (int o = 0; o < kO; ++o) {
         float tm = 0.0f;
-        ATMO_TRY(time_pass(kMS, 0, tile, mm, &tm));
-        t[2 * tile + mm] = tm;
+        ATMO_TRY(time_pass(kMS, 0, tile, o, &tm));
+        t[kO * tile + o] = tm;
       }
     }
-    const int best = pick(t, 2 * dflt.tile + dflt.mu_major);
-    *ms_out = MsShape{best / 2, best % 2};
+    const int best = pick(t, kO * dflt.tile + dflt.row_order);
+    *ms_out = MsShape{best / kO, best % kO};
   }
   TUNE_HIP(hipStreamSynchronize(s));
   return ATMO_OK;
@@ -269,13 +274,44 @@ Tiles tiles_for(const Sizes& z) {
   MsShape ms = ms_default(z);
   const auto b = g_ms.find(ms_key(z));
   if (b != g_ms.end() && ms_tile_fits(z, b->second.tile)) ms = b->second;
-  Tiles t{atmo::sd_default_tile(z.nmu, z.nnu), ms.tile, ms.mu_major};
+  Tiles t{atmo::sd_default_tile(z.nmu, z.nnu), ms.tile, ms.row_order};
   const auto a = g_sd.find(sd_key(z));
   if (a != g_sd.end()) t.sd = a->second;
   return t;
 }
 
 }  // namespace atmo_rt
+
+namespace {
+
+// One line of a tile table: a kind word, then at most 'cap' integers in
+// [-2^20, 2^20] separated by blanks, and nothing else. Returns the number of
+// integers, or -1 for a malformed line (no conversion overflows).
+int parse_tile_line(const char* line, std::string* kind, long* v, int cap) {
+  auto blank = [](char c) { return c == ' ' || c == '\t'; };
+  auto end_of_line = [](char c) { return c == '\n' || c == '\r' || c == '\0'; };
+  const char* p = line;
+  while (blank(*p)) ++p;
+  const char* k0 = p;
+  while (!blank(*p) && !end_of_line(*p)) ++p;
+  kind->assign(k0, p);
+  int n = 0;
+  for (;;) {
+    while (blank(*p)) ++p;
+    if (end_of_line(*p)) break;
+    if (n == cap) return -1;
+    char* end = nullptr;
+    errno = 0;
+    const long x = std::strtol(p, &end, 10);
+    if (end == p || errno == ERANGE || x < -(1L << 20) || x > (1L << 20)) return -1;
+    if (!blank(*end) && !end_of_line(*end)) return -1;
+    v[n++] = x;
+    p = end;
+  }
+  return n;
+}
+
+}  // namespace
 
 using namespace atmo_rt;
 
@@ -304,7 +340,7 @@ atmo_status atmo_get_tiles(const atmo_lut_sizes* sizes, atmo_tiles* out) {
   out->sd_nu_per_thread = atmo::kSdTile[t.sd].chunk;
   out->sd_block_threads = atmo::kSdTile[t.sd].block;
   out->ms_texels_per_thread = atmo::kMsTpt[t.ms];
-  out->ms_mu_major = t.ms_mu_major;
+  out->ms_row_order = t.ms_row_order;
   std::lock_guard<std::mutex> lock(g_tiles_mu);
   out->tuned = (g_sd.count(sd_key(z)) ? 1 : 0) | (g_ms.count(ms_key(z)) ? 2 : 0);
   return ATMO_OK;
@@ -324,11 +360,11 @@ atmo_status atmo_set_tiles(const atmo_lut_sizes* sizes, const atmo_tiles* tiles)
   if (!ms_tile_fits(z, ms))
     return fail(ATMO_ERROR_INVALID_ARGUMENT,
                 "multiple-scattering rows of NU x MU_S texels do not fit this tile");
-  if (tiles->ms_mu_major != 0 && tiles->ms_mu_major != 1)
-    return fail(ATMO_ERROR_INVALID_ARGUMENT, "ms_mu_major must be 0 or 1");
+  if (tiles->ms_row_order < 0 || tiles->ms_row_order >= atmo::kMsRowOrders)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT, "ms_row_order must be 0, 1 or 2");
   std::lock_guard<std::mutex> lock(g_tiles_mu);
   g_sd[sd_key(z)] = sd;
-  g_ms[ms_key(z)] = MsShape{ms, tiles->ms_mu_major};
+  g_ms[ms_key(z)] = MsShape{ms, tiles->ms_row_order};
   return ATMO_OK;
 }
 
@@ -371,7 +407,7 @@ atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
 
 // Text file, one size class per line:
 //   sd <MU> <NU> <nu per thread> <block threads>
-//   ms <R> <MU> <MU_S> <NU> <texels per thread> <mu_major>
+//   ms <R> <MU> <MU_S> <NU> <texels per thread> <row_order>
 atmo_status atmo_tune_save(const char* path) {
   if (path == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "path is NULL");
   FILE* f = std::fopen(path, "w");
@@ -386,7 +422,7 @@ atmo_status atmo_tune_save(const char* path) {
     for (const auto& e : g_ms)
       ok = ok && std::fprintf(f, "ms %d %d %d %d %d %d\n", e.first[0], e.first[1],
                               e.first[2], e.first[3], atmo::kMsTpt[e.second.tile],
-                              e.second.mu_major) > 0;
+                              e.second.row_order) > 0;
   }
   ok = (std::fclose(f) == 0) && ok;
   if (!ok) return fail(ATMO_ERROR_INVALID_ARGUMENT, std::string("cannot write ") + path);
@@ -404,29 +440,35 @@ atmo_status atmo_tune_load(const char* path) {
   atmo_status st = ATMO_OK;
   while (st == ATMO_OK && std::fgets(line, sizeof(line), f) != nullptr) {
     ++lineno;
-    char kind[4] = {};
-    int a = 0, b = 0, c = 0, e = 0, g = 0, h = 0;
-    if (line[0] == '#' || line[0] == '\n') continue;
-    const int n = std::sscanf(line, "%3s %d %d %d %d %d %d", kind, &a, &b, &c, &e, &g, &h);
     const std::string where = std::string(path) + ":" + std::to_string(lineno);
-    if (n == 5 && std::string(kind) == "sd") {
-      const int t = sd_tile_of_shape(c, e);
-      if (t < 0 || a < 2 || b < 2)
+    const size_t len = std::strlen(line);
+    if (len + 1 == sizeof(line) && line[len - 1] != '\n') {
+      st = fail(ATMO_ERROR_INVALID_ARGUMENT, where + ": line too long");
+      break;
+    }
+    if (line[0] == '#' || line[0] == '\n') continue;
+    std::string kind;
+    long v[6] = {};
+    const int n = parse_tile_line(line, &kind, v, 6);
+    if (n == 4 && kind == "sd") {
+      const int t = sd_tile_of_shape(static_cast<int>(v[2]), static_cast<int>(v[3]));
+      if (t < 0 || v[0] < 2 || v[1] < 2)
         st = fail(ATMO_ERROR_INVALID_ARGUMENT, where + ": not a scattering-density entry");
       else
-        sd[{a, b}] = t;
-    } else if (n == 7 && std::string(kind) == "ms") {
-      const int t = ms_tile_of_shape(g);
+        sd[{static_cast<int>(v[0]), static_cast<int>(v[1])}] = t;
+    } else if (n == 6 && kind == "ms") {
+      const int t = ms_tile_of_shape(static_cast<int>(v[4]));
       Sizes z{};
-      z.nr = a;
-      z.nmu = b;
-      z.nmus = c;
-      z.nnu = e;
-      if (t < 0 || a < 2 || b < 2 || c < 2 || e < 2 || (h != 0 && h != 1) ||
-          !ms_tile_fits(z, t))
+      z.nr = static_cast<int>(v[0]);
+      z.nmu = static_cast<int>(v[1]);
+      z.nmus = static_cast<int>(v[2]);
+      z.nnu = static_cast<int>(v[3]);
+      const long h = v[5];
+      if (t < 0 || z.nr < 2 || z.nmu < 2 || z.nmus < 2 || z.nnu < 2 || h < 0 ||
+          h >= atmo::kMsRowOrders || !ms_tile_fits(z, t))
         st = fail(ATMO_ERROR_INVALID_ARGUMENT, where + ": not a multiple-scattering entry");
       else
-        ms[{a, b, c, e}] = MsShape{t, h};
+        ms[{z.nr, z.nmu, z.nmus, z.nnu}] = MsShape{t, static_cast<int>(h)};
     } else {
       st = fail(ATMO_ERROR_INVALID_ARGUMENT, where + ": unrecognized line");
     }

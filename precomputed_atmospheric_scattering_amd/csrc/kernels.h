@@ -78,14 +78,19 @@ ATMO_HD int ms_default_tile(int w) {
   if (w <= 2 * kBlock) return 1;
   return 2;
 }
-// Default row order of the multiple-scattering blocks (Job::ms_mu_major):
-// mu-major once the scattering-density LUT the pass reads (R x MU x W
-// float4 texels) reaches kMsMuMajorBytes. Measured on MI355X: at R x MU x
-// MU_S x NU = 32 x 128 x 32 x 8 (16.8 MB) r-major is 1-2% faster, at
-// 64 x 256 x 64 x 16 (268 MB) mu-major is 5% faster.
-constexpr double kMsMuMajorBytes = 64.0 * 1048576.0;
-ATMO_HD int ms_default_mu_major(int nr, int nmu, int w) {
-  return static_cast<double>(nr) * nmu * w * 16.0 >= kMsMuMajorBytes ? 1 : 0;
+// Row orders of the multiple-scattering blocks (Job::ms_row_order):
+// 0 r-major (the LUT's own order), 1 mu-major, 2 ray order (rows sorted by
+// the impact parameter of their view ray, Job::ms_rows). The samples of a
+// row's ray lie on the ray's line, whose points have the same impact
+// parameter r sqrt(1 - mu^2), so rows of nearly equal impact parameter read
+// nearly the same source rows of the scattering density: in ray order the
+// blocks resident on one XCD share their source rows in its L2.
+constexpr int kMsRowOrders = 3;
+// Default: ray order once the scattering-density LUT the pass reads
+// (R x MU x W RGB float texels) reaches kMsRayOrderBytes, r-major below.
+constexpr double kMsRayOrderBytes = 64.0 * 1048576.0;
+ATMO_HD int ms_default_row_order(int nr, int nmu, int w) {
+  return static_cast<double>(nr) * nmu * w * 12.0 >= kMsRayOrderBytes ? 2 : 0;
 }
 
 // Blocks one job needs for each pass.

@@ -583,8 +583,13 @@ k_scattering_density(
   const int nch = (nnu + kChunk - 1) / kChunk;
   const int plane = nmu * nch;  // threads per (r, mu_s) plane
   const int tiles = (plane + kB - 1) / kB;
-  const int tile = lb % tiles;
-  const int pl = lb / tiles;
+  // The blocks of neighbouring mu_s columns of a layer write the same cache
+  // lines of delta J (a plane's texels are W texels apart): with
+  // XCD-contiguous ranges those blocks share one L2, where the partial
+  // lines merge before they are written back.
+  const int lbx = xcd_contiguous(lb, tiles * nmus * J.nr);
+  const int tile = lbx % tiles;
+  const int pl = lbx / tiles;
   const int imus = pl % nmus, k = pl / nmus;
   const int W = nnu * nmus;
   const bool two = order == 2;
@@ -752,7 +757,7 @@ k_scattering_density(
     const int nui = ch * kChunk + jn;
     if (nui < nnu)
       gstore(J.dJ, (layer + y) * W + nui * nmus + imus,
-             make_float4(acc[jn][0], acc[jn][1], acc[jn][2], 0.0f));
+             make_float3(acc[jn][0], acc[jn][1], acc[jn][2]));
   }
 }
 
@@ -928,15 +933,20 @@ struct MusMap {
 };
 
 // Packed row p of a job's block order: r-major (p = z * MU + y, the LUT's
-// own order) or, with ms_mu_major, mu-major (p = y * R + z), where the
-// blocks that run together share mu and step in r. The texels and their
-// values are the same either way; the order only changes which source rows
-// of the scattering density neighbouring blocks read.
+// own order), mu-major (p = y * R + z), or ray order (the model's table
+// ms_rows of (z << 16) | y, rows sorted by the impact parameter of their
+// view ray). The texels and their values are the same in every order; the
+// order only changes which source rows of the scattering density the blocks
+// that run together read.
 // Returns (z, y). One division, and selects of values: writing the two
 // results through pointers in two branches compiles to stores through a
 // selected address, which puts them in scratch memory.
 __device__ inline int2 ms_row(const Job& J, int p) {
-  const bool mm = J.ms_mu_major != 0;
+  if (J.ms_row_order == 2) {
+    const int v = gload(J.ms_rows, p);
+    return make_int2(v >> 16, v & 0xffff);
+  }
+  const bool mm = J.ms_row_order != 0;
   const int d = mm ? J.nr : J.nmu;
   const int q = p / d, rem = p - q * d;
   return mm ? make_int2(rem, q) : make_int2(q, rem);
@@ -1055,7 +1065,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   __syncthreads();
 
   const MusMap mus_map(J);
-  const float4* __restrict__ D = J.dJ;
+  const float3* __restrict__ D = J.dJ;
   // Source texels of the next sample, loaded one sample ahead so that the
   // loads are in flight during the barrier and the gather of the current one.
   float3 src[kTpt][4];
@@ -1067,10 +1077,10 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
       const bool on = slot_of[j] >= 0;
       const MsRowSample& S = smp[on ? slot_of[j] : 0][i];
       const int x = on ? x_of[j] : 0;
-      src[j][0] = make3(gload32(D, static_cast<uint32_t>(S.o00 + x)));
-      src[j][1] = make3(gload32(D, static_cast<uint32_t>(S.o01 + x)));
-      src[j][2] = make3(gload32(D, static_cast<uint32_t>(S.o10 + x)));
-      src[j][3] = make3(gload32(D, static_cast<uint32_t>(S.o11 + x)));
+      src[j][0] = gload32(D, static_cast<uint32_t>(S.o00 + x));
+      src[j][1] = gload32(D, static_cast<uint32_t>(S.o01 + x));
+      src[j][2] = gload32(D, static_cast<uint32_t>(S.o10 + x));
+      src[j][3] = gload32(D, static_cast<uint32_t>(S.o11 + x));
     }
   };
   fetch(0);
@@ -1217,7 +1227,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
   const int cs0 = s0 * nmus, cs1 = s1 * nmus;
   const double rmus = r * mu_s;
   const MusMap mus_map(J);
-  const float4* __restrict__ D = J.dJ;
+  const float3* __restrict__ D = J.dJ;
   float a0 = 0.f, a1 = 0.f, a2 = 0.f;
   for (int s = 0; s < kMarch; ++s) {
     const MsSample& S = smp[s];
@@ -1241,7 +1251,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
       float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const float4 v = gload(D, rows[q] + cols[c]);
+        const float3 v = gload(D, rows[q] + cols[c]);
         r0 = fmaf(wc[c], v.x, r0);
         r1 = fmaf(wc[c], v.y, r1);
         r2 = fmaf(wc[c], v.z, r2);

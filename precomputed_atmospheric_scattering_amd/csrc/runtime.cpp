@@ -202,8 +202,9 @@ namespace {
 
 void clear_temporary_pointers(atmo_model* m) {
   m->T_it = m->E_it = nullptr;
-  m->dE_it = m->dR_it = m->dM_it = m->dJ_it = nullptr;
-  m->dE = m->dR = m->dM = m->dJ = nullptr;
+  m->dE_it = m->dR_it = m->dM_it = nullptr;
+  m->dE = m->dR = m->dM = nullptr;
+  m->dJ_it = m->dJ = nullptr;
   if (m->spec.half_precision) m->S = m->SM = nullptr;
   m->S64 = m->SM64 = nullptr;
   m->OD = nullptr;
@@ -231,7 +232,7 @@ size_t carve(const Sizes& z, bool half, bool combined, int iters, char* base,
   char* de = take(P * z.e_texels() * sizeof(float4));
   char* dr = take(P * z.s_texels() * sizeof(float4));
   char* dm = take(P * z.s_texels() * sizeof(float4));
-  char* dj = take(P * z.s_texels() * sizeof(float4));
+  char* dj = take(P * z.s_texels() * sizeof(float3));
   char* s32 = half ? take(z.s_texels() * sizeof(float4)) : nullptr;
   char* sm32 = half && !combined ? take(z.s_texels() * sizeof(float4)) : nullptr;
   char* s64 = par ? take(P * z.s_texels() * 4 * sizeof(double)) : nullptr;
@@ -243,7 +244,7 @@ size_t carve(const Sizes& z, bool half, bool combined, int iters, char* base,
     m->dE_it = reinterpret_cast<float4*>(de);
     m->dR_it = reinterpret_cast<float4*>(dr);
     m->dM_it = reinterpret_cast<float4*>(dm);
-    m->dJ_it = reinterpret_cast<float4*>(dj);
+    m->dJ_it = reinterpret_cast<float3*>(dj);
     // The retained intermediates: the last iteration's delta LUTs.
     m->dE = m->dE_it + (P - 1) * z.e_texels();
     m->dR = m->dR_it + (P - 1) * z.s_texels();
@@ -301,6 +302,39 @@ void drop_external_temporaries(atmo_model* m) {
   m->external_temporaries = false;
 }
 
+// The ray order of a model's packed (r, mu) rows (Job::ms_rows): sky rows
+// before ground rows, each sorted by the impact parameter r sqrt(1 - mu^2)
+// of the row's view ray, then by decreasing mu; entries (z << 16) | y. It
+// depends on the radii and the LUT sizes only.
+std::vector<int> ms_ray_order(const atmo::Job& j) {
+  struct Row {
+    int ground;
+    double p, mu;
+    int zy;
+  };
+  std::vector<Row> rows;
+  rows.reserve(static_cast<size_t>(j.nr) * j.nmu);
+  for (int z = 0; z < j.nr; ++z) {
+    double r, rho;
+    atmo::s_layer_r(j, z, &r, &rho);
+    for (int y = 0; y < j.nmu; ++y) {
+      double mu;
+      bool ground;
+      atmo::s_row_mu(j, y, r, rho, &mu, &ground);
+      rows.push_back({ground ? 1 : 0, r * std::sqrt(std::max(0.0, 1.0 - mu * mu)), mu,
+                      (z << 16) | y});
+    }
+  }
+  std::stable_sort(rows.begin(), rows.end(), [](const Row& a, const Row& b) {
+    if (a.ground != b.ground) return a.ground < b.ground;
+    if (a.p != b.p) return a.p < b.p;
+    return a.mu > b.mu;
+  });
+  std::vector<int> out(rows.size());
+  for (size_t i = 0; i < rows.size(); ++i) out[i] = rows[i].zy;
+  return out;
+}
+
 // Job k < iters is Precompute iteration k; k = -1 is a job on the resident
 // LUTs (the final transmittance and accumulator sums, the lookups). In
 // illuminance mode every iteration has its own transmittance, delta LUTs and
@@ -341,7 +375,8 @@ void fill_job_luts(atmo_model* m, atmo::Job* j, int k) {
   const Tiles t = tiles_for(m->sizes);
   j->sd_tile = t.sd;
   j->ms_tile = t.ms;
-  j->ms_mu_major = t.ms_mu_major;
+  j->ms_row_order = t.ms_row_order;
+  j->ms_rows = m->d_ms_rows;
 }
 
 // Rebuild the host job table (LUT pointers may have changed) and upload it.
@@ -413,10 +448,12 @@ atmo_status check_launch(hipError_t e, const char* what) {
   return ATMO_OK;
 }
 
-// Records start/stop events around each launch when timing is enabled.
+// Records start/stop events around each launch when 'on' (the model's
+// ATMO_OPTION_PASS_TIMING, except in the allocation-free Init).
 struct LaunchTimer {
   atmo_model* m;
   hipStream_t s;
+  bool on;
   size_t next_event = 0;
   hipEvent_t take() {
     if (next_event == m->event_pool.size()) {
@@ -429,7 +466,7 @@ struct LaunchTimer {
   template <class F>
   atmo_status run(int pass, int order, const char* what, F&& launch) {
     hipEvent_t a = nullptr, b = nullptr;
-    if (m->timing) {
+    if (on) {
       a = take();
       b = take();
       if (a == nullptr || b == nullptr)
@@ -438,7 +475,7 @@ struct LaunchTimer {
         return fail(ATMO_ERROR_DEVICE, "hipEventRecord failed");
     }
     ATMO_TRY(check_launch(launch(), what));
-    if (m->timing) {
+    if (on) {
       if (hipEventRecord(b, s) != hipSuccess)
         return fail(ATMO_ERROR_DEVICE, "hipEventRecord failed");
       m->timed.push_back({pass, order, a, b});
@@ -479,6 +516,38 @@ atmo_status enqueue_precompute(atmo_model* m, unsigned orders, LaunchTimer& t) {
       return atmo::launch_multiple_scattering(model_launch(m, 0, n, kMS), s);
     }));
   }
+  return ATMO_OK;
+}
+
+// Init after its temporaries are in place: the job table, the passes, the
+// sums of the iterations' accumulators, the final transmittance, FP16
+// conversion and the lookup copy.
+atmo_status enqueue_init(atmo_model* m, unsigned orders, hipStream_t s, bool timing) {
+  m->last_orders = orders;
+  ATMO_TRY(upload_jobs(m, orders, s));
+  const int iters = m->iters;
+  m->timed.clear();
+  LaunchTimer timer{m, s, timing};
+  ATMO_TRY(enqueue_precompute(m, orders, timer));
+  // Job 'iters' works on the resident LUTs: the sum of the iterations'
+  // accumulators, then the transmittance at 680/550/440 nm.
+  if (accumulates_fp64(m->spec)) {
+    ATMO_TRY(check_launch(atmo::launch_acc64_to_f32(model_launch(m, iters, 1, kAcc), s),
+                          "fp64 accumulators"));
+  }
+  if (m->spec.num_precomputed_wavelengths > 3) {
+    ATMO_TRY(timer.run(kT, 0, "transmittance (rgb)", [&] {
+      return atmo::launch_transmittance(model_launch(m, iters, 1, kT), s);
+    }));
+  }
+  if (m->spec.half_precision) {
+    ATMO_TRY(check_launch(atmo::launch_clear_max(model_launch(m, iters, 1, kClear), s),
+                          "fp16 maxima"));
+    ATMO_TRY(check_launch(atmo::launch_to_half(model_launch(m, iters, 1, kHalf), s),
+                          "fp16 conversion"));
+  }
+  ATMO_TRY(build_lookup_copy(m, s));
+  m->initialized = true;
   return ATMO_OK;
 }
 
@@ -668,7 +737,11 @@ atmo_status atmo_model_create(const atmo_model_params* p,
   m->iters = iters;
   ATMO_HIP(hipMalloc(&m->d_jobs, (iters + 2) * sizeof(atmo::Job)));
   ATMO_HIP(hipMalloc(&m->d_first_block, kPasses * (iters + 1) * sizeof(int)));
+  ATMO_HIP(hipMalloc(&m->d_ms_rows, static_cast<size_t>(sz.nr) * sz.nmu * sizeof(int)));
   ATMO_TRY(upload_jobs(m.get(), 4, nullptr));
+  const std::vector<int> ray_rows = ms_ray_order(m->jobs[0]);
+  ATMO_HIP(hipMemcpy(m->d_ms_rows, ray_rows.data(), ray_rows.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
   ATMO_HIP(hipStreamSynchronize(nullptr));
   *out = m.release();
   return ATMO_OK;
@@ -686,6 +759,7 @@ atmo_status atmo_model_destroy(atmo_model* m) {
     if (b != nullptr) (void)hipFree(b);
   if (m->d_jobs != nullptr) (void)hipFree(m->d_jobs);
   if (m->d_first_block != nullptr) (void)hipFree(m->d_first_block);
+  if (m->d_ms_rows != nullptr) (void)hipFree(m->d_ms_rows);
   for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
   delete m;
   return ATMO_OK;
@@ -730,31 +804,7 @@ atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
   if (orders < 1) return fail(ATMO_ERROR_INVALID_ARGUMENT, "num_scattering_orders must be >= 1");
   hipStream_t s = static_cast<hipStream_t>(stream);
   ATMO_TRY(alloc_temporaries(m));
-  m->last_orders = orders;
-  ATMO_TRY(upload_jobs(m, orders, s));
-  const int iters = m->iters;
-  m->timed.clear();
-  LaunchTimer timer{m, s};
-  ATMO_TRY(enqueue_precompute(m, orders, timer));
-  // Job 'iters' works on the resident LUTs: the sum of the iterations'
-  // accumulators, then the transmittance at 680/550/440 nm.
-  if (accumulates_fp64(m->spec)) {
-    ATMO_TRY(check_launch(atmo::launch_acc64_to_f32(model_launch(m, iters, 1, kAcc), s),
-                          "fp64 accumulators"));
-  }
-  if (m->spec.num_precomputed_wavelengths > 3) {
-    ATMO_TRY(timer.run(kT, 0, "transmittance (rgb)", [&] {
-      return atmo::launch_transmittance(model_launch(m, iters, 1, kT), s);
-    }));
-  }
-  if (m->spec.half_precision) {
-    ATMO_TRY(check_launch(atmo::launch_clear_max(model_launch(m, iters, 1, kClear), s),
-                          "fp16 maxima"));
-    ATMO_TRY(check_launch(atmo::launch_to_half(model_launch(m, iters, 1, kHalf), s),
-                          "fp16 conversion"));
-  }
-  ATMO_TRY(build_lookup_copy(m, s));
-  m->initialized = true;
+  ATMO_TRY(enqueue_init(m, orders, s, m->timing));
   // A live batch holding the model keeps reading and writing its temporaries.
   if (!m->retain && m->live_batches == 0) {
     // Init then returns once the precomputation is complete, as the GL
@@ -827,7 +877,10 @@ atmo_status atmo_model_get_lut(const atmo_model* m, atmo_lut_id id, atmo_lut_vie
     case ATMO_LUT_DELTA_IRRADIANCE: v->data = m->dE; break;
     case ATMO_LUT_DELTA_RAYLEIGH: v->data = m->dR; break;
     case ATMO_LUT_DELTA_MIE: v->data = m->dM; break;
-    case ATMO_LUT_DELTA_SCATTERING_DENSITY: v->data = m->dJ; break;
+    case ATMO_LUT_DELTA_SCATTERING_DENSITY:
+      v->data = m->dJ;
+      v->channels = 3;  // RGB texels
+      break;
     default: return fail(ATMO_ERROR_INVALID_ARGUMENT, "unknown LUT id");
   }
   if (v->data == nullptr)
@@ -841,6 +894,16 @@ atmo_status atmo_model_copy_lut(const atmo_model* m, atmo_lut_id id, float* host
   if (host == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "host is NULL");
   ATMO_HIP(hipDeviceSynchronize());
   const size_t texels = static_cast<size_t>(v.width) * v.height * v.depth;
+  if (v.dtype == ATMO_DTYPE_F32 && v.channels == 3) {
+    std::vector<float> rgb(texels * 3);
+    ATMO_HIP(hipMemcpy(rgb.data(), v.data, texels * 3 * sizeof(float),
+                       hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < texels; ++t) {
+      for (int k = 0; k < 3; ++k) host[4 * t + k] = rgb[3 * t + k];
+      host[4 * t + 3] = 0.0f;
+    }
+    return ATMO_OK;
+  }
   if (v.dtype == ATMO_DTYPE_F32) {
     ATMO_HIP(hipMemcpy(host, v.data, texels * 4 * sizeof(float), hipMemcpyDeviceToHost));
     return ATMO_OK;
@@ -855,13 +918,13 @@ atmo_status atmo_model_copy_lut(const atmo_model* m, atmo_lut_id id, float* host
   return ATMO_OK;
 }
 
-atmo_status atmo_model_footprint(const atmo_model* m, size_t* resident, size_t* scratch) {
+atmo_status atmo_model_footprint(const atmo_model* m, size_t* resident, size_t* init_scratch) {
   if (m == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "model is NULL");
   const Sizes& z = m->sizes;
   if (resident != nullptr) *resident = lut_bytes(m, !m->spec.combined);
-  if (scratch != nullptr) {
+  if (init_scratch != nullptr) {
     (void)z;
-    *scratch = temporaries_bytes(m);
+    *init_scratch = temporaries_bytes(m);
   }
   return ATMO_OK;
 }

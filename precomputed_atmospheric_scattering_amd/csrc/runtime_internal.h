@@ -83,8 +83,9 @@ struct atmo_model {
   // dJ point at the last iteration's copies (the retained intermediates).
   char* temp_block = nullptr;
   float4 *T_it = nullptr, *E_it = nullptr;
-  float4 *dE_it = nullptr, *dR_it = nullptr, *dM_it = nullptr, *dJ_it = nullptr;
-  float4 *dE = nullptr, *dR = nullptr, *dM = nullptr, *dJ = nullptr;
+  float4 *dE_it = nullptr, *dR_it = nullptr, *dM_it = nullptr;
+  float4 *dE = nullptr, *dR = nullptr, *dM = nullptr;
+  float3 *dJ_it = nullptr, *dJ = nullptr;  // delta J: RGB texels
   // FP16 storage: resident half LUTs; S and SM above are then FP32
   // accumulators that live only during Init, like the deltas.
   uint2 *S16 = nullptr, *SM16 = nullptr;
@@ -101,6 +102,8 @@ struct atmo_model {
   double* OD = nullptr;
   float* d_scales = nullptr;
   unsigned* d_maxbits = nullptr;
+  // The (r, mu) rows in ray order (Job::ms_rows), R x MU entries.
+  int* d_ms_rows = nullptr;
   // Device job table: [iterations..., rgb transmittance job, query job].
   std::vector<atmo::Job> jobs;
   atmo::Job* d_jobs = nullptr;
@@ -148,7 +151,7 @@ int pass_blocks(Pass p, const atmo::Job& j);
 // default shapes for sizes it has not tuned (autotune.cpp).
 struct Tiles {
   int sd, ms;
-  int ms_mu_major;  // row order of the multiple-scattering blocks
+  int ms_row_order;  // row order of the multiple-scattering blocks
 };
 Tiles tiles_for(const Sizes& z);
 // Rebuild a model's job table for 'orders' and copy it to the device on

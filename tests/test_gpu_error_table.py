@@ -1,10 +1,9 @@
 """Error of every texture against the FP64 oracle for every scattering order
-(Init(n), n = 1..4), for the Earth atmosphere (3 wavelengths) at the
-reference LUT sizes and the spectral atmosphere (15 wavelengths, ozone) at
-half the reference size in every dimension but nu (the full FP64 chain of
-the spectral atmosphere at the reference sizes takes minutes of host time),
-each with
-FP32 and FP16 storage. The tables are printed (pytest -s, or the log of -v)
+(Init(n), n = 1..4), for the Earth atmosphere (3 wavelengths) and the
+spectral atmosphere (15 wavelengths, ozone) at the reference LUT sizes - the
+spectral FP64 chain takes about a minute of host time on 16 threads - and
+the spectral atmosphere at half those sizes in every dimension but nu, each
+with FP32 and FP16 storage. The tables are printed (pytest -s, or the log of -v)
 and asserted against the bars: 1e-3 for FP32 textures, 5e-3 for FP16-stored
 ones (metric of parity.rel_err, floor 1e-6 of the texture's channel maximum).
 """
@@ -59,3 +58,8 @@ def test_earth_reference_sizes():
 def test_spectral_half_sizes():
     table(configs.spectral(lut_sizes=parity.MID_SIZES),
           "spectral, 15 wavelengths, LUTs 128x32 / 16x64x16x8 / 32x16")
+
+
+def test_spectral_reference_sizes():
+    table(configs.spectral(),
+          "spectral, 15 wavelengths, LUTs 256x64 / 32x128x32x8 / 64x16")

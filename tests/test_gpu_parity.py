@@ -114,9 +114,9 @@ LONG_ROW_SIZES = dict(transmittance_width=64, transmittance_height=16, scatterin
                       irradiance_width=16, irradiance_height=8)
 
 
-def test_long_rows_multiple_scattering_both_row_orders():
+def test_long_rows_multiple_scattering_every_row_order():
     """The gather kernel of the multiple-scattering pass, pass-local against
-    the oracle on the GPU's own delta J, in both row orders of its blocks
+    the oracle on the GPU's own delta J, in every row order of its blocks
     (which must give bit-identical LUTs)."""
     import ctypes as C
     import oracle_lib as ol
@@ -130,9 +130,9 @@ def test_long_rows_multiple_scattering_both_row_orders():
     ip = idx.ctypes.data_as(C.POINTER(C.c_int))
     out = {}
     try:
-        for mm in (False, True):
-            print("case long rows, mu-major" if mm else "case long rows, r-major", flush=True)
-            atmo.set_tiles(LONG_ROW_SIZES, 4, 256, 1, ms_mu_major=mm)
+        for mm in atmo.MS_ROW_ORDERS:
+            print("case long rows, %s" % mm, flush=True)
+            atmo.set_tiles(LONG_ROW_SIZES, 4, 256, 1, ms_row_order=mm)
             m = parity.gpu_model(cfg, retain_intermediates=True)
             m.init(2)
             T = np.ascontiguousarray(m.lut("transmittance"), dtype=np.float64)
@@ -150,8 +150,9 @@ def test_long_rows_multiple_scattering_both_row_orders():
             m.close()
     finally:
         atmo.reset_tiles()
-    for k in out[False]:
-        assert np.array_equal(out[False][k], out[True][k]), k
+    for mm in atmo.MS_ROW_ORDERS[1:]:
+        for k in out["r-major"]:
+            assert np.array_equal(out["r-major"][k], out[mm][k]), (mm, k)
 
 
 # The ends of the table sizes of the scattering-density and indirect-

@@ -47,10 +47,10 @@ def shapes_for(sizes):
     sd, ms = atmo.tile_candidates()
     d = atmo.get_tiles(sizes)
     dsd = (d["sd_nu_per_thread"], d["sd_block_threads"])
-    dmm = d["ms_mu_major"]
+    dmm = d["ms_row_order"]
     out = [(s, d["ms_texels_per_thread"], dmm) for s in sd if s != dsd]
     w = sizes["scattering_nu"] * sizes["scattering_mu_s"]
-    out += [(dsd, t, mm) for t in ms for mm in (False, True)
+    out += [(dsd, t, mm) for t in ms for mm in atmo.MS_ROW_ORDERS
             if (t, mm) != (d["ms_texels_per_thread"], dmm) and w <= 256 * t]
     return out
 
@@ -61,8 +61,8 @@ def test_every_shape_gives_identical_luts(name):
     cfg = configs.earth(orders=4, lut_sizes=sizes)
     ref = run(cfg)
     for (chunk, block), tpt, mm in shapes_for(sizes):
-        print("case", name, chunk, block, tpt, "mu-major" if mm else "r-major", flush=True)
-        atmo.set_tiles(sizes, chunk, block, tpt, ms_mu_major=mm)
+        print("case", name, chunk, block, tpt, mm, flush=True)
+        atmo.set_tiles(sizes, chunk, block, tpt, ms_row_order=mm)
         got = run(cfg)
         for n in NAMES:
             assert np.array_equal(got[n], ref[n]), (name, chunk, block, tpt, mm, n)
@@ -74,10 +74,10 @@ def test_every_shape_identical_illuminance_mode():
     sizes = SIZES["small"]
     cfg = configs.spectral(orders=3, lut_sizes=sizes)
     ref = run(cfg)
-    for (chunk, block), tpt, mm in [((1, 256), 1, True), ((4, 64), 2, False),
-                                     ((2, 128), 4, True)]:
+    for (chunk, block), tpt, mm in [((1, 256), 1, "mu-major"), ((4, 64), 2, "r-major"),
+                                     ((2, 128), 4, "ray")]:
         print("case spectral", chunk, block, tpt, mm, flush=True)
-        atmo.set_tiles(sizes, chunk, block, tpt, ms_mu_major=mm)
+        atmo.set_tiles(sizes, chunk, block, tpt, ms_row_order=mm)
         got = run(cfg)
         for n in NAMES:
             assert np.array_equal(got[n], ref[n]), (chunk, block, tpt, n)
@@ -100,9 +100,9 @@ def test_autotune_then_mixed_shape_batch_equals_alone(use_graph):
     # Force two shapes on the first two size classes so that the batch has
     # jobs of different shapes in one pass level whatever the tuner chose.
     atmo.set_tiles(sizes[0], 1, 256, 4 if sizes[0]["scattering_nu"] *
-                   sizes[0]["scattering_mu_s"] <= 1024 else 1, ms_mu_major=True)
+                   sizes[0]["scattering_mu_s"] <= 1024 else 1, ms_row_order="ray")
     atmo.set_tiles(sizes[1], 4, 64, 4 if sizes[1]["scattering_nu"] *
-                   sizes[1]["scattering_mu_s"] <= 1024 else 1, ms_mu_major=False)
+                   sizes[1]["scattering_mu_s"] <= 1024 else 1, ms_row_order="r-major")
     models = [atmo.Model(**configs.model_kwargs(c)[0]) for c in cfgs]
     b = atmo.Batch(models, [c["orders"] for c in cfgs], use_graph=use_graph)
     # The launches of one level that differ in tile shape run concurrently

@@ -38,13 +38,15 @@ def test_defaults_follow_row_length():
 
 
 def test_default_row_order_follows_lut_bytes():
-    # r-major up to a 64 MiB scattering-density LUT, mu-major from there.
-    assert not atmo.get_tiles()["ms_mu_major"]  # 32 x 128 x 32 x 8: 16 MiB
-    assert not atmo.get_tiles(dict(scattering_r=64, scattering_mu_s=48))["ms_mu_major"]
+    # r-major below a 64 MiB scattering-density LUT (RGB float texels), ray
+    # order from there.
+    assert atmo.get_tiles()["ms_row_order"] == "r-major"  # 32 x 128 x 32 x 8: 12 MiB
+    assert atmo.get_tiles(dict(scattering_r=64, scattering_mu_s=48))["ms_row_order"] == \
+        "r-major"
     big = dict(scattering_r=64, scattering_mu=256, scattering_mu_s=64, scattering_nu=16)
-    assert atmo.get_tiles(big)["ms_mu_major"]  # 256 MiB
-    assert atmo.get_tiles(dict(big, scattering_nu=4))["ms_mu_major"]  # 64 MiB
-    assert not atmo.get_tiles(dict(big, scattering_nu=2))["ms_mu_major"]
+    assert atmo.get_tiles(big)["ms_row_order"] == "ray"  # 192 MiB
+    assert atmo.get_tiles(dict(big, scattering_nu=8))["ms_row_order"] == "ray"  # 96 MiB
+    assert atmo.get_tiles(dict(big, scattering_nu=4))["ms_row_order"] == "r-major"  # 48 MiB
 
 
 def test_set_get_by_size_class():
@@ -61,11 +63,13 @@ def test_set_get_by_size_class():
     assert t2["sd_tuned"] and not t2["ms_tuned"] and t2["ms_texels_per_thread"] == 1
     assert atmo.get_tiles()["sd_nu_per_thread"] == 4  # default class untouched
     # The row order is pinned with the rest of the shape.
-    atmo.set_tiles(SMALL, 4, 256, 1, ms_mu_major=True)
-    assert atmo.get_tiles(SMALL)["ms_mu_major"]
+    atmo.set_tiles(SMALL, 4, 256, 1, ms_row_order="mu-major")
+    assert atmo.get_tiles(SMALL)["ms_row_order"] == "mu-major"
     atmo.set_tiles(SMALL, 4, 256, 2)  # keeps the class's order
-    assert atmo.get_tiles(SMALL)["ms_mu_major"]
-    assert not atmo.get_tiles(other)["ms_mu_major"]
+    assert atmo.get_tiles(SMALL)["ms_row_order"] == "mu-major"
+    atmo.set_tiles(SMALL, 4, 256, 2, ms_row_order="ray")
+    assert atmo.get_tiles(SMALL)["ms_row_order"] == "ray"
+    assert atmo.get_tiles(other)["ms_row_order"] == "r-major"
 
 
 @pytest.mark.parametrize("shape", [(3, 256, 1), (4, 96, 1), (4, 256, 3), (4, 512, 1)])
@@ -84,7 +88,7 @@ def test_multiple_scattering_tile_must_hold_the_row():
 
 def test_save_load_round_trip(tmp_path):
     atmo.set_tiles(SMALL, 2, 128, 4)
-    atmo.set_tiles(WIDE, 4, 64, 4, ms_mu_major=True)
+    atmo.set_tiles(WIDE, 4, 64, 4, ms_row_order="ray")
     path = tmp_path / "tiles.txt"
     atmo.save_tiles(path)
     before = {k: atmo.get_tiles(z) for k, z in (("s", SMALL), ("w", WIDE), ("d", None))}
@@ -95,11 +99,11 @@ def test_save_load_round_trip(tmp_path):
     assert after == before
     lines = [l for l in path.read_text().splitlines() if not l.startswith("#")]
     assert sorted(lines) == sorted(["sd 64 4 2 128", "sd 128 16 4 64",
-                                    "ms 16 64 16 4 4 0", "ms 32 128 64 16 4 1"])
+                                    "ms 16 64 16 4 4 0", "ms 32 128 64 16 4 2"])
 
 
 @pytest.mark.parametrize("text", ["sd 64 4 3 256\n", "ms 32 128 64 16 1 0\n",
-                                  "ms 32 128 8 4 1 2\n", "ms 128 8 4 1\n",
+                                  "ms 32 128 8 4 1 3\n", "ms 128 8 4 1\n",
                                   "xx 1 2 3 4\n", "sd 64 4\n"])
 def test_load_rejects_bad_entries_atomically(tmp_path, text):
     path = tmp_path / "bad.txt"
