@@ -175,6 +175,25 @@ atmo_status atmo_model_pass_timings(atmo_model* model, atmo_pass_timing* out,
  * LUTs with num_scattering_orders scattering orders (>= 1). */
 atmo_status atmo_model_init(atmo_model* model, unsigned num_scattering_orders,
                             void* stream);
+/* Allocation-free Init. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). It computes the same LUTs
+ * as atmo_model_init, bit for bit, but carves the Init temporaries from a
+ * static device array of ATMO_STATIC_SCRATCH_BYTES per GPU (held from the
+ * library's first use on that GPU until the process ends) and calls no
+ * hipMalloc or hipFree. A model whose temporaries exceed that size
+ * (atmo_model_footprint's init_scratch; e.g. 50.3 MB for 3 wavelengths at
+ * the reference LUT sizes in FP32, 587 MB for 15 wavelengths) is rejected
+ * with ATMO_ERROR_INVALID_ARGUMENT, as are models in a live batch or owned
+ * by a server. Returns without waiting; the temporaries are not retained and
+ * no pass timings are recorded. Every call on one GPU shares that GPU's
+ * array: calls whose work overlaps on the device (on different streams, or
+ * from different threads) race, so the caller orders them (one stream, or
+ * events). The model must have been created on the current device. */
+#define ATMO_STATIC_SCRATCH_BYTES (64u << 20)
+atmo_status atmo_model_init_static_scratch(atmo_model* model,
+                                           unsigned num_scattering_orders,
+                                           void* stream);
 /* Init of several models in one batched submission (no counterpart in the
  * reference, which precomputes one Model at a time): all models' passes of
  * one kind run in a single launch, the launch sequence is captured once in a

@@ -21,6 +21,8 @@ LUT_IDS = {
 OPTION_RETAIN_INTERMEDIATES = 0
 OPTION_PASS_TIMING = 1
 OPTION_INTERLEAVED_LOOKUPS = 2
+# ATMO_STATIC_SCRATCH_BYTES: the static scratch of atmo_model_init_static_scratch.
+STATIC_SCRATCH_BYTES = 64 << 20
 PASS_NAMES = ("transmittance", "direct_irradiance", "single_scattering",
               "scattering_density", "indirect_irradiance", "multiple_scattering")
 
@@ -131,6 +133,7 @@ SIGNATURES = {
     "atmo_model_pass_timings": (ST, [VP, C.POINTER(PassTiming), C.c_int,
                                      C.POINTER(C.c_int)]),
     "atmo_model_init": (ST, [VP, C.c_uint, VP]),
+    "atmo_model_init_static_scratch": (ST, [VP, C.c_uint, VP]),
     "atmo_model_init_batch": (ST, [C.POINTER(VP), C.POINTER(C.c_uint), C.c_int, VP]),
     "atmo_batch_create": (ST, [C.POINTER(VP), C.POINTER(C.c_uint), C.c_int, C.c_int,
                                C.POINTER(VP)]),

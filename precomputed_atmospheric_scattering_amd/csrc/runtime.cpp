@@ -551,6 +551,41 @@ atmo_status enqueue_init(atmo_model* m, unsigned orders, hipStream_t s, bool tim
   return ATMO_OK;
 }
 
+// The Init temporaries of atmo_model_init_static_scratch: a file-scope
+// device array, one copy per GPU, allocated when the code object is loaded
+// and held until the process ends. Every call on a GPU shares that GPU's
+// copy. Host code never passes the array's name to a kernel or a copy (it is
+// a host-side stand-in): it uses the device address that
+// hipGetSymbolAddress returns, fetched once per device with that device
+// current and kept in g_scratch_ptr.
+constexpr size_t kStaticScratchBytes = static_cast<size_t>(ATMO_STATIC_SCRATCH_BYTES);
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+atmo_status static_scratch(int device, char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  int current = -1;
+  ATMO_HIP(hipGetDevice(&current));
+  if (current != device)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT,
+                "model was created on device " + std::to_string(device) +
+                    ", current device is " + std::to_string(current));
+  if (device < 0 || device >= kMaxDevices)
+    return fail(ATMO_ERROR_UNSUPPORTED, "device index beyond the static scratch table");
+  if (g_scratch_ptr[device] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(ATMO_ERROR_DEVICE,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return ATMO_OK;
+}
+
 atmo_status validate_layers(int n, const atmo_density_layer* l,
                             const char* what) {
   if (n < 0 || n > 2 || (n > 0 && l == nullptr))
@@ -814,6 +849,36 @@ atmo_status atmo_model_init(atmo_model* m, unsigned orders, void* stream) {
     free_temporaries(m);
   }
   return ATMO_OK;
+}
+
+atmo_status atmo_model_init_static_scratch(atmo_model* m, unsigned orders, void* stream) {
+  if (m == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "model is NULL");
+  if (orders < 1) return fail(ATMO_ERROR_INVALID_ARGUMENT, "num_scattering_orders must be >= 1");
+  if (m->live_batches > 0)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT,
+                "model is in a live batch, whose launches use its own temporaries");
+  if (m->external_temporaries)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT,
+                "model's temporaries are carved from caller memory (a server model)");
+  const size_t need = temporaries_bytes(m);
+  if (need > kStaticScratchBytes)
+    return fail(ATMO_ERROR_INVALID_ARGUMENT,
+                "Init temporaries of this model take " + std::to_string(need) +
+                    " bytes; the static scratch holds " +
+                    std::to_string(kStaticScratchBytes) + " (use atmo_model_init)");
+  char* base = nullptr;
+  ATMO_TRY(static_scratch(m->device, &base));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // The passes write every temporary before reading it (the first pass of
+  // each writes, never accumulates), so the scratch needs no clearing.
+  use_external_temporaries(m, base);
+  const atmo_status st = enqueue_init(m, orders, s, false);
+  drop_external_temporaries(m);
+  // Retained temporaries of an earlier Init stay the model's; the job table
+  // points at them again at the next Init.
+  if (m->temp_block != nullptr)
+    carve(m->sizes, m->spec.half_precision, m->spec.combined, m->iters, m->temp_block, m);
+  return st;
 }
 
 atmo_status atmo_model_pass_timings(atmo_model* m, atmo_pass_timing* out, int capacity,

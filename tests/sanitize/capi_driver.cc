@@ -121,6 +121,8 @@ int main(int argc, char** argv) {
            "valid parameters");
     expect(atmo_model_destroy(nullptr) == ATMO_OK, "destroy NULL");
     expect(atmo_model_init(nullptr, 4, nullptr) == ATMO_ERROR_INVALID_ARGUMENT, "init NULL");
+    expect(atmo_model_init_static_scratch(nullptr, 4, nullptr) == ATMO_ERROR_INVALID_ARGUMENT,
+           "static init NULL");
     (void)nan;
   }
 

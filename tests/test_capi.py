@@ -208,3 +208,9 @@ def test_fp16_channel_scale_stays_finite():
         assert 1.0 / sc >= np.finfo(np.float32).tiny
     for mx in (0.0, -1.0, float("inf"), float("nan")):
         assert f(mx) == 1.0
+
+
+def test_static_scratch_size_matches_header():
+    text = open(os.path.join(ROOT, "include", "atmo.h")).read()
+    m = re.search(r"#define ATMO_STATIC_SCRATCH_BYTES \((\d+)u << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == _capi.STATIC_SCRATCH_BYTES
