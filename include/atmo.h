@@ -108,7 +108,15 @@ typedef enum atmo_lut_id {
   ATMO_LUT_DELTA_SCATTERING_DENSITY = 7,
 } atmo_lut_id;
 
-typedef enum atmo_dtype { ATMO_DTYPE_F32 = 0, ATMO_DTYPE_F16 = 1 } atmo_dtype;
+/* F16 texels hold value * scale[c]. RGB21 (the delta scattering density of
+ * FP16 models): 8 bytes per texel, three 21-bit floats (the FP32 sign,
+ * exponent and top 12 mantissa bits, rounded to nearest), channel c in bits
+ * 21c .. 21c + 20 of the little-endian 64-bit word. */
+typedef enum atmo_dtype {
+  ATMO_DTYPE_F32 = 0,
+  ATMO_DTYPE_F16 = 1,
+  ATMO_DTYPE_RGB21 = 2,
+} atmo_dtype;
 
 /* A LUT in device memory: width x height x depth texels of 'channels'
  * channels (4, RGBA; 3, RGB, for the delta scattering density), x fastest,
@@ -129,10 +137,14 @@ typedef enum atmo_option {
   ATMO_OPTION_RETAIN_INTERMEDIATES = 0, /* keep the delta LUTs after Init */
   ATMO_OPTION_PASS_TIMING = 1,          /* time every kernel launch of Init */
   /* Separate scattering textures only (default on; no counterpart in the
-   * reference): also keep S and S_Mie interleaved texel by texel for the
-   * lookups, which then fetch both with one access per tap. Costs the size
-   * of S and S_Mie again; the LUT views and files are unchanged. Turning it
-   * on for an initialized model builds the copy at once (synchronous). */
+   * reference): also keep a lookup copy of S and S_Mie in which, for every
+   * (r, mu) row and (nu, mu_s) anchor, the S and S_Mie texels of the 2 x 2
+   * (nu, mu_s) taps a lookup reads there are contiguous (64 B FP16, 128 B
+   * FP32), so that a 16-tap scattering lookup reads 4 contiguous blocks.
+   * Costs about 4 times the size of S and S_Mie (x (NU+1)(MU_S+1)/(NU MU_S));
+   * lookups give identical results with and without it; the LUT views and
+   * files are unchanged. Turning it on for an initialized model builds the
+   * copy at once (synchronous). */
   ATMO_OPTION_INTERLEAVED_LOOKUPS = 2,
 } atmo_option;
 
@@ -182,8 +194,8 @@ atmo_status atmo_model_init(atmo_model* model, unsigned num_scattering_orders,
  * static device array of ATMO_STATIC_SCRATCH_BYTES per GPU (held from the
  * library's first use on that GPU until the process ends) and calls no
  * hipMalloc or hipFree. A model whose temporaries exceed that size
- * (atmo_model_footprint's init_scratch; e.g. 50.3 MB for 3 wavelengths at
- * the reference LUT sizes in FP32, 587 MB for 15 wavelengths) is rejected
+ * (atmo_model_footprint's init_scratch; e.g. 46.2 MB for 3 wavelengths at
+ * the reference LUT sizes in FP32, 568 MB for 15 wavelengths) is rejected
  * with ATMO_ERROR_INVALID_ARGUMENT, as are models in a live batch or owned
  * by a server. Returns without waiting; the temporaries are not retained and
  * no pass timings are recorded. Every call on one GPU shares that GPU's

@@ -49,6 +49,19 @@ ATMO_HD T gload32(const T* p, uint32_t i) {
 #endif
 }
 
+// Load at an unsigned 32-bit byte offset from a uniform base (the same
+// scalar-base + vector-offset form, with the offset computed by the caller).
+template <class T>
+ATMO_HD T gload_at(const T* p, uint32_t byte_offset) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __attribute__((address_space(1))) char* b =
+      (const __attribute__((address_space(1))) char*)p;
+  return *(const __attribute__((address_space(1))) T*)(b + byte_offset);
+#else
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + byte_offset);
+#endif
+}
+
 namespace atmo {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -106,9 +119,11 @@ struct Job {
   float4* dE;
   float4* dR;  // delta Rayleigh; delta multiple scattering from order 2 on
   float4* dM;
-  // Delta scattering density: RGB texels (12 bytes; the reference's
-  // GL_RGB delta textures), read only by the multiple-scattering pass.
-  float3* dJ;
+  // Delta scattering density, read only by the multiple-scattering pass:
+  // RGB float texels (float3, 12 bytes; the reference's GL_RGB delta
+  // textures), or for FP16-storage jobs (half_luts) three 21-bit floats
+  // (uint2, 8 bytes, pack_f21x3).
+  void* dJ;
   // FP16 storage (half_precision): the scattering LUTs are accumulated in
   // FP32 in S / SM during Init, then stored as 4 x FP16 texels times a
   // per-channel power-of-two scale: S16[t].c = S[t].c * scale[c],
@@ -134,8 +149,8 @@ struct Job {
   int half_luts;
   uint2* S16;
   uint2* SM16;
-  // S and SM interleaved for the lookups: texel t's S at Q[2 t] and SM at
-  // Q[2 t + 1] (float4), or both in the uint4 Q[t] (FP16); null if absent.
+  // The lookup copy of S and SM (s_quad_taps): quads of 4 (S, SM) texel
+  // pairs, 8 float4 (FP32) or 4 uint4 (FP16) per quad; null if absent.
   void* Q;
   float* scales;    // 8 floats, written by the conversion kernel
   unsigned* maxbits;  // 8 words: per-channel max |value| (float bits)
@@ -251,6 +266,35 @@ ATMO_HD float4 unpack_half4(uint2 u) {
   return make_float4(static_cast<float>(h[0]), static_cast<float>(h[1]),
                      static_cast<float>(h[2]), static_cast<float>(h[3]));
 }
+// Delta J of FP16-storage models: three 21-bit floats in 64 bits. The
+// reference keeps its delta textures as GL_RGB16F in half-precision mode
+// (atmosphere/model.cc:872-891); this keeps the FP32 sign and 8-bit
+// exponent of each channel and rounds its mantissa to nearest (even) at 12
+// bits: relative error at most 2^-13 at any magnitude (plain FP16 would lose
+// the small texels of high orders, the night side, and channels far below
+// the texel's largest), in 8 bytes instead of 12. Channel c's 21 bits are
+// bits 21c .. 21c + 20 of the 64-bit word (x low, y high).
+ATMO_HD uint32_t f21_bits(float v) {
+  uint32_t b;
+  __builtin_memcpy(&b, &v, sizeof(b));
+  b += 0x3FFu + ((b >> 11) & 1u);
+  return b >> 11;
+}
+ATMO_HD uint2 pack_f21x3(float r, float g, float b) {
+  const uint32_t c0 = f21_bits(r), c1 = f21_bits(g), c2 = f21_bits(b);
+  return make_uint2(c0 | (c1 << 21), (c1 >> 11) | (c2 << 10));
+}
+ATMO_HD float3 unpack_f21x3(uint2 u) {
+  const uint32_t b0 = u.x << 11;
+  const uint32_t b1 = ((u.x >> 21) | (u.y << 11)) << 11;
+  const uint32_t b2 = (u.y >> 10) << 11;
+  float3 v;
+  __builtin_memcpy(&v.x, &b0, sizeof(float));
+  __builtin_memcpy(&v.y, &b1, sizeof(float));
+  __builtin_memcpy(&v.z, &b2, sizeof(float));
+  return v;
+}
+
 ATMO_HD uint2 pack_half4(float4 v) {
   _Float16 h[4] = {static_cast<_Float16>(v.x), static_cast<_Float16>(v.y),
                    static_cast<_Float16>(v.z), static_cast<_Float16>(v.w)};
@@ -452,39 +496,96 @@ ATMO_HD D4 s_lookup_d(const Job& j, const Texels& lut, const SCoords& c) {
   return acc;
 }
 
-// Interleaved S and SM texels (Job::Q): both texels of a tap in one place,
-// FP32 (float4 pairs) or FP16 (one uint4 of two 4 x FP16 texels, with the
-// per-channel inverse scales of S and SM).
-struct F32Pairs {
-  const float4* q;
-  ATMO_HD void load(int64_t i, float4* s, float4* m) const {
-    *s = gload(q, 2 * i);
-    *m = gload(q, 2 * i + 1);
+// The lookup copy of S and SM (Job::Q) as quads. For each (r, mu) row and
+// each anchor (s, i) in [-1, NU - 1] x [-1, MU_S - 1], a quad holds the four
+// texels that a bilinear (nu slice, mu_s) tap set anchored there reads, in
+// the tap order of s_taps: (s, i), (s + 1, i), (s, i + 1), (s + 1, i + 1),
+// each index clamped to the LUT as the taps clamp it, each texel an S and SM
+// pair. A scattering lookup (16 taps) then reads 4 contiguous quads (one per
+// (r, mu) tap: 128 B FP32, 64 B FP16) instead of 16 scattered texels, with
+// the same texels, weights and sums.
+ATMO_HD int64_t quad_count(int nr, int nmu, int nmus, int nnu) {
+  return static_cast<int64_t>(nr) * nmu * (nnu + 1) * (nmus + 1);
+}
+ATMO_HD int64_t quad_index(const Job& j, int z, int y, int s, int i) {
+  return ((static_cast<int64_t>(z) * j.nmu + y) * (j.nnu + 1) + (s + 1)) * (j.nmus + 1) +
+         (i + 1);
+}
+// Anchor and fraction along one axis: the floor of the texel-space
+// coordinate clamped to [-1, n], and its fraction; x = n takes anchor n - 1
+// with fraction 1 (both of its texels are n - 1, whose weight is then 1 as
+// in the taps, and the other 0).
+struct QuadAxis {
+  int a;
+  double w;
+};
+ATMO_HD QuadAxis quad_axis(double x, int n) {
+  if (!(x >= -1.0)) x = -1.0;
+  if (x > static_cast<double>(n)) x = static_cast<double>(n);
+  const double f = floor(x);
+  QuadAxis q{static_cast<int>(f), x - f};
+  if (q.a > n - 1) {
+    q.a = n - 1;
+    q.w = 1.0;
+  }
+  return q;
+}
+// The 4 quads of a lookup: f(quad index, weights of its 4 texels), the
+// weights the products of s_taps in the same order.
+template <class Real, class F>
+ATMO_HD void s_quad_taps(const Job& j, const SCoords& c, F&& f) {
+  const QuadAxis qs = quad_axis(c.nu, j.nnu), qx = quad_axis(c.x_in, j.nmus);
+  double xz = c.z, xy = c.y;
+  Taps tz = axis_taps(xz, j.nr), ty = axis_taps(xy, j.nmu);
+  auto frac = [](double x, int n) {
+    if (!(x >= -1.0)) x = -1.0;
+    if (x > static_cast<double>(n)) x = static_cast<double>(n);
+    return x - floor(x);
+  };
+  const double fz = frac(xz, j.nr), fy = frac(xy, j.nmu);
+  const int zs[2] = {tz.i0, tz.i1};
+  const int ys[2] = {ty.i0, ty.i1};
+  const Real wz[2] = {static_cast<Real>(1.0 - fz), static_cast<Real>(fz)};
+  const Real wy[2] = {static_cast<Real>(1.0 - fy), static_cast<Real>(fy)};
+  const Real wx[2] = {static_cast<Real>(1.0 - qx.w), static_cast<Real>(qx.w)};
+  const Real wsl[2] = {static_cast<Real>(1.0 - qs.w), static_cast<Real>(qs.w)};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      Real w[4];
+      for (int e = 0; e < 2; ++e)
+        for (int q = 0; q < 2; ++q) w[2 * e + q] = wz[a] * wy[b] * wx[e] * wsl[q];
+      f(quad_index(j, zs[a], ys[b], qs.a, qx.a), w);
+    }
+}
+struct F32Quads {
+  const float4* q;  // 8 float4 per quad: (S, SM) of its 4 texels
+  ATMO_HD void load(int64_t quad, int k, float4* s, float4* m) const {
+    *s = gload(q, 8 * quad + 2 * k);
+    *m = gload(q, 8 * quad + 2 * k + 1);
   }
 };
-struct F16Pairs {
-  const uint4* q;
+struct F16Quads {
+  const uint4* q;  // 4 uint4 per quad: S and SM of a texel as 4 x FP16 each
   float4 inv_s, inv_m;
-  ATMO_HD void load(int64_t i, float4* s, float4* m) const {
-    const uint4 u = gload(q, i);
+  ATMO_HD void load(int64_t quad, int k, float4* s, float4* m) const {
+    const uint4 u = gload(q, 4 * quad + k);
     const float4 a = unpack_half4(make_uint2(u.x, u.y));
     const float4 b = unpack_half4(make_uint2(u.z, u.w));
     *s = make_float4(a.x * inv_s.x, a.y * inv_s.y, a.z * inv_s.z, a.w * inv_s.w);
     *m = make_float4(b.x * inv_m.x, b.y * inv_m.y, b.z * inv_m.z, b.w * inv_m.w);
   }
 };
-
-// S and SM lookups at the same coordinates, one fetch per tap; the same
-// sums as two s_lookup_d calls.
-template <class Pairs>
-ATMO_HD void s_lookup_pair_d(const Job& j, const Pairs& lut, const SCoords& c, D4* s,
-                             D4* m) {
+template <class Quads>
+ATMO_HD void s_lookup_quads_d(const Job& j, const Quads& lut, const SCoords& c, D4* s,
+                              D4* m) {
   D4 as{0.0, 0.0, 0.0, 0.0}, am{0.0, 0.0, 0.0, 0.0};
-  s_taps(j, c, [&](int64_t i, double wt) {
-    float4 vs, vm;
-    lut.load(i, &vs, &vm);
-    acc_d4(&as, wt, vs);
-    acc_d4(&am, wt, vm);
+  s_quad_taps<double>(j, c, [&](int64_t quad, const double* w) {
+    for (int k = 0; k < 4; ++k) {
+      float4 vs, vm;
+      lut.load(quad, k, &vs, &vm);
+      acc_d4(&as, w[k], vs);
+      acc_d4(&am, w[k], vm);
+    }
   });
   *s = as;
   *m = am;
@@ -508,15 +609,17 @@ ATMO_HD D4 s_lookup_f(const Job& j, const Texels& lut, const SCoords& c) {
   return to_d4(acc);
 }
 
-template <class Pairs>
-ATMO_HD void s_lookup_pair_f(const Job& j, const Pairs& lut, const SCoords& c, D4* s,
-                             D4* m) {
+template <class Quads>
+ATMO_HD void s_lookup_quads_f(const Job& j, const Quads& lut, const SCoords& c, D4* s,
+                              D4* m) {
   float4 as = make_float4(0.0f, 0.0f, 0.0f, 0.0f), am = as;
-  s_taps<float>(j, c, [&](int64_t i, float wt) {
-    float4 vs, vm;
-    lut.load(i, &vs, &vm);
-    acc_f4(&as, wt, vs);
-    acc_f4(&am, wt, vm);
+  s_quad_taps<float>(j, c, [&](int64_t quad, const float* w) {
+    for (int k = 0; k < 4; ++k) {
+      float4 vs, vm;
+      lut.load(quad, k, &vs, &vm);
+      acc_f4(&as, w[k], vs);
+      acc_f4(&am, w[k], vm);
+    }
   });
   *s = to_d4(as);
   *m = to_d4(am);

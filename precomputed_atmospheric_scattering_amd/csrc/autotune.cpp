@@ -208,6 +208,7 @@ atmo_status tune_size(const atmo_lut_sizes& lz, bool do_sd, bool do_ms, hipStrea
     pl.min_w = w;
     pl.max_lds_w = w <= atmo::kMsMaxRow ? w : 0;
     pl.kinds = atmo::kJobsF32;
+    pl.dj_kinds = atmo::kDjF32;
     pl.tile = tile;
     auto launch = [&]() {
       return pass == kSD ? atmo::launch_scattering_density(pl, order, s)

@@ -23,7 +23,11 @@ struct PassLaunch {
   int kinds;    // kJobsF32 | kJobsAcc64: which kinds of jobs the launch holds
   int tile;     // tile shape of the jobs (scattering density, multiple
                 // scattering): every job of a launch has the same one
+  int dj_kinds;  // kDjF32 | kDjRgb21: the delta J formats of the jobs
 };
+
+constexpr int kDjF32 = 1;    // delta J as RGB float (FP32-storage jobs)
+constexpr int kDjRgb21 = 2;  // delta J as 3 x 21-bit floats (FP16-storage jobs)
 
 constexpr int kJobsF32 = 1;    // jobs accumulating S and SM in FP32
 constexpr int kJobsAcc64 = 2;  // illuminance-mode jobs with FP64 accumulators
@@ -63,7 +67,7 @@ constexpr int kMsTiles = 3;
 constexpr int kMsTpt[kMsTiles] = {1, 2, 4};
 constexpr int kMsMaxRow = 4 * kBlock;
 // Rows per block are capped at 4 (one texel per thread) or 8, which keeps
-// the block's LDS (2 x kBlock x tpt float4 + 3.2 KB of samples per row)
+// the block's LDS (2 x kBlock x tpt RGB floats + 3.2 KB of samples per row)
 // under 64 KB.
 ATMO_HD int ms_rows_per_block(int w, int tpt) {
   if (w > kMsMaxRow) return 0;
@@ -102,6 +106,7 @@ int blocks_scattering_density(const Job& j);
 int blocks_indirect_irradiance(const Job& j);
 int blocks_multiple_scattering(const Job& j);
 int blocks_half_conversion(const Job& j);
+int blocks_lookup_copy(const Job& j);
 int blocks_combine(const Job& j);
 
 // The optical depths of the jobs' transmittance texels into their OD (one
@@ -127,7 +132,7 @@ hipError_t launch_clear_max(const PassLaunch& p, hipStream_t s);
 // per job; the job's niter, S64, SM64 and E_it name the accumulators).
 hipError_t launch_acc64_to_f32(const PassLaunch& p, hipStream_t s);
 hipError_t launch_to_half(const PassLaunch& p, hipStream_t s);
-// The interleaved lookup copy of S and SM (blocks_half_conversion blocks per
+// The lookup copy of S and SM, in quads (blocks_lookup_copy blocks per
 // job).
 hipError_t launch_interleave(const PassLaunch& p, hipStream_t s);
 

@@ -584,13 +584,19 @@ k_scattering_density(
   const int plane = nmu * nch;  // threads per (r, mu_s) plane
   const int tiles = (plane + kB - 1) / kB;
   // The blocks of neighbouring mu_s columns of a layer write the same cache
-  // lines of delta J (a plane's texels are W texels apart): with
-  // XCD-contiguous ranges those blocks share one L2, where the partial
-  // lines merge before they are written back.
+  // lines of delta J (a plane's texels are W texels apart). The block order
+  // walks groups of G consecutive mu_s columns (G = the largest power of two
+  // dividing MU_S, at most 16; 16 x 12 B spans a 128 B line and more), layer
+  // by layer within a group, and each XCD takes a contiguous range of that
+  // order: the blocks writing a line share one L2, where the partial lines
+  // merge before they are written back, and each XCD gets planes of many
+  // layers (the ground-heavy low layers cost more).
+  const int G = min(nmus & -nmus, 16);
   const int lbx = xcd_contiguous(lb, tiles * nmus * J.nr);
   const int tile = lbx % tiles;
   const int pl = lbx / tiles;
-  const int imus = pl % nmus, k = pl / nmus;
+  const int im = pl % G, rest = pl / G;
+  const int k = rest % J.nr, imus = (rest / J.nr) * G + im;
   const int W = nnu * nmus;
   const bool two = order == 2;
   // Per theta a table of NU + 1 entries (table_lerp); then the
@@ -755,9 +761,13 @@ k_scattering_density(
 #pragma unroll
   for (int jn = 0; jn < kChunk; ++jn) {
     const int nui = ch * kChunk + jn;
-    if (nui < nnu)
-      gstore(J.dJ, (layer + y) * W + nui * nmus + imus,
-             make_float3(acc[jn][0], acc[jn][1], acc[jn][2]));
+    if (nui < nnu) {
+      const int64_t t = (layer + y) * W + nui * nmus + imus;
+      if (J.half_luts)
+        gstore(static_cast<uint2*>(J.dJ), t, pack_f21x3(acc[jn][0], acc[jn][1], acc[jn][2]));
+      else
+        gstore(static_cast<float3*>(J.dJ), t, make_float3(acc[jn][0], acc[jn][1], acc[jn][2]));
+    }
   }
 }
 
@@ -955,28 +965,38 @@ __device__ inline int2 ms_row(const Job& J, int p) {
 struct MsRowSample {
   double d;      // distance along the view ray
   double inv_r;  // 1 / r_i
-  int o00, o01, o10, o11;  // texel offsets of the (r, mu) source rows
+  // Byte offsets of the (r, mu) source rows in delta J (12-byte RGB float
+  // or 8-byte 3 x 21-bit texels; a LUT of at most 2^28 texels spans less
+  // than 2^32 bytes).
+  uint32_t o00, o01, o10, o11;
   float w00, w01, w10, w11;
   float f[3];    // trapezoid weight * dx * view transmittance
   float pad;
 };
 
-template <int kTpt>
-__global__ __launch_bounds__(kBlock) void k_multiple_scattering(
+// kF16: the kernel serves the jobs whose delta J is 3 x 21-bit floats
+// (FP16-storage models); otherwise the FP32 ones. Both are launched over the
+// same blocks when a launch holds both kinds; each returns at once on the
+// other's jobs.
+template <int kTpt, bool kF16, int kB = kBlock>
+__global__ __launch_bounds__(kB) void k_multiple_scattering(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
     int njobs, int max_rows) {
-  // Dynamic LDS: two buffers of kBlock * kTpt blended texels, then the
+  using Src = typename std::conditional<kF16, uint2, float3>::type;
+  // Dynamic LDS: two buffers of kB * kTpt blended texels, then the
   // samples of max_rows rows (the launch's largest rows per block).
+  // Blended texels are RGB (12 bytes), so that more blocks fit a CU's LDS.
   extern __shared__ float4 ms_lds[];
-  float4* const buf = ms_lds;
-  MsRowSample(*const smp)[kMarch] =
-      reinterpret_cast<MsRowSample(*)[kMarch]>(ms_lds + 2 * kBlock * kTpt);
+  float3* const buf = reinterpret_cast<float3*>(ms_lds);
+  MsRowSample(*const smp)[kMarch] = reinterpret_cast<MsRowSample(*)[kMarch]>(
+      reinterpret_cast<char*>(ms_lds) + 2 * kB * kTpt * sizeof(float3));
   const int ji = find_job(first_block, njobs, blockIdx.x);
   const Job& J = jobs[ji];
+  if ((J.half_luts != 0) != kF16) return;
   const int lb = blockIdx.x - first_block[ji];
   const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
   const int W = nnu * nmus;
-  const int rpb = ms_rows_per_block(W, kTpt);
+  const int rpb = ms_rows_per_block(W, kTpt * kB / kBlock);
   // Gather-path job (uniform per block). The launcher groups jobs by tile
   // and sizes max_rows over them; the tests only keep LDS indexing in
   // bounds should it not.
@@ -987,7 +1007,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
 
   // Per-row sample geometry: thread t < rpb * kMarch sets up sample
   // t % kMarch of row slot t / kMarch.
-  for (int e = tid; e < rpb * kMarch; e += kBlock) {
+  for (int e = tid; e < rpb * kMarch; e += kB) {
     const int slot = e / kMarch, i = e % kMarch;
     const int row = row0 + slot;
     if (row >= nrows) continue;
@@ -1008,10 +1028,11 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
     MsRowSample& S = smp[slot][i];
     S.d = d;
     S.inv_r = 1.0 / r_i;
-    S.o00 = (tz.i0 * nmu + ty.i0) * W;
-    S.o01 = (tz.i0 * nmu + ty.i1) * W;
-    S.o10 = (tz.i1 * nmu + ty.i0) * W;
-    S.o11 = (tz.i1 * nmu + ty.i1) * W;
+    const uint32_t row_bytes = static_cast<uint32_t>(W) * sizeof(Src);
+    S.o00 = static_cast<uint32_t>(tz.i0 * nmu + ty.i0) * row_bytes;
+    S.o01 = static_cast<uint32_t>(tz.i0 * nmu + ty.i1) * row_bytes;
+    S.o10 = static_cast<uint32_t>(tz.i1 * nmu + ty.i0) * row_bytes;
+    S.o11 = static_cast<uint32_t>(tz.i1 * nmu + ty.i1) * row_bytes;
     const float wz1 = tz.w, wz0 = 1.0f - tz.w, wy1 = ty.w, wy0 = 1.0f - ty.w;
     S.w00 = wz0 * wy0;
     S.w01 = wz0 * wy1;
@@ -1030,7 +1051,7 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   float acc[kTpt][3];
 #pragma unroll
   for (int j = 0; j < kTpt; ++j) {
-    const int f = tid + j * kBlock;
+    const int f = tid + j * kB;
     const int slot = f / W;
     slot_of[j] = (slot < rpb && row0 + slot < nrows) ? slot : -1;
     x_of[j] = f - slot * W;
@@ -1065,10 +1086,14 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
   __syncthreads();
 
   const MusMap mus_map(J);
-  const float3* __restrict__ D = J.dJ;
+  const Src* __restrict__ D = static_cast<const Src*>(J.dJ);
+  uint32_t x_bytes[kTpt];
+#pragma unroll
+  for (int j = 0; j < kTpt; ++j)
+    x_bytes[j] = static_cast<uint32_t>(x_of[j]) * static_cast<uint32_t>(sizeof(Src));
   // Source texels of the next sample, loaded one sample ahead so that the
   // loads are in flight during the barrier and the gather of the current one.
-  float3 src[kTpt][4];
+  Src src[kTpt][4];
   auto fetch = [&](int i) {
 #pragma unroll
     for (int j = 0; j < kTpt; ++j) {
@@ -1076,28 +1101,41 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
       // the loads need no branch.
       const bool on = slot_of[j] >= 0;
       const MsRowSample& S = smp[on ? slot_of[j] : 0][i];
-      const int x = on ? x_of[j] : 0;
-      src[j][0] = gload32(D, static_cast<uint32_t>(S.o00 + x));
-      src[j][1] = gload32(D, static_cast<uint32_t>(S.o01 + x));
-      src[j][2] = gload32(D, static_cast<uint32_t>(S.o10 + x));
-      src[j][3] = gload32(D, static_cast<uint32_t>(S.o11 + x));
+      const uint32_t xb = on ? x_bytes[j] : 0u;
+      src[j][0] = gload_at(D, S.o00 + xb);
+      src[j][1] = gload_at(D, S.o01 + xb);
+      src[j][2] = gload_at(D, S.o10 + xb);
+      src[j][3] = gload_at(D, S.o11 + xb);
     }
   };
   fetch(0);
   for (int i = 0; i < kMarch; ++i) {
-    float4* B = buf + (i & 1) * (kBlock * kTpt);
+    float3* B = buf + (i & 1) * (kB * kTpt);
     // Blend the four (r, mu) source rows of sample i into LDS.
 #pragma unroll
     for (int j = 0; j < kTpt; ++j) {
       if (slot_of[j] < 0) continue;
       const MsRowSample& S = smp[slot_of[j]][i];
-      const float3 a = src[j][0], b = src[j][1], c = src[j][2], e = src[j][3];
-      float4 v;
-      v.x = fmaf(S.w11, e.x, fmaf(S.w10, c.x, fmaf(S.w01, b.x, S.w00 * a.x)));
-      v.y = fmaf(S.w11, e.y, fmaf(S.w10, c.y, fmaf(S.w01, b.y, S.w00 * a.y)));
-      v.z = fmaf(S.w11, e.z, fmaf(S.w10, c.z, fmaf(S.w01, b.z, S.w00 * a.z)));
-      v.w = 0.0f;
-      B[tid + j * kBlock] = v;
+      float3 a, b, c, e;
+      const float w00 = S.w00, w01 = S.w01, w10 = S.w10, w11 = S.w11;
+      if (kF16) {
+        const uint2* u = reinterpret_cast<const uint2*>(src[j]);
+        a = unpack_f21x3(u[0]);
+        b = unpack_f21x3(u[1]);
+        c = unpack_f21x3(u[2]);
+        e = unpack_f21x3(u[3]);
+      } else {
+        const float3* f = reinterpret_cast<const float3*>(src[j]);
+        a = f[0];
+        b = f[1];
+        c = f[2];
+        e = f[3];
+      }
+      float3 v;
+      v.x = fmaf(w11, e.x, fmaf(w10, c.x, fmaf(w01, b.x, w00 * a.x)));
+      v.y = fmaf(w11, e.y, fmaf(w10, c.y, fmaf(w01, b.y, w00 * a.y)));
+      v.z = fmaf(w11, e.z, fmaf(w10, c.z, fmaf(w01, b.z, w00 * a.z)));
+      B[tid + j * kB] = v;
     }
     if (i + 1 < kMarch) fetch(i + 1);
     // One barrier per sample: buffer i & 1 is written again at sample i + 2,
@@ -1115,8 +1153,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering(
       float wx;
       const int i0 = split_coord(xin, &wx);
       const int i1 = min(i0 + 1, nmus - 1);
-      const float4 p = B[c0_of[j] + i0], q = B[c0_of[j] + i1];
-      const float4 u = B[c1_of[j] + i0], v = B[c1_of[j] + i1];
+      const float3 p = B[c0_of[j] + i0], q = B[c0_of[j] + i1];
+      const float3 u = B[c1_of[j] + i0], v = B[c1_of[j] + i1];
       const float ws = ws_of[j];
       const float lo0 = fmaf(wx, q.x - p.x, p.x), hi0 = fmaf(wx, v.x - u.x, u.x);
       const float lo1 = fmaf(wx, q.y - p.y, p.y), hi1 = fmaf(wx, v.y - u.y, u.y);
@@ -1227,7 +1265,8 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
   const int cs0 = s0 * nmus, cs1 = s1 * nmus;
   const double rmus = r * mu_s;
   const MusMap mus_map(J);
-  const float3* __restrict__ D = J.dJ;
+  const void* const D = J.dJ;
+  const bool f16 = J.half_luts != 0;
   float a0 = 0.f, a1 = 0.f, a2 = 0.f;
   for (int s = 0; s < kMarch; ++s) {
     const MsSample& S = smp[s];
@@ -1251,7 +1290,9 @@ __global__ __launch_bounds__(kBlock) void k_multiple_scattering_gather(
       float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const float3 v = gload(D, rows[q] + cols[c]);
+        const int64_t t = rows[q] + cols[c];
+        const float3 v = f16 ? unpack_f21x3(gload(static_cast<const uint2*>(D), t))
+                             : gload(static_cast<const float3*>(D), t);
         r0 = fmaf(wc[c], v.x, r0);
         r1 = fmaf(wc[c], v.y, r1);
         r2 = fmaf(wc[c], v.z, r2);
@@ -1325,25 +1366,38 @@ __global__ __launch_bounds__(kBlock) void k_acc64_to_f32(const Job* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// The interleaved lookup copy of S and SM (ATMO_OPTION_INTERLEAVED_LOOKUPS):
-// texel t's S and SM side by side, so that a lookup tap reads both from one
-// place. FP16 models interleave their FP16 texels.
+// The lookup copy of S and SM (ATMO_OPTION_INTERLEAVED_LOOKUPS), in quads
+// (s_quad_taps): one thread per quad writes the (S, SM) pairs of its 4
+// texels, FP32 or, for FP16 models, their FP16 texels.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_interleave(const Job* __restrict__ jobs,
                                                        const int* __restrict__ first_block,
                                                        int njobs) {
   const int ji = find_job(first_block, njobs, blockIdx.x);
   const Job& J = jobs[ji];
-  const int64_t n = static_cast<int64_t>(J.nr) * J.nmu * J.nmus * J.nnu;
-  const int64_t t = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
-  if (t >= n || J.Q == nullptr || J.combined) return;
-  if (J.half_luts) {
-    const uint2 a = gload(J.S16, t), b = gload(J.SM16, t);
-    gstore(static_cast<uint4*>(J.Q), t, make_uint4(a.x, a.y, b.x, b.y));
-  } else {
-    float4* q = static_cast<float4*>(J.Q);
-    gstore(q, 2 * t, gload(J.S, t));
-    gstore(q, 2 * t + 1, gload(J.SM, t));
+  const int64_t n = quad_count(J.nr, J.nmu, J.nmus, J.nnu);
+  const int64_t qd = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
+  if (qd >= n || J.Q == nullptr || J.combined) return;
+  // Quad qd = ((z * MU + y) * (NU + 1) + s + 1) * (MU_S + 1) + i + 1.
+  const int ia = static_cast<int>(qd % (J.nmus + 1)) - 1;
+  const int64_t rest = qd / (J.nmus + 1);
+  const int sa = static_cast<int>(rest % (J.nnu + 1)) - 1;
+  const int64_t row = rest / (J.nnu + 1);
+  for (int e = 0; e < 2; ++e) {
+    const int i = min(max(ia + e, 0), J.nmus - 1);
+    for (int q = 0; q < 2; ++q) {
+      const int sl = min(max(sa + q, 0), J.nnu - 1);
+      const int64_t t = row * (J.nnu * J.nmus) + sl * J.nmus + i;
+      const int k = 2 * e + q;
+      if (J.half_luts) {
+        const uint2 a = gload(J.S16, t), b = gload(J.SM16, t);
+        gstore(static_cast<uint4*>(J.Q), 4 * qd + k, make_uint4(a.x, a.y, b.x, b.y));
+      } else {
+        float4* out = static_cast<float4*>(J.Q);
+        gstore(out, 8 * qd + 2 * k, gload(J.S, t));
+        gstore(out, 8 * qd + 2 * k + 1, gload(J.SM, t));
+      }
+    }
   }
 }
 
@@ -1484,6 +1538,9 @@ int blocks_combine(const Job& j) {
   const int64_t ne = static_cast<int64_t>(j.ew) * j.eh;
   return static_cast<int>(((n > ne ? n : ne) + kBlock - 1) / kBlock);
 }
+int blocks_lookup_copy(const Job& j) {
+  return static_cast<int>((quad_count(j.nr, j.nmu, j.nmus, j.nnu) + kBlock - 1) / kBlock);
+}
 int blocks_half_conversion(const Job& j) {
   const int64_t n = static_cast<int64_t>(j.nr) * j.nmu * j.nmus * j.nnu;
   return static_cast<int>((n + kBlock - 1) / kBlock);
@@ -1568,15 +1625,19 @@ hipError_t launch_to_half(const PassLaunch& p, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int kTpt>
+template <int kTpt, int kB = kBlock>
 void launch_ms_tile(const PassLaunch& p, hipStream_t s) {
   // Rows per block shrink as rows grow: the shortest row sets the launch's
   // largest count.
-  const int rows = ms_rows_per_block(p.min_w, kTpt);
-  const size_t lds = sizeof(float4) * 2 * kBlock * kTpt +
+  const int rows = ms_rows_per_block(p.min_w, kTpt * kB / kBlock);
+  const size_t lds = sizeof(float3) * 2 * kB * kTpt +
                      sizeof(MsRowSample) * kMarch * static_cast<size_t>(rows);
-  hipLaunchKernelGGL(k_multiple_scattering<kTpt>, dim3(p.total_blocks), dim3(kBlock), lds, s,
-                     p.jobs, p.first_block, p.njobs, rows);
+  if (p.dj_kinds & kDjF32)
+    hipLaunchKernelGGL((k_multiple_scattering<kTpt, false, kB>), dim3(p.total_blocks),
+                       dim3(kB), lds, s, p.jobs, p.first_block, p.njobs, rows);
+  if (p.dj_kinds & kDjRgb21)
+    hipLaunchKernelGGL((k_multiple_scattering<kTpt, true, kB>), dim3(p.total_blocks),
+                       dim3(kB), lds, s, p.jobs, p.first_block, p.njobs, rows);
 }
 
 hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s) {

@@ -56,23 +56,24 @@ __device__ D4 combined_scattering(const Job& J, double r, double mu, double mu_s
   auto lookup = [&](const auto& lut) {
     return kF32 ? s_lookup_f(J, lut, c) : s_lookup_d(J, lut, c);
   };
-  auto lookup_pair = [&](const auto& lut, D4* s, D4* m) {
+  auto lookup_quads = [&](const auto& lut, D4* s, D4* m) {
     if (kF32)
-      s_lookup_pair_f(J, lut, c, s, m);
+      s_lookup_quads_f(J, lut, c, s, m);
     else
-      s_lookup_pair_d(J, lut, c, s, m);
+      s_lookup_quads_d(J, lut, c, s, m);
   };
   D4 s;
   if (J.Q != nullptr && !J.combined) {
-    // Interleaved copy: the S and SM texels of a tap in one fetch.
+    // Lookup copy: the S and SM texels of the 4 (nu, mu_s) taps of each
+    // (r, mu) tap in one contiguous quad.
     if (J.half_luts) {
       const float* sc = J.scales;
-      const F16Pairs pt{static_cast<const uint4*>(J.Q),
+      const F16Quads qt{static_cast<const uint4*>(J.Q),
                         make_float4(1.0f / sc[0], 1.0f / sc[1], 1.0f / sc[2], 1.0f / sc[3]),
                         make_float4(1.0f / sc[4], 1.0f / sc[5], 1.0f / sc[6], 0.0f)};
-      lookup_pair(pt, &s, single_mie);
+      lookup_quads(qt, &s, single_mie);
     } else {
-      lookup_pair(F32Pairs{static_cast<const float4*>(J.Q)}, &s, single_mie);
+      lookup_quads(F32Quads{static_cast<const float4*>(J.Q)}, &s, single_mie);
     }
     *single_mie = rgb(*single_mie);
     return s;

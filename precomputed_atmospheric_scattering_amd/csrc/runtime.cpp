@@ -186,7 +186,7 @@ int pass_blocks(Pass p, const atmo::Job& j) {
     case kHalf: return atmo::blocks_half_conversion(j);
     case kAcc: return atmo::blocks_combine(j);
     case kClear: return 1;
-    case kPack: return atmo::blocks_half_conversion(j);
+    case kPack: return atmo::blocks_lookup_copy(j);
     case kOD: return atmo::blocks_optical_depth(j);
     default: return 0;
   }
@@ -232,7 +232,8 @@ size_t carve(const Sizes& z, bool half, bool combined, int iters, char* base,
   char* de = take(P * z.e_texels() * sizeof(float4));
   char* dr = take(P * z.s_texels() * sizeof(float4));
   char* dm = take(P * z.s_texels() * sizeof(float4));
-  char* dj = take(P * z.s_texels() * sizeof(float3));
+  // Delta J: RGB float, or 3 x 21-bit floats in FP16 mode.
+  char* dj = take(P * z.s_texels() * (half ? sizeof(uint2) : sizeof(float3)));
   char* s32 = half ? take(z.s_texels() * sizeof(float4)) : nullptr;
   char* sm32 = half && !combined ? take(z.s_texels() * sizeof(float4)) : nullptr;
   char* s64 = par ? take(P * z.s_texels() * 4 * sizeof(double)) : nullptr;
@@ -244,12 +245,13 @@ size_t carve(const Sizes& z, bool half, bool combined, int iters, char* base,
     m->dE_it = reinterpret_cast<float4*>(de);
     m->dR_it = reinterpret_cast<float4*>(dr);
     m->dM_it = reinterpret_cast<float4*>(dm);
-    m->dJ_it = reinterpret_cast<float3*>(dj);
+    m->dJ_it = dj;
     // The retained intermediates: the last iteration's delta LUTs.
     m->dE = m->dE_it + (P - 1) * z.e_texels();
     m->dR = m->dR_it + (P - 1) * z.s_texels();
     m->dM = m->dM_it + (P - 1) * z.s_texels();
-    m->dJ = m->dJ_it + (P - 1) * z.s_texels();
+    m->dJ = dj == nullptr ? nullptr
+                          : dj + (P - 1) * z.s_texels() * (half ? sizeof(uint2) : sizeof(float3));
     if (half) {
       m->S = reinterpret_cast<float4*>(s32);
       m->SM = reinterpret_cast<float4*>(sm32);
@@ -354,7 +356,9 @@ void fill_job_luts(atmo_model* m, atmo::Job* j, int k) {
   j->dE = temps ? m->dE_it + kk * z.e_texels() : nullptr;
   j->dR = temps ? m->dR_it + kk * z.s_texels() : nullptr;
   j->dM = temps ? m->dM_it + kk * z.s_texels() : nullptr;
-  j->dJ = temps ? m->dJ_it + kk * z.s_texels() : nullptr;
+  j->dJ = temps ? m->dJ_it + kk * z.s_texels() *
+                                (m->spec.half_precision ? sizeof(uint2) : sizeof(float3))
+               : nullptr;
   j->niter = m->iters;
   j->OD = m->OD;  // every job of the model, the final transmittance's too
   set_sizes(j, m->sizes);
@@ -431,6 +435,7 @@ atmo::PassLaunch model_launch(const atmo_model* m, int first_job, int njobs, Pas
   l.min_w = l.max_w;
   l.max_lds_w = l.max_w <= atmo::kMsMaxRow ? l.max_w : 0;
   l.kinds = accumulates_fp64(m->spec) ? atmo::kJobsAcc64 : atmo::kJobsF32;
+  l.dj_kinds = m->spec.half_precision ? atmo::kDjRgb21 : atmo::kDjF32;
   const atmo::Job& j = m->jobs[first_job];
   l.tile = p == kSD ? j.sd_tile : (p == kMS ? j.ms_tile : 0);
   return l;
@@ -945,6 +950,7 @@ atmo_status atmo_model_get_lut(const atmo_model* m, atmo_lut_id id, atmo_lut_vie
     case ATMO_LUT_DELTA_SCATTERING_DENSITY:
       v->data = m->dJ;
       v->channels = 3;  // RGB texels
+      if (m->spec.half_precision) v->dtype = ATMO_DTYPE_RGB21;
       break;
     default: return fail(ATMO_ERROR_INVALID_ARGUMENT, "unknown LUT id");
   }
@@ -959,6 +965,18 @@ atmo_status atmo_model_copy_lut(const atmo_model* m, atmo_lut_id id, float* host
   if (host == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "host is NULL");
   ATMO_HIP(hipDeviceSynchronize());
   const size_t texels = static_cast<size_t>(v.width) * v.height * v.depth;
+  if (v.dtype == ATMO_DTYPE_RGB21) {
+    std::vector<uint2> raw(texels);
+    ATMO_HIP(hipMemcpy(raw.data(), v.data, texels * sizeof(uint2), hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < texels; ++t) {
+      const float3 f = atmo::unpack_f21x3(raw[t]);
+      host[4 * t] = f.x;
+      host[4 * t + 1] = f.y;
+      host[4 * t + 2] = f.z;
+      host[4 * t + 3] = 0.0f;
+    }
+    return ATMO_OK;
+  }
   if (v.dtype == ATMO_DTYPE_F32 && v.channels == 3) {
     std::vector<float> rgb(texels * 3);
     ATMO_HIP(hipMemcpy(rgb.data(), v.data, texels * 3 * sizeof(float),
@@ -1327,6 +1345,7 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
     l.min_w = std::numeric_limits<int>::max();
     l.max_lds_w = 0;
     l.kinds = 0;
+    l.dj_kinds = 0;
     int total = 0;
     for (const auto& mj : sel) {
       const atmo::Job& j = models[mj.first]->jobs[mj.second];
@@ -1340,6 +1359,7 @@ void plan_levels(atmo_model* const* models, const unsigned* orders, int count,
       if (j.nnu * j.nmus <= atmo::kMsMaxRow)
         l.max_lds_w = std::max(l.max_lds_w, j.nnu * j.nmus);
       l.kinds |= j.acc64 ? atmo::kJobsAcc64 : atmo::kJobsF32;
+      l.dj_kinds |= j.half_luts ? atmo::kDjRgb21 : atmo::kDjF32;
     }
     plan->first_block.push_back(total);
     l.total_blocks = total;
@@ -1435,6 +1455,7 @@ atmo_status launch_level(const Level& l, const atmo::Job* d_jobs, const int* d_f
   pl.min_w = l.min_w;
   pl.max_lds_w = l.max_lds_w;
   pl.kinds = l.kinds;
+  pl.dj_kinds = l.dj_kinds;
   pl.tile = l.tile;
   hipError_t e = hipSuccess;
   switch (l.pass) {

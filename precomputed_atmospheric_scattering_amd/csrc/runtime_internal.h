@@ -46,7 +46,7 @@ struct Level {
   Pass pass;
   int order;  // SD: order computed; II: order read; others unused
   int job_offset, njobs, fb_offset, total_blocks, max_nnu, max_ew, max_w, min_w, max_lds_w,
-      kinds;
+      kinds, dj_kinds;
   int tile;   // SD and MS: the tile shape shared by the level's jobs
 };
 
@@ -85,12 +85,13 @@ struct atmo_model {
   float4 *T_it = nullptr, *E_it = nullptr;
   float4 *dE_it = nullptr, *dR_it = nullptr, *dM_it = nullptr;
   float4 *dE = nullptr, *dR = nullptr, *dM = nullptr;
-  float3 *dJ_it = nullptr, *dJ = nullptr;  // delta J: RGB texels
+  // Delta J: RGB float texels, or 3 x 21-bit floats in FP16 mode.
+  char *dJ_it = nullptr, *dJ = nullptr;
   // FP16 storage: resident half LUTs; S and SM above are then FP32
   // accumulators that live only during Init, like the deltas.
   uint2 *S16 = nullptr, *SM16 = nullptr;
-  // ATMO_OPTION_INTERLEAVED_LOOKUPS: S and SM interleaved texel by texel
-  // (float4 pairs, or one uint4 of two FP16 texels), for the lookups.
+  // ATMO_OPTION_INTERLEAVED_LOOKUPS: the lookup copy of S and SM, in quads
+  // of the texels a (nu, mu_s) tap set reads (interleaved_bytes).
   bool interleave = true;
   void* Q = nullptr;
   unsigned last_orders = 4;
@@ -141,10 +142,13 @@ atmo_status validate_sizes(const atmo_lut_sizes& z);
 int num_iterations(const Spec& s);
 // Whether Init accumulates S and SM in FP64 (illuminance mode).
 inline bool accumulates_fp64(const Spec& s) { return s.num_precomputed_wavelengths > 3; }
-// Bytes of the interleaved lookup copy of a model with these sizes and
-// storage (0 with combined textures, which have no S_Mie).
+// Bytes of the lookup copy of a model with these sizes and storage: quads
+// of 4 (S, SM) texel pairs (64 B FP16, 128 B FP32) per (r, mu) row and
+// anchor (atmo::quad_count); 0 with combined textures, which have no S_Mie.
 inline size_t interleaved_bytes(const Sizes& z, bool half, bool combined) {
-  return combined ? 0 : 2 * z.s_texels() * (half ? sizeof(uint2) : sizeof(float4));
+  return combined ? 0
+                  : static_cast<size_t>(atmo::quad_count(z.nr, z.nmu, z.nmus, z.nnu)) *
+                        (half ? 4 * sizeof(uint4) : 8 * sizeof(float4));
 }
 int pass_blocks(Pass p, const atmo::Job& j);
 // Tile shapes for a model's LUT sizes: the autotuner's choice, or the

@@ -6,6 +6,7 @@
 // table (set/get validation, save, and load of malformed files) and
 // atmo_last_error. Exits 0 if every call returns the expected status; the
 // sanitizers abort on the first error.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -14,6 +15,7 @@
 #include <vector>
 
 #include "../../include/atmo.h"
+#include "../../precomputed_atmospheric_scattering_amd/csrc/atmo_device.h"
 
 namespace {
 
@@ -165,6 +167,33 @@ int main(int argc, char** argv) {
     expect(std::isfinite(sc) && sc > 0.0f && std::frexp(sc, &e) == 0.5f &&
                std::isfinite(1.0f / sc),
            "fp16 scale");
+  }
+
+  // 3 x 21-bit float texels (delta J of FP16 models): each channel keeps
+  // its FP32 exponent and 12 mantissa bits rounded to nearest, so the
+  // relative error is at most 2^-13 at any magnitude (normal FP32 values),
+  // whatever the other channels; zero and signs are kept.
+  {
+    const float vals[] = {0.0f, 1.17549435e-38f, 3e-20f, 1e-7f, 0.37f, 1.0f, 1.9999999f,
+                          6.5e4f, 1e30f, 3.0e38f, -2.5f, 0.1f, 0.3333333f};
+    for (float x : vals)
+      for (float y : vals)
+        for (float z : {0.0f, 7.0e-12f, 123.456f}) {
+          const float in[3] = {x, y, z};
+          const float3 out = atmo::unpack_f21x3(atmo::pack_f21x3(x, y, z));
+          const float o[3] = {out.x, out.y, out.z};
+          for (int c = 0; c < 3; ++c)
+            expect(std::fabs(o[c] - in[c]) <= std::fabs(in[c]) * 1.2207031e-4f &&
+                       std::signbit(o[c]) == std::signbit(in[c]),
+                   "3 x 21-bit float round trip");
+        }
+    // Rounding is to nearest: 1 + 2^-13 rounds to 1, 1 + 3 * 2^-13 up to
+    // 1 + 2^-11, and a value already on the 12-bit grid is kept exactly.
+    const float3 q = atmo::unpack_f21x3(atmo::pack_f21x3(1.0f + 0.00012207031f,
+                                                        1.0f + 3.0f * 0.00012207031f,
+                                                        1.0f + 0.00048828125f));
+    expect(q.x == 1.0f && q.y == 1.0f + 0.00048828125f && q.z == 1.0f + 0.00048828125f,
+           "3 x 21-bit rounding");
   }
 
   // Tile table: validation, save, and loads of malformed files.

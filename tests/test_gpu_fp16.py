@@ -98,3 +98,47 @@ def test_combined_textures_with_other_options(spectral, half):
     for name, e in errs.items():
         tol = parity.TOL_FP16 if half and name in HALF else parity.TOL_FP32
         assert e <= tol, (name, e)
+
+
+def test_fp16_delta_j_21bit_pass_local():
+    """FP16 models keep delta J as three 21-bit floats per texel (the FP32
+    sign and exponent and 12 mantissa bits, ATMO_DTYPE_RGB21). Pass-local at
+    the reference sizes (spectral atmosphere, last Precompute iteration): the
+    density pass's output, decoded, against the oracle's density from the
+    GPU's order-1 deltas at 2000 texels (rounding 2^-13 plus the FP32 pass
+    error), and the multiple-scattering pass against the oracle's on that
+    decoded delta J (FP32 bar: the decoded values enter the sums exactly)."""
+    import ctypes as C
+    import oracle_lib as ol
+    cfg = configs.spectral(half_precision=True)
+    spec, s, _ = parity.oracle_spec(cfg)
+    kw, _ = configs.model_kwargs(cfg)
+    m = atmo.Model(retain_intermediates=True, **kw)
+    lambdas, _ = m.params.iteration(m.params.num_iterations() - 1)
+    a = ol.lower(spec, lambdas)
+    T = np.zeros(ol.transmittance_shape(s))
+    ol.lib().or_pass_transmittance(C.byref(a), s, ol.ptr(T))
+    n_s = s.scattering_r * s.scattering_mu * s.scattering_mu_s * s.scattering_nu
+    rng = np.random.Generator(np.random.PCG64(17))
+    idx = np.ascontiguousarray(rng.integers(0, n_s, 2000), dtype=np.int32)
+    ip = idx.ctypes.data_as(C.POINTER(C.c_int))
+    f64 = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    m.init(1)
+    dR, dM, dE = (f64(m.lut(k)) for k in ("delta_rayleigh", "delta_mie", "delta_irradiance"))
+    m.init(2)
+    v = m.lut_view("delta_scattering_density")
+    assert v.dtype == 2 and v.channels == 3  # ATMO_DTYPE_RGB21
+    dJ = f64(m.lut("delta_scattering_density"))
+    J = np.zeros((len(idx), 4))
+    ol.lib().or_texels_scattering_density(C.byref(a), s, ol.ptr(T), ol.ptr(dR), ol.ptr(dM),
+                                          ol.ptr(dR), ol.ptr(dE), 2, len(idx), ip, ol.ptr(J))
+    e_sd = parity.rel_err(dJ.reshape(-1, 4)[idx], J)[0]
+    ms = np.zeros((len(idx), 4))
+    ol.lib().or_texels_multiple_scattering(C.byref(a), s, ol.ptr(T), ol.ptr(dJ), len(idx), ip,
+                                           ol.ptr(ms))
+    e_ms = parity.rel_err(m.lut("delta_rayleigh").reshape(-1, 4)[idx], ms)[0]
+    print("FP16 model: 21-bit delta J vs oracle density %.2e, multiple "
+          "scattering on it %.2e" % (e_sd, e_ms), flush=True)
+    m.close()
+    assert e_sd <= 2.0 ** -13 + 1e-4, e_sd
+    assert e_ms <= parity.TOL_FP32, e_ms

@@ -1,7 +1,7 @@
 """The allocation-free Init (atmo_model_init_static_scratch) against Init.
 
 Two calls in a row share the GPU's static scratch: an Earth-like model at the
-reference LUT sizes (FP32, 4 orders; 50.3 MB of temporaries), then a smaller
+reference LUT sizes (FP32, 4 orders; 46.2 MB of temporaries), then a smaller
 one (quarter sizes, FP16 storage, 3 orders), which runs on the first call's
 leftovers. Each must give the LUTs and lookups of atmo_model_init bit for bit.
 A model whose temporaries exceed the scratch is rejected.
@@ -70,7 +70,7 @@ def test_static_scratch_init_equals_init_twice(first, second):
 
 
 def test_static_scratch_rejects_large_temporaries():
-    # 15 precomputed wavelengths: 5 iterations of temporaries, 587 MB.
+    # 15 precomputed wavelengths: 5 iterations of temporaries, 568 MB.
     m = parity.gpu_model(configs.spectral(half_precision=False))
     assert scratch_bytes(m) > _capi.STATIC_SCRATCH_BYTES
     stream = atmo.Stream()
