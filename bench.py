@@ -434,8 +434,9 @@ def section_high_resolution(args, steps, warmup, with_lookups=True, keep_luts=Fa
         "init_kept_temporaries_ms": round(kept, 3),
         "tiles": {"scattering_density": "%d nu x %d threads" % (
                       tiles["sd_nu_per_thread"], tiles["sd_block_threads"]),
-                  "multiple_scattering": "%d texels per thread, %s rows" % (
-                      tiles["ms_texels_per_thread"], tiles["ms_row_order"])},
+                  "multiple_scattering": "%d texels x %d threads, %s rows" % (
+                      tiles["ms_texels_per_thread"], tiles["ms_block_threads"],
+                      tiles["ms_row_order"])},
         "footprint_bytes": {"resident": resident, "init_temporaries": temporaries},
     }
     if lk is not None:

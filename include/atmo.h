@@ -235,8 +235,8 @@ void atmo_batch_destroy(atmo_batch* batch);
 /* Tile shapes and their autotuner (no counterpart in the reference, whose
  * passes are fragment-shader draws). The scattering-density pass runs with
  * sd_nu_per_thread nu outputs per thread in blocks of sd_block_threads; the
- * multiple-scattering pass with ms_texels_per_thread texels per thread,
- * its blocks walking the (r, mu) rows of the LUT r-major (ms_row_order 0, the
+ * multiple-scattering pass with ms_texels_per_thread texels per thread in
+ * blocks of ms_block_threads, its blocks walking the (r, mu) rows of the LUT r-major (ms_row_order 0, the
  * LUT's own order), mu-major (1) or in ray order (2: rows sorted by the
  * impact parameter r sqrt(1 - mu^2) of their view ray, so that blocks that
  * run together read the same source rows).
@@ -246,20 +246,23 @@ void atmo_batch_destroy(atmo_batch* batch);
  * shared by the process; a model's Init, batch or serving request reads it
  * when its job table is built. Classes never tuned or set use the defaults
  * (4 nu per thread in blocks of 128 threads when MU x ceil(NU / 4) <= 256,
- * else 256; the fewest texels per thread that hold a row; ray order once
+ * else 256; the smallest multiple-scattering tile that holds a row (1 x 256,
+ * 2 x 256, then 2 texels x 512 threads); ray order once
  * the R x MU x MU_S x NU RGB float density LUT reaches 64 MiB, r-major
  * below). */
 typedef struct atmo_tiles {
   int sd_nu_per_thread;
   int sd_block_threads;
   int ms_texels_per_thread;
+  int ms_block_threads;
   int ms_row_order;
   int tuned; /* atmo_get_tiles: bit 0 density class, bit 1 multiple class set */
 } atmo_tiles;
 /* Candidate shapes (returns the count; fills up to capacity entries). */
 int atmo_sd_tile_candidates(int* nu_per_thread, int* block_threads,
                             int capacity);
-int atmo_ms_tile_candidates(int* texels_per_thread, int capacity);
+int atmo_ms_tile_candidates(int* texels_per_thread, int* block_threads,
+                            int capacity);
 /* Times every candidate shape of the two passes on a batch of Earth-like
  * atmospheres of each listed size (enough copies to fill the GPU) on
  * 'stream', and records the fastest shape per size class; a shape replaces
@@ -269,13 +272,13 @@ atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
                           void* stream);
 atmo_status atmo_get_tiles(const atmo_lut_sizes* sizes, atmo_tiles* out);
 /* Pins the shapes of the size classes of 'sizes' (the tuned field is
- * ignored); an error if a shape is not a candidate, ms_row_order is not 0 or
- * 1, or if NU x MU_S texels
- * per row exceed 256 x ms_texels_per_thread (rows of at most 1024 texels;
- * longer rows take a gather kernel whatever the shape). */
+ * ignored); an error if a shape is not a candidate, ms_row_order is not 0,
+ * 1 or 2, or if NU x MU_S texels per row exceed ms_texels_per_thread x
+ * ms_block_threads (rows of at most 1024 texels; longer rows take a gather
+ * kernel whatever the shape). */
 atmo_status atmo_set_tiles(const atmo_lut_sizes* sizes, const atmo_tiles* tiles);
 /* The table as text ("sd MU NU nu_per_thread block_threads",
- * "ms R MU MU_S NU texels_per_thread mu_major"), merged back by
+ * "ms R MU MU_S NU texels_per_thread block_threads row_order"), merged back by
  * atmo_tune_load. */
 atmo_status atmo_tune_save(const char* path);
 atmo_status atmo_tune_load(const char* path);

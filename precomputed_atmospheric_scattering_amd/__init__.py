@@ -689,7 +689,8 @@ def synchronize():
 # Tile shapes of the scattering-density and multiple-scattering passes, and
 # their autotuner (atmo_autotune and friends). A shape is
 # {"sd_nu_per_thread", "sd_block_threads", "ms_texels_per_thread",
-# "ms_row_order"} (row order: MS_ROW_ORDERS); sizes are LUT-size dicts
+# "ms_block_threads", "ms_row_order"} (row order: MS_ROW_ORDERS); sizes are
+# LUT-size dicts
 # (DEFAULT_SIZES keys; missing keys take the defaults).
 # ---------------------------------------------------------------------------
 
@@ -706,14 +707,14 @@ def _sizes(sizes):
 
 def tile_candidates():
     """(scattering-density shapes [(nu per thread, block threads)],
-    multiple-scattering shapes [texels per thread])."""
+    multiple-scattering shapes [(texels per thread, block threads)])."""
     n = lib().atmo_sd_tile_candidates(None, None, 0)
     a, b = (C.c_int * n)(), (C.c_int * n)()
     lib().atmo_sd_tile_candidates(a, b, n)
-    m = lib().atmo_ms_tile_candidates(None, 0)
-    c = (C.c_int * m)()
-    lib().atmo_ms_tile_candidates(c, m)
-    return list(zip(a, b)), list(c)
+    m = lib().atmo_ms_tile_candidates(None, None, 0)
+    c, d = (C.c_int * m)(), (C.c_int * m)()
+    lib().atmo_ms_tile_candidates(c, d, m)
+    return list(zip(a, b)), list(zip(c, d))
 
 
 def autotune(sizes_list, retune=False, stream=None):
@@ -731,18 +732,18 @@ def get_tiles(sizes=None):
     check(lib().atmo_get_tiles(C.byref(z), C.byref(t)), "atmo_get_tiles")
     return dict(sd_nu_per_thread=t.sd_nu_per_thread, sd_block_threads=t.sd_block_threads,
                 ms_texels_per_thread=t.ms_texels_per_thread,
-                ms_row_order=MS_ROW_ORDERS[t.ms_row_order], sd_tuned=bool(t.tuned & 1),
+                ms_block_threads=t.ms_block_threads, ms_row_order=MS_ROW_ORDERS[t.ms_row_order], sd_tuned=bool(t.tuned & 1),
                 ms_tuned=bool(t.tuned & 2))
 
 
 def set_tiles(sizes, sd_nu_per_thread, sd_block_threads, ms_texels_per_thread,
-              ms_row_order=None):
+              ms_block_threads, ms_row_order=None):
     """Pins the shapes of the size classes of 'sizes'; ms_row_order (a name
     of MS_ROW_ORDERS) None keeps the class's current row order."""
     if ms_row_order is None:
         ms_row_order = get_tiles(sizes)["ms_row_order"]
     t = _capi.Tiles(sd_nu_per_thread, sd_block_threads, ms_texels_per_thread,
-                    MS_ROW_ORDERS.index(ms_row_order), 0)
+                    ms_block_threads, MS_ROW_ORDERS.index(ms_row_order), 0)
     z = _sizes(sizes)
     check(lib().atmo_set_tiles(C.byref(z), C.byref(t)), "atmo_set_tiles")
 

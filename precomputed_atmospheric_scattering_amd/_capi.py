@@ -114,8 +114,8 @@ class ServerStats(C.Structure):
 
 class Tiles(C.Structure):
     _fields_ = [("sd_nu_per_thread", C.c_int), ("sd_block_threads", C.c_int),
-                ("ms_texels_per_thread", C.c_int), ("ms_row_order", C.c_int),
-                ("tuned", C.c_int)]
+                ("ms_texels_per_thread", C.c_int), ("ms_block_threads", C.c_int),
+                ("ms_row_order", C.c_int), ("tuned", C.c_int)]
 
 
 REQUEST_PRECOMPUTE, REQUEST_LOOKUP = 0, 1
@@ -141,7 +141,7 @@ SIGNATURES = {
     "atmo_batch_num_launches": (C.c_int, [VP]),
     "atmo_batch_destroy": (None, [VP]),
     "atmo_sd_tile_candidates": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
-    "atmo_ms_tile_candidates": (C.c_int, [C.POINTER(C.c_int), C.c_int]),
+    "atmo_ms_tile_candidates": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
     "atmo_autotune": (ST, [C.POINTER(LutSizes), C.c_int, C.c_int, VP]),
     "atmo_get_tiles": (ST, [C.POINTER(LutSizes), C.POINTER(Tiles)]),
     "atmo_set_tiles": (ST, [C.POINTER(LutSizes), C.POINTER(Tiles)]),

@@ -98,7 +98,7 @@ struct Job {
   float sun_k[3];
   int combined;  // COMBINED_SCATTERING_TEXTURES: no separate S_Mie output
   int orders;    // number of scattering orders of this job
-  // Tile shapes (indices into kSdTile and kMsTpt of kernels.h): host-side
+  // Tile shapes (indices into kSdTile and kMsTile of kernels.h): host-side
   // block counts and launch grouping; the kernels are instantiated per shape.
   int sd_tile, ms_tile;
   // Order in which the multiple-scattering blocks walk the packed (r, mu)

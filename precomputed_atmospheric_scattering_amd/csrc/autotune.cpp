@@ -1,6 +1,6 @@
 // Tile-shape autotuner of libatmo (include/atmo.h, atmo_autotune and
 // friends). The scattering-density and multiple-scattering passes each come
-// in several tile shapes (kernels.h: kSdTile, kMsTpt) that compute every
+// in several tile shapes (kernels.h: kSdTile, kMsTile) that compute every
 // texel identically. The best shape depends on the LUT sizes: a (r, mu_s)
 // plane of the density pass holds MU x NU outputs, which a 256-thread block
 // of 4-nu threads only half fills at MU = 64, NU = 8, and a packed row of
@@ -8,8 +8,9 @@
 // shape on a batch of Earth-like atmospheres of the size in question, enough
 // copies to fill the GPU, and records the fastest per size class:
 //   scattering density:   (MU, NU)
-//   multiple scattering:  (R, MU, MU_S, NU), texels per thread and the
-//                         order of the blocks over the (r, mu) rows
+//   multiple scattering:  (R, MU, MU_S, NU), texels per thread, block
+//                         threads and the order of the blocks over the
+//                         (r, mu) rows
 // The choice is read when a model's job table is built (Init, batches, the
 // serving loop); sizes never tuned get the default shapes.
 #include <hip/hip_runtime.h>
@@ -39,7 +40,7 @@ namespace {
 using SdKey = std::array<int, 2>;  // (MU, NU)
 using MsKey = std::array<int, 4>;  // (R, MU, MU_S, NU)
 
-// A multiple-scattering shape: tile (index into kMsTpt) and row order.
+// A multiple-scattering shape: tile (index into kMsTile) and row order.
 struct MsShape {
   int tile, row_order;
 };
@@ -65,7 +66,7 @@ Sizes sizes_of(const atmo_lut_sizes& z) {
 // Whether multiple-scattering tile t can hold the packed rows of z (rows
 // too long for LDS take the gather kernel whatever the tile).
 bool ms_tile_fits(const Sizes& z, int t) {
-  return atmo::ms_rows_per_block(z.nnu * z.nmus, atmo::kMsTpt[t]) >= 0;
+  return atmo::ms_rows_per_block(z.nnu * z.nmus, atmo::ms_tile_capacity(t)) >= 0;
 }
 
 int sd_tile_of_shape(int chunk, int block) {
@@ -74,9 +75,9 @@ int sd_tile_of_shape(int chunk, int block) {
   return -1;
 }
 
-int ms_tile_of_shape(int tpt) {
+int ms_tile_of_shape(int tpt, int block) {
   for (int t = 0; t < atmo::kMsTiles; ++t)
-    if (atmo::kMsTpt[t] == tpt) return t;
+    if (atmo::kMsTile[t].tpt == tpt && atmo::kMsTile[t].block == block) return t;
   return -1;
 }
 
@@ -326,9 +327,11 @@ int atmo_sd_tile_candidates(int* nu_per_thread, int* block_threads, int capacity
   return atmo::kSdTiles;
 }
 
-int atmo_ms_tile_candidates(int* texels_per_thread, int capacity) {
-  for (int t = 0; t < atmo::kMsTiles && t < capacity; ++t)
-    if (texels_per_thread != nullptr) texels_per_thread[t] = atmo::kMsTpt[t];
+int atmo_ms_tile_candidates(int* texels_per_thread, int* block_threads, int capacity) {
+  for (int t = 0; t < atmo::kMsTiles && t < capacity; ++t) {
+    if (texels_per_thread != nullptr) texels_per_thread[t] = atmo::kMsTile[t].tpt;
+    if (block_threads != nullptr) block_threads[t] = atmo::kMsTile[t].block;
+  }
   return atmo::kMsTiles;
 }
 
@@ -340,7 +343,8 @@ atmo_status atmo_get_tiles(const atmo_lut_sizes* sizes, atmo_tiles* out) {
   const Tiles t = tiles_for(z);
   out->sd_nu_per_thread = atmo::kSdTile[t.sd].chunk;
   out->sd_block_threads = atmo::kSdTile[t.sd].block;
-  out->ms_texels_per_thread = atmo::kMsTpt[t.ms];
+  out->ms_texels_per_thread = atmo::kMsTile[t.ms].tpt;
+  out->ms_block_threads = atmo::kMsTile[t.ms].block;
   out->ms_row_order = t.ms_row_order;
   std::lock_guard<std::mutex> lock(g_tiles_mu);
   out->tuned = (g_sd.count(sd_key(z)) ? 1 : 0) | (g_ms.count(ms_key(z)) ? 2 : 0);
@@ -355,7 +359,7 @@ atmo_status atmo_set_tiles(const atmo_lut_sizes* sizes, const atmo_tiles* tiles)
   const int sd = sd_tile_of_shape(tiles->sd_nu_per_thread, tiles->sd_block_threads);
   if (sd < 0)
     return fail(ATMO_ERROR_INVALID_ARGUMENT, "not a scattering-density tile shape");
-  const int ms = ms_tile_of_shape(tiles->ms_texels_per_thread);
+  const int ms = ms_tile_of_shape(tiles->ms_texels_per_thread, tiles->ms_block_threads);
   if (ms < 0)
     return fail(ATMO_ERROR_INVALID_ARGUMENT, "not a multiple-scattering tile shape");
   if (!ms_tile_fits(z, ms))
@@ -408,7 +412,7 @@ atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
 
 // Text file, one size class per line:
 //   sd <MU> <NU> <nu per thread> <block threads>
-//   ms <R> <MU> <MU_S> <NU> <texels per thread> <row_order>
+//   ms <R> <MU> <MU_S> <NU> <texels per thread> <block threads> <row_order>
 atmo_status atmo_tune_save(const char* path) {
   if (path == nullptr) return fail(ATMO_ERROR_INVALID_ARGUMENT, "path is NULL");
   FILE* f = std::fopen(path, "w");
@@ -421,9 +425,9 @@ atmo_status atmo_tune_save(const char* path) {
                               atmo::kSdTile[e.second].chunk,
                               atmo::kSdTile[e.second].block) > 0;
     for (const auto& e : g_ms)
-      ok = ok && std::fprintf(f, "ms %d %d %d %d %d %d\n", e.first[0], e.first[1],
-                              e.first[2], e.first[3], atmo::kMsTpt[e.second.tile],
-                              e.second.row_order) > 0;
+      ok = ok && std::fprintf(f, "ms %d %d %d %d %d %d %d\n", e.first[0], e.first[1],
+                              e.first[2], e.first[3], atmo::kMsTile[e.second.tile].tpt,
+                              atmo::kMsTile[e.second.tile].block, e.second.row_order) > 0;
   }
   ok = (std::fclose(f) == 0) && ok;
   if (!ok) return fail(ATMO_ERROR_INVALID_ARGUMENT, std::string("cannot write ") + path);
@@ -449,22 +453,22 @@ atmo_status atmo_tune_load(const char* path) {
     }
     if (line[0] == '#' || line[0] == '\n') continue;
     std::string kind;
-    long v[6] = {};
-    const int n = parse_tile_line(line, &kind, v, 6);
+    long v[7] = {};
+    const int n = parse_tile_line(line, &kind, v, 7);
     if (n == 4 && kind == "sd") {
       const int t = sd_tile_of_shape(static_cast<int>(v[2]), static_cast<int>(v[3]));
       if (t < 0 || v[0] < 2 || v[1] < 2)
         st = fail(ATMO_ERROR_INVALID_ARGUMENT, where + ": not a scattering-density entry");
       else
         sd[{static_cast<int>(v[0]), static_cast<int>(v[1])}] = t;
-    } else if (n == 6 && kind == "ms") {
-      const int t = ms_tile_of_shape(static_cast<int>(v[4]));
+    } else if (n == 7 && kind == "ms") {
+      const int t = ms_tile_of_shape(static_cast<int>(v[4]), static_cast<int>(v[5]));
       Sizes z{};
       z.nr = static_cast<int>(v[0]);
       z.nmu = static_cast<int>(v[1]);
       z.nmus = static_cast<int>(v[2]);
       z.nnu = static_cast<int>(v[3]);
-      const long h = v[5];
+      const long h = v[6];
       if (t < 0 || z.nr < 2 || z.nmu < 2 || z.nmus < 2 || z.nnu < 2 || h < 0 ||
           h >= atmo::kMsRowOrders || !ms_tile_fits(z, t))
         st = fail(ATMO_ERROR_INVALID_ARGUMENT, where + ": not a multiple-scattering entry");

@@ -60,23 +60,33 @@ ATMO_HD int sd_default_tile(int nmu, int nnu) {
 }
 
 // Multiple scattering: packed rows of at most kMsMaxRow texels are processed
-// through LDS by blocks of kBlock threads, each thread holding kMsTpt[tile]
-// texels: ms_rows_per_block(W, tpt) whole rows per block (0: the row is too
-// long for LDS and takes the gather kernel; -1: it does not fit this tile).
+// through LDS by blocks of kMsTile[tile].block threads, each thread holding
+// kMsTile[tile].tpt texels: ms_rows_per_block(W, capacity) whole rows per
+// block, capacity = tpt x block texels (0: the row is too long for LDS and
+// takes the gather kernel; -1: it does not fit this tile). Rows of up to
+// 1024 texels take 2 texels per thread in 512-thread blocks rather than 4 in
+// 256: the blocks hold the same rows and the kernel runs at more waves per
+// SIMD (the high-resolution pass, rows of 1024 texels: 83.0 against 88.8 ms
+// per Init with FP32 storage, 81.5 against 83.0 with FP16; 1 texel in
+// 1024-thread blocks took 103-105 ms).
+struct MsTile {
+  int tpt, block;
+};
 constexpr int kMsTiles = 3;
-constexpr int kMsTpt[kMsTiles] = {1, 2, 4};
+constexpr MsTile kMsTile[kMsTiles] = {{1, 256}, {2, 256}, {2, 512}};
 constexpr int kMsMaxRow = 4 * kBlock;
-// Rows per block are capped at 4 (one texel per thread) or 8, which keeps
-// the block's LDS (2 x kBlock x tpt RGB floats + 3.2 KB of samples per row)
+constexpr int ms_tile_capacity(int t) { return kMsTile[t].tpt * kMsTile[t].block; }
+// Rows per block are capped at 4 (blocks of 256 texels) or 8, which keeps
+// the block's LDS (2 x capacity RGB floats + 3.2 KB of samples per row)
 // under 64 KB.
-ATMO_HD int ms_rows_per_block(int w, int tpt) {
+ATMO_HD int ms_rows_per_block(int w, int capacity) {
   if (w > kMsMaxRow) return 0;
-  if (w > kBlock * tpt) return -1;
-  const int r = kBlock * tpt / w;
-  const int cap = tpt == 1 ? 4 : 8;
+  if (w > capacity) return -1;
+  const int r = capacity / w;
+  const int cap = capacity <= kBlock ? 4 : 8;
   return r < cap ? r : cap;
 }
-// The smallest texels-per-thread tile that holds a row of w texels.
+// The smallest tile that holds a row of w texels.
 ATMO_HD int ms_default_tile(int w) {
   if (w <= kBlock) return 0;
   if (w <= 2 * kBlock) return 1;

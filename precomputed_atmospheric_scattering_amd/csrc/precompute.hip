@@ -906,9 +906,9 @@ __global__ __launch_bounds__(64) void k_indirect_irradiance(
 // one row in LDS with coalesced loads (every texel of a row reads the same
 // four rows), and each texel then reads its 4 (mu_s, nu) taps from LDS:
 // 4 coalesced global loads per texel and sample instead of 16 gathers.
-// Each thread holds kTpt texels; a block takes ms_rows_per_block(W, kTpt)
-// whole rows (several short rows, or one row of up to kBlock * kTpt
-// texels). Rows longer than kMsMaxRow take the per-texel 16-tap gather of
+// Each of the kB threads holds kTpt texels; a block takes
+// ms_rows_per_block(W, kTpt * kB) whole rows (several short rows, or one row
+// of up to kTpt * kB texels). Rows longer than kMsMaxRow take the per-texel 16-tap gather of
 // k_multiple_scattering_gather. Both kernels are launched over the same
 // blocks and each returns at once on the other's jobs, so the path of a job
 // depends on its own row length only.
@@ -978,7 +978,7 @@ struct MsRowSample {
 // (FP16-storage models); otherwise the FP32 ones. Both are launched over the
 // same blocks when a launch holds both kinds; each returns at once on the
 // other's jobs.
-template <int kTpt, bool kF16, int kB = kBlock>
+template <int kTpt, bool kF16, int kB>
 __global__ __launch_bounds__(kB) void k_multiple_scattering(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
     int njobs, int max_rows) {
@@ -996,7 +996,7 @@ __global__ __launch_bounds__(kB) void k_multiple_scattering(
   const int lb = blockIdx.x - first_block[ji];
   const int nnu = J.nnu, nmus = J.nmus, nmu = J.nmu;
   const int W = nnu * nmus;
-  const int rpb = ms_rows_per_block(W, kTpt * kB / kBlock);
+  const int rpb = ms_rows_per_block(W, kTpt * kB);
   // Gather-path job (uniform per block). The launcher groups jobs by tile
   // and sizes max_rows over them; the tests only keep LDS indexing in
   // bounds should it not.
@@ -1527,7 +1527,7 @@ int blocks_scattering_density(const Job& j) {
 int blocks_indirect_irradiance(const Job& j) { return j.ew * j.eh; }
 int blocks_multiple_scattering(const Job& j) {
   const int w = j.nnu * j.nmus;
-  const int rpb = ms_rows_per_block(w, kMsTpt[j.ms_tile]);
+  const int rpb = ms_rows_per_block(w, ms_tile_capacity(j.ms_tile));
   const int rows = j.nmu * j.nr;
   if (rpb == 0) return ((w + kBlock - 1) / kBlock) * rows;
   if (rpb < 0) return 0;  // not a tile of this row length (rejected by the host)
@@ -1625,11 +1625,11 @@ hipError_t launch_to_half(const PassLaunch& p, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int kTpt, int kB = kBlock>
+template <int kTpt, int kB>
 void launch_ms_tile(const PassLaunch& p, hipStream_t s) {
   // Rows per block shrink as rows grow: the shortest row sets the launch's
   // largest count.
-  const int rows = ms_rows_per_block(p.min_w, kTpt * kB / kBlock);
+  const int rows = ms_rows_per_block(p.min_w, kTpt * kB);
   const size_t lds = sizeof(float3) * 2 * kB * kTpt +
                      sizeof(MsRowSample) * kMarch * static_cast<size_t>(rows);
   if (p.dj_kinds & kDjF32)
@@ -1644,12 +1644,12 @@ hipError_t launch_multiple_scattering(const PassLaunch& p, hipStream_t s) {
   if (p.tile < 0 || p.tile >= kMsTiles) return hipErrorInvalidValue;
   if (p.min_w <= kMsMaxRow) {
     // Every LDS-path row of the launch must fit the tile.
-    if (p.max_lds_w > kBlock * kMsTpt[p.tile]) return hipErrorInvalidValue;
+    if (p.max_lds_w > ms_tile_capacity(p.tile)) return hipErrorInvalidValue;
     static_assert(kMsTiles == 3, "one case per tile shape");
     switch (p.tile) {
-      case 0: launch_ms_tile<kMsTpt[0]>(p, s); break;
-      case 1: launch_ms_tile<kMsTpt[1]>(p, s); break;
-      default: launch_ms_tile<kMsTpt[2]>(p, s); break;
+      case 0: launch_ms_tile<kMsTile[0].tpt, kMsTile[0].block>(p, s); break;
+      case 1: launch_ms_tile<kMsTile[1].tpt, kMsTile[1].block>(p, s); break;
+      default: launch_ms_tile<kMsTile[2].tpt, kMsTile[2].block>(p, s); break;
     }
   }
   if (p.max_w > kMsMaxRow)

@@ -208,16 +208,19 @@ int main(int argc, char** argv) {
     bad = t;
     bad.sd_block_threads = 96;
     expect(atmo_set_tiles(&z, &bad) == ATMO_ERROR_INVALID_ARGUMENT, "bad block");
+    bad = t;
+    bad.ms_block_threads = 1024;
+    expect(atmo_set_tiles(&z, &bad) == ATMO_ERROR_INVALID_ARGUMENT, "bad ms block");
     t.ms_row_order = 2;
     expect(atmo_set_tiles(&z, &t) == ATMO_OK, "set tiles");
     const std::string good = tmp + "/tiles_good.txt";
     expect(atmo_tune_save(good.c_str()) == ATMO_OK, "save tiles");
     expect(atmo_tune_load(good.c_str()) == ATMO_OK, "load tiles");
     const char* bad_files[] = {
-        "sd 64 4 3 256\n", "ms 32 128 8 4 1 9\n", "ms 128 8 4 1\n", "xx\n",
+        "sd 64 4 3 256\n", "ms 32 128 8 4 1 256 9\n", "ms 128 8 4 1 256\n", "xx\n",
         "sd -5 4 4 256\n", "sd 99999999999999999999 4 4 256\n",
-        "ms 32 128 32 8 1 0 trailing words and more words and more\n",
-        "sd 64 4 4 256 ms 32 128 32 8 1 0\n"};
+        "ms 32 128 32 8 1 256 0 trailing words and more words and more\n",
+        "sd 64 4 4 256 ms 32 128 32 8 1 256 0\n", "ms 32 128 32 8 4 256 0\n"};
     int k = 0;
     for (const char* text : bad_files) {
       const std::string path = tmp + "/tiles_bad" + std::to_string(k++) + ".txt";

@@ -132,7 +132,7 @@ def test_long_rows_multiple_scattering_every_row_order():
     try:
         for mm in atmo.MS_ROW_ORDERS:
             print("case long rows, %s" % mm, flush=True)
-            atmo.set_tiles(LONG_ROW_SIZES, 4, 256, 1, ms_row_order=mm)
+            atmo.set_tiles(LONG_ROW_SIZES, 4, 256, 1, 256, ms_row_order=mm)
             m = parity.gpu_model(cfg, retain_intermediates=True)
             m.init(2)
             T = np.ascontiguousarray(m.lut("transmittance"), dtype=np.float64)

@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 NAMES = ("transmittance", "scattering", "single_mie_scattering", "irradiance")
 
 # Size classes with half-empty default blocks (MU x NU = 64 x 4), the
-# reference sizes, and rows of 512 texels (two texels per thread by default).
+# reference sizes, and rows of 512 texels (two texels per thread in
+# 256-thread blocks by default).
 SIZES = {
     "small": dict(transmittance_width=64, transmittance_height=16, scattering_r=8,
                   scattering_mu=64, scattering_mu_s=16, scattering_nu=4,
@@ -47,11 +48,12 @@ def shapes_for(sizes):
     sd, ms = atmo.tile_candidates()
     d = atmo.get_tiles(sizes)
     dsd = (d["sd_nu_per_thread"], d["sd_block_threads"])
+    dms = (d["ms_texels_per_thread"], d["ms_block_threads"])
     dmm = d["ms_row_order"]
-    out = [(s, d["ms_texels_per_thread"], dmm) for s in sd if s != dsd]
+    out = [(s, dms, dmm) for s in sd if s != dsd]
     w = sizes["scattering_nu"] * sizes["scattering_mu_s"]
     out += [(dsd, t, mm) for t in ms for mm in atmo.MS_ROW_ORDERS
-            if (t, mm) != (d["ms_texels_per_thread"], dmm) and w <= 256 * t]
+            if (t, mm) != (dms, dmm) and w <= t[0] * t[1]]
     return out
 
 
@@ -60,12 +62,12 @@ def test_every_shape_gives_identical_luts(name):
     sizes = SIZES[name]
     cfg = configs.earth(orders=4, lut_sizes=sizes)
     ref = run(cfg)
-    for (chunk, block), tpt, mm in shapes_for(sizes):
-        print("case", name, chunk, block, tpt, mm, flush=True)
-        atmo.set_tiles(sizes, chunk, block, tpt, ms_row_order=mm)
+    for (chunk, block), (tpt, mblock), mm in shapes_for(sizes):
+        print("case", name, chunk, block, tpt, mblock, mm, flush=True)
+        atmo.set_tiles(sizes, chunk, block, tpt, mblock, ms_row_order=mm)
         got = run(cfg)
         for n in NAMES:
-            assert np.array_equal(got[n], ref[n]), (name, chunk, block, tpt, mm, n)
+            assert np.array_equal(got[n], ref[n]), (name, chunk, block, tpt, mblock, mm, n)
         atmo.reset_tiles()
 
 
@@ -74,10 +76,11 @@ def test_every_shape_identical_illuminance_mode():
     sizes = SIZES["small"]
     cfg = configs.spectral(orders=3, lut_sizes=sizes)
     ref = run(cfg)
-    for (chunk, block), tpt, mm in [((1, 256), 1, "mu-major"), ((4, 64), 2, "r-major"),
-                                     ((2, 128), 4, "ray")]:
-        print("case spectral", chunk, block, tpt, mm, flush=True)
-        atmo.set_tiles(sizes, chunk, block, tpt, ms_row_order=mm)
+    for (chunk, block), (tpt, mblock), mm in [((1, 256), (1, 256), "mu-major"),
+                                               ((4, 64), (2, 256), "r-major"),
+                                               ((2, 128), (2, 512), "ray")]:
+        print("case spectral", chunk, block, tpt, mblock, mm, flush=True)
+        atmo.set_tiles(sizes, chunk, block, tpt, mblock, ms_row_order=mm)
         got = run(cfg)
         for n in NAMES:
             assert np.array_equal(got[n], ref[n]), (chunk, block, tpt, n)
@@ -95,14 +98,12 @@ def test_autotune_then_mixed_shape_batch_equals_alone(use_graph):
         assert t["sd_tuned"] and t["ms_tuned"]
         assert (t["sd_nu_per_thread"], t["sd_block_threads"]) in sd
         w = z["scattering_nu"] * z["scattering_mu_s"]
-        assert t["ms_texels_per_thread"] in ms
-        assert w > 1024 or w <= 256 * t["ms_texels_per_thread"]
+        assert (t["ms_texels_per_thread"], t["ms_block_threads"]) in ms
+        assert w > 1024 or w <= t["ms_texels_per_thread"] * t["ms_block_threads"]
     # Force two shapes on the first two size classes so that the batch has
     # jobs of different shapes in one pass level whatever the tuner chose.
-    atmo.set_tiles(sizes[0], 1, 256, 4 if sizes[0]["scattering_nu"] *
-                   sizes[0]["scattering_mu_s"] <= 1024 else 1, ms_row_order="ray")
-    atmo.set_tiles(sizes[1], 4, 64, 4 if sizes[1]["scattering_nu"] *
-                   sizes[1]["scattering_mu_s"] <= 1024 else 1, ms_row_order="r-major")
+    atmo.set_tiles(sizes[0], 1, 256, 2, 512, ms_row_order="ray")
+    atmo.set_tiles(sizes[1], 4, 64, 2, 512, ms_row_order="r-major")
     models = [atmo.Model(**configs.model_kwargs(c)[0]) for c in cfgs]
     b = atmo.Batch(models, [c["orders"] for c in cfgs], use_graph=use_graph)
     # The launches of one level that differ in tile shape run concurrently

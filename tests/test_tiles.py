@@ -21,7 +21,7 @@ def test_candidates():
     sd, ms = atmo.tile_candidates()
     assert sd[0] == (4, 256) and (4, 128) in sd
     assert len(set(sd)) == len(sd) and all(b % 64 == 0 for _, b in sd)
-    assert ms == [1, 2, 4]
+    assert ms == [(1, 256), (2, 256), (2, 512)]
 
 
 def test_defaults_follow_row_length():
@@ -31,10 +31,13 @@ def test_defaults_follow_row_length():
     assert (t["sd_nu_per_thread"], t["sd_block_threads"]) == (4, 128)
     big = atmo.get_tiles(dict(scattering_mu=256, scattering_nu=16))
     assert (big["sd_nu_per_thread"], big["sd_block_threads"]) == (4, 256)
-    assert t["ms_texels_per_thread"] == 1  # 8 x 32 = 256 texels per row
+    def ms(z=None):
+        g = atmo.get_tiles(z)
+        return g["ms_texels_per_thread"], g["ms_block_threads"]
+    assert ms() == (1, 256)  # 8 x 32 = 256 texels per row
     assert not t["sd_tuned"] and not t["ms_tuned"]
-    assert atmo.get_tiles(dict(scattering_mu_s=64))["ms_texels_per_thread"] == 2
-    assert atmo.get_tiles(WIDE)["ms_texels_per_thread"] == 4
+    assert ms(dict(scattering_mu_s=64)) == (2, 256)
+    assert ms(WIDE) == (2, 512)
 
 
 def test_default_row_order_follows_lut_bytes():
@@ -50,10 +53,10 @@ def test_default_row_order_follows_lut_bytes():
 
 
 def test_set_get_by_size_class():
-    atmo.set_tiles(SMALL, 1, 256, 2)
+    atmo.set_tiles(SMALL, 1, 256, 2, 256)
     t = atmo.get_tiles(SMALL)
-    assert (t["sd_nu_per_thread"], t["sd_block_threads"], t["ms_texels_per_thread"]) == \
-        (1, 256, 2)
+    assert (t["sd_nu_per_thread"], t["sd_block_threads"], t["ms_texels_per_thread"],
+            t["ms_block_threads"]) == (1, 256, 2, 256)
     assert t["sd_tuned"] and t["ms_tuned"]
     # The density class is (MU, NU) and the multiple-scattering class
     # (R, MU, MU_S, NU): other R (or other MU_S) is another class.
@@ -63,16 +66,18 @@ def test_set_get_by_size_class():
     assert t2["sd_tuned"] and not t2["ms_tuned"] and t2["ms_texels_per_thread"] == 1
     assert atmo.get_tiles()["sd_nu_per_thread"] == 4  # default class untouched
     # The row order is pinned with the rest of the shape.
-    atmo.set_tiles(SMALL, 4, 256, 1, ms_row_order="mu-major")
+    atmo.set_tiles(SMALL, 4, 256, 1, 256, ms_row_order="mu-major")
     assert atmo.get_tiles(SMALL)["ms_row_order"] == "mu-major"
-    atmo.set_tiles(SMALL, 4, 256, 2)  # keeps the class's order
+    atmo.set_tiles(SMALL, 4, 256, 2, 512)  # keeps the class's order
     assert atmo.get_tiles(SMALL)["ms_row_order"] == "mu-major"
-    atmo.set_tiles(SMALL, 4, 256, 2, ms_row_order="ray")
+    atmo.set_tiles(SMALL, 4, 256, 2, 256, ms_row_order="ray")
     assert atmo.get_tiles(SMALL)["ms_row_order"] == "ray"
     assert atmo.get_tiles(other)["ms_row_order"] == "r-major"
 
 
-@pytest.mark.parametrize("shape", [(3, 256, 1), (4, 96, 1), (4, 256, 3), (4, 512, 1)])
+@pytest.mark.parametrize("shape", [(3, 256, 1, 256), (4, 96, 1, 256), (4, 256, 3, 256),
+                                   (4, 512, 1, 256), (4, 256, 4, 256), (4, 256, 1, 512),
+                                   (4, 256, 2, 128)])
 def test_unknown_shapes_rejected(shape):
     with pytest.raises(AtmoError):
         atmo.set_tiles(None, *shape)
@@ -80,15 +85,15 @@ def test_unknown_shapes_rejected(shape):
 
 def test_multiple_scattering_tile_must_hold_the_row():
     with pytest.raises(AtmoError):
-        atmo.set_tiles(WIDE, 4, 256, 2)  # 1024 texels > 2 x 256
-    atmo.set_tiles(WIDE, 4, 256, 4)
+        atmo.set_tiles(WIDE, 4, 256, 2, 256)  # 1024 texels > 2 x 256
+    atmo.set_tiles(WIDE, 4, 256, 2, 512)
     # Gather-path rows take any shape.
-    atmo.set_tiles(LONG, 4, 256, 1)
+    atmo.set_tiles(LONG, 4, 256, 1, 256)
 
 
 def test_save_load_round_trip(tmp_path):
-    atmo.set_tiles(SMALL, 2, 128, 4)
-    atmo.set_tiles(WIDE, 4, 64, 4, ms_row_order="ray")
+    atmo.set_tiles(SMALL, 2, 128, 2, 512)
+    atmo.set_tiles(WIDE, 4, 64, 2, 512, ms_row_order="ray")
     path = tmp_path / "tiles.txt"
     atmo.save_tiles(path)
     before = {k: atmo.get_tiles(z) for k, z in (("s", SMALL), ("w", WIDE), ("d", None))}
@@ -99,11 +104,12 @@ def test_save_load_round_trip(tmp_path):
     assert after == before
     lines = [l for l in path.read_text().splitlines() if not l.startswith("#")]
     assert sorted(lines) == sorted(["sd 64 4 2 128", "sd 128 16 4 64",
-                                    "ms 16 64 16 4 4 0", "ms 32 128 64 16 4 2"])
+                                    "ms 16 64 16 4 2 512 0", "ms 32 128 64 16 2 512 2"])
 
 
-@pytest.mark.parametrize("text", ["sd 64 4 3 256\n", "ms 32 128 64 16 1 0\n",
-                                  "ms 32 128 8 4 1 3\n", "ms 128 8 4 1\n",
+@pytest.mark.parametrize("text", ["sd 64 4 3 256\n", "ms 32 128 64 16 1 256 0\n",
+                                  "ms 32 128 8 4 1 256 3\n", "ms 128 8 4 1 256\n",
+                                  "ms 32 128 8 4 4 256 0\n", "ms 32 128 8 4 1 0\n",
                                   "xx 1 2 3 4\n", "sd 64 4\n"])
 def test_load_rejects_bad_entries_atomically(tmp_path, text):
     path = tmp_path / "bad.txt"
