@@ -58,8 +58,19 @@ LANE_OPS = dict(transmittance=40, single_scattering=178, scattering_density_2=11
                 multiple_scattering=260)
 SAMPLES = dict(transmittance=501, single_scattering=51, scattering_density=512,
                indirect_irradiance=1024, multiple_scattering=51)
+# The survey's compulsory traffic of the three lookups: inputs and outputs
+# only (LUT texels taken as cache hits).
 QUERY_BYTES_PER_RAY = 124
 QUERY_LANE_OPS_PER_RAY = 1500
+# What the three lookup kernels read and write per ray: GetSkyLuminance
+# (camera, view, sun, shadow length; 2 RGB outputs) 40 + 24 B,
+# GetSkyLuminanceToPoint (camera, point, sun, shadow length) 40 + 24 B,
+# GetSunAndSkyIlluminance (point, normal, sun) 36 + 24 B.
+QUERY_IO_BYTES_PER_RAY = 188
+# Scattering lookups per ray over the three (GetSkyRadiance 1, ToPoint 2),
+# each 16 texels (2 r x 2 mu x 2 mu_s x 2 nu) of S and of S_Mie.
+QUERY_SCATTERING_LOOKUPS = 3
+QUERY_TEXELS_PER_LOOKUP = 16
 WIDTH, HEIGHT = 1920, 1080
 PROFILES = os.path.join(ROOT, "profiles")
 COUNTERS_HIGH_RES = os.path.join(PROFILES, "counters_high_resolution.md")
@@ -295,6 +306,8 @@ class Lookups:
 
     def __init__(self, atmo, configs, kw, seed=2024):
         self.n = WIDTH * HEIGHT
+        self.half = bool(kw.get("half_precision", False))
+        self.luts = 1 if kw.get("combine_scattering_textures", False) else 2
         self.q = configs.sky_queries(seed, self.n, kw["bottom_radius"] / 1000.0,
                                      kw["top_radius"] / 1000.0)
         self.dev = {k: atmo.DeviceArray.from_host(v) for k, v in self.q.items()}
@@ -319,25 +332,90 @@ class Lookups:
         for k in range(3):
             self.ms[k].append(self.ev[k].elapsed_ms(self.ev[k + 1]))
 
+    def roofline(self, batch_s):
+        """Roofline of one batch of the three lookups. On random rays no two
+        rays share a texel, so every texel a lookup reads (the 16 taps of S
+        and of S_Mie, in the LUTs' storage format, 8 B per texel FP16, 16 B
+        FP32) is read from beyond the caches when the LUTs exceed them (the
+        high-resolution LUTs do: 268 MB FP16, 537 MB FP32, against 256 MB of
+        Infinity Cache); those bytes plus the rays' inputs and outputs are
+        the algorithmic bytes. Transmittance and irradiance LUTs (1 MB, 64
+        KB) stay cached. The lookup copy reads those texels as 128-byte
+        lines, 4 per scattering lookup FP32 and 2 FP16; random gathers cost
+        per line, so their rate is set against the measured line rate of a
+        random 128-byte gather over a buffer of the copy's size."""
+        texel = 8 if self.half else 16
+        texel_bytes = QUERY_SCATTERING_LOOKUPS * QUERY_TEXELS_PER_LOOKUP * self.luts * texel
+        per_ray = QUERY_IO_BYTES_PER_RAY + texel_bytes
+        achieved = per_ray * self.n / batch_s / 1e9
+        ideal_io = max(QUERY_BYTES_PER_RAY * self.n / (PEAK_HBM_GBS * 1e9),
+                       QUERY_LANE_OPS_PER_RAY * self.n / (PEAK_VALU_TOPS * 1e12))
+        out = {"bound": "hbm", "bytes_per_ray": per_ray,
+               "io_bytes_per_ray": QUERY_IO_BYTES_PER_RAY, "texel_bytes_per_ray": texel_bytes,
+               "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+               "frac": round(achieved / PEAK_HBM_GBS, 4),
+               "io_only_frac": round(ideal_io / batch_s, 4)}
+        if self.luts == 2:
+            lines = QUERY_SCATTERING_LOOKUPS * (2 if self.half else 4)
+            out["lines_per_ray"] = lines
+            out["lines_per_s"] = round(lines * self.n / batch_s, 1)
+            rate = MEASURED_PEAKS.get("gather_lines_per_s_2_25gib")
+            if rate:
+                out["frac_of_measured_gather_line_rate"] = round(
+                    lines * self.n / batch_s / rate, 4)
+        return out
+
     def summary(self):
         steps = len(self.ms[0])
         batch_s = sum(sum(l) for l in self.ms) * 1e-3 / steps
-        ideal = max(QUERY_BYTES_PER_RAY * self.n / (PEAK_HBM_GBS * 1e9),
-                    QUERY_LANE_OPS_PER_RAY * self.n / (PEAK_VALU_TOPS * 1e12))
+        rl = self.roofline(batch_s)
         return {
             "sky_rays_per_s": round(3.0 * self.n / batch_s, 1),
             "lookup_ms": {k: round(sum(l) / steps, 4) for k, l in zip(LOOKUP_NAMES, self.ms)},
             "lookup_p99_ms": {k: round(percentile(l, 0.99), 4)
                               for k, l in zip(LOOKUP_NAMES, self.ms)},
-            # 124 B/ray of compulsory HBM traffic (the inputs and the six
-            # output vectors of the three lookups) and ~1500 VALU lane-ops per
-            # ray for the three, against the time of the three lookups.
-            "lookup_roofline_frac": round(ideal / batch_s, 4),
+            "lookup_roofline": rl,
+            "lookup_roofline_frac": rl["frac"],
         }
 
     def free(self):
         for d in list(self.dev.values()) + self.outs:
             d.free()
+
+
+# On-device rate ceilings (libatmo_peaks.so, csrc/peaks.hip), measured once
+# per run by measure_peaks().
+MEASURED_PEAKS = {}
+PEAK_KEYS = ("fp32_fma_lane_instr_per_s", "valu_dual_issue_lane_instr_per_s",
+             "fp64_fma_lane_instr_per_s", "hbm_read_bytes_per_s", "hbm_copy_bytes_per_s",
+             "gather_lines_per_s_1gib", "gather_lines_per_s_2_25gib")
+
+
+def measure_peaks():
+    """Microbenchmarks of the rate ceilings on this GPU: FP32 FMA issue
+    (plain, non-packed instructions), the issue rate of an FMA/add/fract mix
+    that the SIMDs issue two at a time, FP64 FMA issue, HBM streaming read
+    and copy, and random 128-byte line gathers over 1 GiB and 2.25 GiB."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "precomputed_atmospheric_scattering_amd",
+                              "libatmo_peaks.so"))
+    lib.atmo_peaks_measure.restype = C.c_int
+    lib.atmo_peaks_measure.argtypes = [C.POINTER(C.c_double), C.c_int]
+    out = (C.c_double * len(PEAK_KEYS))()
+    rc = lib.atmo_peaks_measure(out, len(PEAK_KEYS))
+    if rc != 0:
+        raise RuntimeError("atmo_peaks_measure failed: HIP error %d" % rc)
+    MEASURED_PEAKS.update({k: float(v) for k, v in zip(PEAK_KEYS, out)})
+    return {
+        "fp32_fma_tops": round(out[0] / 1e12, 2),
+        "valu_dual_issue_tops": round(out[1] / 1e12, 2),
+        "fp64_fma_tops": round(out[2] / 1e12, 2),
+        "hbm_read_gb_per_s": round(out[3] / 1e9, 1),
+        "hbm_copy_gb_per_s": round(out[4] / 1e9, 1),
+        "gather_lines_per_s_1gib": round(out[5], 1),
+        "gather_lines_per_s_2_25gib": round(out[6], 1),
+        "unit_note": "TOP/s are VALU lane-instructions per second (an FMA counts 1)",
+    }
 
 
 def pass_key(name, order):
@@ -491,7 +569,14 @@ def roofline(passes):
     name = max(have, key=lambda k: have[k]["ms_per_atmosphere"])
     p = have[name]
     valu = p["valu_frac"] >= p["hbm_frac"]
-    return {
+    measured = MEASURED_PEAKS.get("valu_dual_issue_lane_instr_per_s" if valu
+                                  else "hbm_read_bytes_per_s")
+    extra = {}
+    if measured:
+        m = measured / 1e12 if valu else measured / 1e9
+        a = p["valu_tops"] if valu else p["hbm_gb_per_s"]
+        extra = {"peak_measured": round(m, 2), "frac_of_measured_peak": round(a / m, 4)}
+    return dict({
         "kernel": PASS_KERNELS[name],
         "bound": "valu" if valu else "hbm",
         "achieved": p["valu_tops"] if valu else p["hbm_gb_per_s"],
@@ -504,7 +589,7 @@ def roofline(passes):
         "valu_frac": p["valu_frac"],
         "hbm_frac": p["hbm_frac"],
         "counters": os.path.relpath(COUNTERS_HIGH_RES, ROOT),
-    }
+    }, **extra)
 
 
 def section_spectral(args, steps, warmup):
@@ -858,6 +943,8 @@ def section_serving(args, rate, duration, seed):
 
 
 def run_headline(args):
+    _progress("on-device rate ceilings")
+    peaks = measure_peaks()
     res, launches, meta = section_high_resolution(
         args, args.steps, args.warmup, keep_luts=not args.no_cpu_baseline)
     passes = pass_figures(launches, meta["steps"], meta["n_s"], meta["iters"])
@@ -889,6 +976,7 @@ def run_headline(args):
     result.update(res)
     result["passes"] = passes
     result["roofline"] = roofline(passes)
+    result["measured_peaks"] = peaks
     _progress("spectral section")
     result["spectral"] = section_spectral(args, max(3, args.steps // 2), max(1, args.warmup))
     _progress("batched section")

@@ -139,9 +139,11 @@ typedef enum atmo_option {
   /* Separate scattering textures only (default on; no counterpart in the
    * reference): also keep a lookup copy of S and S_Mie in which, for every
    * (r, mu) row and (nu, mu_s) anchor, the S and S_Mie texels of the 2 x 2
-   * (nu, mu_s) taps a lookup reads there are contiguous (64 B FP16, 128 B
-   * FP32), so that a 16-tap scattering lookup reads 4 contiguous blocks.
-   * Costs about 4 times the size of S and S_Mie (x (NU+1)(MU_S+1)/(NU MU_S));
+   * (nu, mu_s) taps a lookup reads there are contiguous (128 B FP32), and
+   * with FP16 storage those of both mu taps too (128 B), so that a 16-tap
+   * scattering lookup reads 4 (FP32) or 2 (FP16) 128-byte blocks. Costs
+   * about 4 (FP32) or 8 (FP16) times the size of S and S_Mie
+   * (x (NU+1)(MU_S+1)/(NU MU_S), and (MU+1)/MU with FP16);
    * lookups give identical results with and without it; the LUT views and
    * files are unchanged. Turning it on for an initialized model builds the
    * copy at once (synchronous). */

@@ -149,8 +149,9 @@ struct Job {
   int half_luts;
   uint2* S16;
   uint2* SM16;
-  // The lookup copy of S and SM (s_quad_taps): quads of 4 (S, SM) texel
-  // pairs, 8 float4 (FP32) or 4 uint4 (FP16) per quad; null if absent.
+  // The lookup copy of S and SM (s_quad_taps): 128-byte blocks of quads of
+  // 4 (S, SM) texel pairs, 8 float4 (FP32) or 4 uint4 (FP16) per quad; null
+  // if absent.
   void* Q;
   float* scales;    // 8 floats, written by the conversion kernel
   unsigned* maxbits;  // 8 words: per-channel max |value| (float bits)
@@ -496,19 +497,30 @@ ATMO_HD D4 s_lookup_d(const Job& j, const Texels& lut, const SCoords& c) {
   return acc;
 }
 
-// The lookup copy of S and SM (Job::Q) as quads. For each (r, mu) row and
-// each anchor (s, i) in [-1, NU - 1] x [-1, MU_S - 1], a quad holds the four
-// texels that a bilinear (nu slice, mu_s) tap set anchored there reads, in
-// the tap order of s_taps: (s, i), (s + 1, i), (s, i + 1), (s + 1, i + 1),
-// each index clamped to the LUT as the taps clamp it, each texel an S and SM
-// pair. A scattering lookup (16 taps) then reads 4 contiguous quads (one per
-// (r, mu) tap: 128 B FP32, 64 B FP16) instead of 16 scattered texels, with
-// the same texels, weights and sums.
-ATMO_HD int64_t quad_count(int nr, int nmu, int nmus, int nnu) {
-  return static_cast<int64_t>(nr) * nmu * (nnu + 1) * (nmus + 1);
+// The lookup copy of S and SM (Job::Q), in 128-byte blocks. For each (r,
+// mu) row and each anchor (s, i) in [-1, NU - 1] x [-1, MU_S - 1], a quad
+// holds the four texels that a bilinear (nu slice, mu_s) tap set anchored
+// there reads, in the tap order of s_taps: (s, i), (s + 1, i), (s, i + 1),
+// (s + 1, i + 1), each index clamped to the LUT as the taps clamp it, each
+// texel an S and SM pair.
+//   FP32 models: a block is one quad (8 float4), for each mu texel y.
+//   FP16 models: a block is the quads of rows clamp(ya) and clamp(ya + 1)
+//   (8 x 4 FP16 x 4), for each mu anchor ya in [-1, MU - 1]: the two mu
+//   taps of a lookup.
+// A scattering lookup (16 taps) then reads 4 (FP32) or 2 (FP16) blocks
+// instead of 32 scattered texels, with the same texels, weights and sums.
+// Random gathers cost per cache line, whatever part of it is used, so a
+// block is one 128-byte line in either format.
+ATMO_HD int lookup_copy_rows(int nmu, bool half) { return half ? nmu + 1 : nmu; }
+ATMO_HD int64_t lookup_copy_blocks(int nr, int nmu, int nmus, int nnu, bool half) {
+  return static_cast<int64_t>(nr) * lookup_copy_rows(nmu, half) * (nnu + 1) * (nmus + 1);
 }
-ATMO_HD int64_t quad_index(const Job& j, int z, int y, int s, int i) {
-  return ((static_cast<int64_t>(z) * j.nmu + y) * (j.nnu + 1) + (s + 1)) * (j.nmus + 1) +
+// Block of layer z, row ry (y, or ya + 1 for FP16 models) and anchor (s, i).
+ATMO_HD int64_t lookup_copy_block(const Job& j, int z, int ry, int s, int i) {
+  return ((static_cast<int64_t>(z) * lookup_copy_rows(j.nmu, j.half_luts != 0) + ry) *
+              (j.nnu + 1) +
+          (s + 1)) *
+             (j.nmus + 1) +
          (i + 1);
 }
 // Anchor and fraction along one axis: the floor of the texel-space
@@ -530,8 +542,10 @@ ATMO_HD QuadAxis quad_axis(double x, int n) {
   }
   return q;
 }
-// The 4 quads of a lookup: f(quad index, weights of its 4 texels), the
-// weights the products of s_taps in the same order.
+// The 4 quads of a lookup: f(quad slot, weights of its 4 texels), the
+// weights the products of s_taps in the same order. The slot of a quad is
+// its index in units of quads: the block for FP32 models, 2 x block + (mu
+// tap) for FP16 ones.
 template <class Real, class F>
 ATMO_HD void s_quad_taps(const Job& j, const SCoords& c, F&& f) {
   const QuadAxis qs = quad_axis(c.nu, j.nnu), qx = quad_axis(c.x_in, j.nmus);
@@ -545,6 +559,10 @@ ATMO_HD void s_quad_taps(const Job& j, const SCoords& c, F&& f) {
   const double fz = frac(xz, j.nr), fy = frac(xy, j.nmu);
   const int zs[2] = {tz.i0, tz.i1};
   const int ys[2] = {ty.i0, ty.i1};
+  // The mu anchor of the FP16 blocks: its rows clamp(ya) and clamp(ya + 1)
+  // are ty.i0 and ty.i1.
+  const int ya = quad_axis(xy, j.nmu).a;
+  const bool half = j.half_luts != 0;
   const Real wz[2] = {static_cast<Real>(1.0 - fz), static_cast<Real>(fz)};
   const Real wy[2] = {static_cast<Real>(1.0 - fy), static_cast<Real>(fy)};
   const Real wx[2] = {static_cast<Real>(1.0 - qx.w), static_cast<Real>(qx.w)};
@@ -554,7 +572,10 @@ ATMO_HD void s_quad_taps(const Job& j, const SCoords& c, F&& f) {
       Real w[4];
       for (int e = 0; e < 2; ++e)
         for (int q = 0; q < 2; ++q) w[2 * e + q] = wz[a] * wy[b] * wx[e] * wsl[q];
-      f(quad_index(j, zs[a], ys[b], qs.a, qx.a), w);
+      const int64_t slot =
+          half ? 2 * lookup_copy_block(j, zs[a], ya + 1, qs.a, qx.a) + b
+               : lookup_copy_block(j, zs[a], ys[b], qs.a, qx.a);
+      f(slot, w);
     }
 }
 struct F32Quads {

@@ -1366,38 +1366,41 @@ __global__ __launch_bounds__(kBlock) void k_acc64_to_f32(const Job* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// The lookup copy of S and SM (ATMO_OPTION_INTERLEAVED_LOOKUPS), in quads
-// (s_quad_taps): one thread per quad writes the (S, SM) pairs of its 4
-// texels, FP32 or, for FP16 models, their FP16 texels.
+// The lookup copy of S and SM (ATMO_OPTION_INTERLEAVED_LOOKUPS), in 128-byte
+// blocks of quads (lookup_copy_block): one thread per 16-byte element, so
+// that each store instruction writes whole lines. An element is a float4 of
+// S or SM (FP32 models: 8 per quad) or a texel's S and SM as 4 FP16 each
+// (FP16 models: 4 per quad, 2 quads per block).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_interleave(const Job* __restrict__ jobs,
                                                        const int* __restrict__ first_block,
                                                        int njobs) {
   const int ji = find_job(first_block, njobs, blockIdx.x);
   const Job& J = jobs[ji];
-  const int64_t n = quad_count(J.nr, J.nmu, J.nmus, J.nnu);
-  const int64_t qd = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
-  if (qd >= n || J.Q == nullptr || J.combined) return;
-  // Quad qd = ((z * MU + y) * (NU + 1) + s + 1) * (MU_S + 1) + i + 1.
+  const bool half = J.half_luts != 0;
+  const int64_t n = 8 * lookup_copy_blocks(J.nr, J.nmu, J.nmus, J.nnu, half);
+  const int64_t u = static_cast<int64_t>(blockIdx.x - first_block[ji]) * kBlock + threadIdx.x;
+  if (u >= n || J.Q == nullptr || J.combined) return;
+  // Block qd = ((z * rows + ry) * (NU + 1) + s + 1) * (MU_S + 1) + i + 1.
+  const int64_t qd = u >> 3;
+  const int part = static_cast<int>(u & 7);
   const int ia = static_cast<int>(qd % (J.nmus + 1)) - 1;
   const int64_t rest = qd / (J.nmus + 1);
   const int sa = static_cast<int>(rest % (J.nnu + 1)) - 1;
-  const int64_t row = rest / (J.nnu + 1);
-  for (int e = 0; e < 2; ++e) {
-    const int i = min(max(ia + e, 0), J.nmus - 1);
-    for (int q = 0; q < 2; ++q) {
-      const int sl = min(max(sa + q, 0), J.nnu - 1);
-      const int64_t t = row * (J.nnu * J.nmus) + sl * J.nmus + i;
-      const int k = 2 * e + q;
-      if (J.half_luts) {
-        const uint2 a = gload(J.S16, t), b = gload(J.SM16, t);
-        gstore(static_cast<uint4*>(J.Q), 4 * qd + k, make_uint4(a.x, a.y, b.x, b.y));
-      } else {
-        float4* out = static_cast<float4*>(J.Q);
-        gstore(out, 8 * qd + 2 * k, gload(J.S, t));
-        gstore(out, 8 * qd + 2 * k + 1, gload(J.SM, t));
-      }
-    }
+  const int64_t zr = rest / (J.nnu + 1);
+  const int rows = lookup_copy_rows(J.nmu, half);
+  const int z = static_cast<int>(zr / rows), ry = static_cast<int>(zr % rows);
+  // Tap k = 2 e + q of the quad: (s + q, i + e).
+  const int k = half ? (part & 3) : (part >> 1);
+  const int y = half ? min(max(ry - 1 + (part >> 2), 0), J.nmu - 1) : ry;
+  const int i = min(max(ia + (k >> 1), 0), J.nmus - 1);
+  const int sl = min(max(sa + (k & 1), 0), J.nnu - 1);
+  const int64_t t = (static_cast<int64_t>(z) * J.nmu + y) * (J.nnu * J.nmus) + sl * J.nmus + i;
+  if (half) {
+    const uint2 a = gload(J.S16, t), b = gload(J.SM16, t);
+    gstore(static_cast<uint4*>(J.Q), u, make_uint4(a.x, a.y, b.x, b.y));
+  } else {
+    gstore(static_cast<float4*>(J.Q), u, (part & 1) ? gload(J.SM, t) : gload(J.S, t));
   }
 }
 
@@ -1539,7 +1542,9 @@ int blocks_combine(const Job& j) {
   return static_cast<int>(((n > ne ? n : ne) + kBlock - 1) / kBlock);
 }
 int blocks_lookup_copy(const Job& j) {
-  return static_cast<int>((quad_count(j.nr, j.nmu, j.nmus, j.nnu) + kBlock - 1) / kBlock);
+  return static_cast<int>(
+      (8 * lookup_copy_blocks(j.nr, j.nmu, j.nmus, j.nnu, j.half_luts != 0) + kBlock - 1) /
+      kBlock);
 }
 int blocks_half_conversion(const Job& j) {
   const int64_t n = static_cast<int64_t>(j.nr) * j.nmu * j.nmus * j.nnu;

@@ -90,7 +90,7 @@ struct atmo_model {
   // FP16 storage: resident half LUTs; S and SM above are then FP32
   // accumulators that live only during Init, like the deltas.
   uint2 *S16 = nullptr, *SM16 = nullptr;
-  // ATMO_OPTION_INTERLEAVED_LOOKUPS: the lookup copy of S and SM, in quads
+  // ATMO_OPTION_INTERLEAVED_LOOKUPS: the lookup copy of S and SM, in blocks
   // of the texels a (nu, mu_s) tap set reads (interleaved_bytes).
   bool interleave = true;
   void* Q = nullptr;
@@ -142,13 +142,15 @@ atmo_status validate_sizes(const atmo_lut_sizes& z);
 int num_iterations(const Spec& s);
 // Whether Init accumulates S and SM in FP64 (illuminance mode).
 inline bool accumulates_fp64(const Spec& s) { return s.num_precomputed_wavelengths > 3; }
-// Bytes of the lookup copy of a model with these sizes and storage: quads
-// of 4 (S, SM) texel pairs (64 B FP16, 128 B FP32) per (r, mu) row and
-// anchor (atmo::quad_count); 0 with combined textures, which have no S_Mie.
+// Bytes of the lookup copy of a model with these sizes and storage: one
+// 128-byte block per (r, mu) row (FP32) or (r, mu anchor) (FP16) and
+// (nu, mu_s) anchor (atmo::lookup_copy_blocks); 0 with combined textures,
+// which have no S_Mie.
 inline size_t interleaved_bytes(const Sizes& z, bool half, bool combined) {
   return combined ? 0
-                  : static_cast<size_t>(atmo::quad_count(z.nr, z.nmu, z.nmus, z.nnu)) *
-                        (half ? 4 * sizeof(uint4) : 8 * sizeof(float4));
+                  : static_cast<size_t>(
+                        atmo::lookup_copy_blocks(z.nr, z.nmu, z.nmus, z.nnu, half)) *
+                        8 * sizeof(uint4);
 }
 int pass_blocks(Pass p, const atmo::Job& j);
 // Tile shapes for a model's LUT sizes: the autotuner's choice, or the

@@ -30,6 +30,17 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_capi.SIGNATURES) == names
 
 
+def test_peaks_tool_loads():
+    # The measurement library of bench.py (on-device rate ceilings) is built
+    # beside libatmo.so and exports its one entry point; it rejects a NULL
+    # output array before touching the device.
+    lib = C.CDLL(os.path.join(ROOT, "precomputed_atmospheric_scattering_amd",
+                              "libatmo_peaks.so"))
+    lib.atmo_peaks_measure.restype = C.c_int
+    lib.atmo_peaks_measure.argtypes = [C.POINTER(C.c_double), C.c_int]
+    assert lib.atmo_peaks_measure(None, 7) != 0
+
+
 def params(**over):
     kw, _ = configs.model_kwargs(configs.earth())
     kw.pop("lut_sizes")
