@@ -1004,6 +1004,172 @@ def run_headline(args):
     _json_line(result)
 
 
+# ---------------------------------------------------------------------------
+# Summaries of rocprofv3 outputs (CSV) as the markdown tables of profiles/:
+#   python bench.py --summarize stats DIR
+#   python bench.py --summarize counters TRACE_DIR SQ_DIR FETCH_DIR WRITE_DIR
+#   python bench.py --summarize plain-gaps TRACE_DIR
+# ---------------------------------------------------------------------------
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    for f in sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)):
+        with open(f) as fh:
+            yield from csv.DictReader(fh)
+
+
+def _kernel_base(name):
+    """The kernel's own name in a (possibly demangled, qualified) trace name."""
+    import re
+    m = re.search(r"\b(k_[A-Za-z0-9_]+|__amd_[A-Za-z0-9_]+)", name)
+    return m.group(1) if m else name.split("(")[0].split("<")[0].strip()
+
+
+def _kernel_stats(d):
+    return {_kernel_base(r["Name"]): r for r in _csv_rows(d, "kernel_stats.csv")}
+
+
+def _counters(d):
+    """kernel -> counter -> {dispatch id: value summed over dimensions}."""
+    out = {}
+    for r in _csv_rows(d, "counter_collection.csv"):
+        per = out.setdefault(_kernel_base(r["Kernel_Name"]), {}).setdefault(
+            r["Counter_Name"], {})
+        per[r["Dispatch_Id"]] = per.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def summarize_stats(d):
+    lines = ["| kernel | calls | avg ms | min ms | max ms | total ms | % |",
+             "|---|---|---|---|---|---|---|"]
+    for k, r in sorted(_kernel_stats(d).items(), key=lambda e: -float(e[1]["TotalDurationNs"])):
+        lines.append("| %s | %s | %.3f | %.3f | %.3f | %.2f | %.1f |" % (
+            k, r["Calls"], float(r["AverageNs"]) / 1e6, float(r["MinNs"]) / 1e6,
+            float(r["MaxNs"]) / 1e6, float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
+    return "\n".join(lines)
+
+
+def summarize_counters(trace_dir, sq_dir, fetch_dir, write_dir, probe_mib=256):
+    """Per kernel: trace durations, SQ counters, and HBM bytes from FETCH_SIZE
+    and WRITE_SIZE (KiB) scaled by the factor that makes the copy probe (the
+    largest copyBuffer dispatch, --copy-probe MiB read and written) read and
+    write its own size."""
+    trace, sq = _kernel_stats(trace_dir), _counters(sq_dir)
+    fe, wr = _counters(fetch_dir), _counters(write_dir)
+    cal_f = cal_w = None
+    for k in fe:
+        if "copyBuffer" in k:
+            cal_f = probe_mib * 1024.0 / max(fe[k]["FETCH_SIZE"].values())
+            cal_w = probe_mib * 1024.0 / max(wr[k]["WRITE_SIZE"].values())
+
+    def avg(c, k, n):
+        v = c.get(k, {}).get(n)
+        return sum(v.values()) / len(v) if v else None
+
+    fmt = lambda x, p=1: "-" if x is None else "%.*f" % (p, x)
+    lines = ["calibration: FETCH_SIZE x %s, WRITE_SIZE x %s" % (fmt(cal_f, 3), fmt(cal_w, 3)),
+             "",
+             "| kernel | dispatches (trace) | avg ms (trace) | waves / dispatch | VALU instr / "
+             "dispatch | VALU instr / wave | LDS instr / dispatch | SQ_ACTIVE_INST_VALU / "
+             "dispatch | GRBM_GUI_ACTIVE / dispatch | fetch MB / dispatch | write MB / dispatch "
+             "| HBM GB/s | VALU Tlane-op/s |",
+             "|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    keys = sorted((k for k in set(sq) | set(fe) if not k.startswith("__amd")),
+                  key=lambda k: -float(trace.get(k, {}).get("TotalDurationNs", 0) or 0))
+    for k in keys:
+        t = trace.get(k)
+        ms = float(t["AverageNs"]) / 1e6 if t else None
+        waves, valu = avg(sq, k, "SQ_WAVES"), avg(sq, k, "SQ_INSTS_VALU")
+        f, w = avg(fe, k, "FETCH_SIZE"), avg(wr, k, "WRITE_SIZE")
+        fmb = f * 1024 * cal_f / 1e6 if f is not None and cal_f else None
+        wmb = w * 1024 * cal_w / 1e6 if w is not None and cal_w else None
+        gbs = (fmb + wmb) / ms if ms and fmb is not None and wmb is not None else None
+        vt = valu * 64 / (ms * 1e-3) / 1e12 if ms and valu else None
+        lines.append("| %s | %d | %s | %s | %s | %s | %s | %s | %s | %s | %s | %s | %s |" % (
+            k, int(t["Calls"]) if t else 0, fmt(ms, 3), fmt(waves, 0), fmt(valu, 0),
+            fmt(valu / waves if valu and waves else None, 0), fmt(avg(sq, k, "SQ_INSTS_LDS"), 0),
+            fmt(avg(sq, k, "SQ_ACTIVE_INST_VALU"), 0), fmt(avg(sq, k, "GRBM_GUI_ACTIVE"), 0),
+            fmt(fmb, 1), fmt(wmb, 1), fmt(gbs, 0), fmt(vt, 2)))
+    return "\n".join(lines)
+
+
+def summarize_plain_gaps(trace_dir, seed=2024):
+    """Where the plain one-at-a-time Init loop of the batched workload spends
+    its time, from a kernel trace (--kernel-trace --hip-runtime-trace) of
+    `bench.py --workload batched`: the section runs the 64 Inits three times
+    (warm-up and two timed loops) before anything else; each Init of these
+    3-wavelength models starts with its one k_transmittance dispatch. For the two timed loops, per size class
+    (scattering texels): the Init's span (its first kernel's start to the next
+    Init's), the union of its kernels' intervals, and the HIP runtime calls of
+    the Init's host thread that overlap its idle time."""
+    from precomputed_atmospheric_scattering_amd import configs
+    ks = sorted(((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                  _kernel_base(r["Kernel_Name"])) for r in _csv_rows(trace_dir,
+                                                                     "kernel_trace.csv")))
+    starts = [i for i, k in enumerate(ks) if k[2] == "k_transmittance"]
+    if len(starts) < 3 * 64 + 1:
+        raise SystemExit("expected at least %d Inits in the trace, found %d" %
+                         (3 * 64 + 1, len(starts)))
+    api = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Function"])
+                 for r in _csv_rows(trace_dir, "hip_api_trace.csv"))
+    cfgs = configs.batch(seed, 64)
+    n_s = [texel_counts(c["lut_sizes"])[1] for c in cfgs]
+    classes = {}
+    for loop in (1, 2):
+        for m in range(64):
+            i = loop * 64 + m
+            a, b = starts[i], starts[i + 1]
+            t0, t1 = ks[a][0], ks[b][0]
+            if i == 3 * 64 - 1:
+                # The last plain Init: what follows it is other work of the
+                # section (LUT copies among it), so its span ends with its
+                # last pass kernel.
+                t1 = max(e for _, e, k in ks[a:b] if k.startswith("k_"))
+            busy, cur_s, cur_e = 0, None, None
+            for s, e, _ in ks[a:b]:
+                if s >= t1:
+                    break
+                e = min(e, t1)
+                if cur_e is None or s > cur_e:
+                    if cur_e is not None:
+                        busy += cur_e - cur_s
+                    cur_s, cur_e = s, e
+                else:
+                    cur_e = max(cur_e, e)
+            if cur_e is not None:
+                busy += cur_e - cur_s
+            calls = {}
+            for s, e, f in api:
+                if e < t0 or s > t1:
+                    continue
+                calls[f] = calls.get(f, 0) + min(e, t1) - max(s, t0)
+            c = classes.setdefault(n_s[m], dict(n=0, span=0, busy=0, orders=0, calls={}))
+            c["n"] += 1
+            c["span"] += t1 - t0
+            c["busy"] += busy
+            c["orders"] += cfgs[m]["orders"]
+            for f, d in calls.items():
+                c["calls"][f] = c["calls"].get(f, 0) + d
+    lines = ["| scattering texels | Inits | mean orders | Init span ms | kernels busy ms | "
+             "idle ms | busy fraction | HIP calls in the span (ms per Init) |",
+             "|---|---|---|---|---|---|---|---|"]
+    tot = dict(n=0, span=0, busy=0)
+    for k in sorted(classes):
+        c = classes[k]
+        top = sorted(c["calls"].items(), key=lambda e: -e[1])[:4]
+        lines.append("| %d | %d | %.1f | %.3f | %.3f | %.3f | %.3f | %s |" % (
+            k, c["n"], c["orders"] / c["n"], c["span"] / c["n"] / 1e6,
+            c["busy"] / c["n"] / 1e6, (c["span"] - c["busy"]) / c["n"] / 1e6,
+            c["busy"] / c["span"],
+            ", ".join("%s %.3f" % (f, d / c["n"] / 1e6) for f, d in top)))
+        for key in tot:
+            tot[key] += c[key]
+    lines.append("| all | %d | | %.3f | %.3f | %.3f | %.3f | |" % (
+        tot["n"], tot["span"] / tot["n"] / 1e6, tot["busy"] / tot["n"] / 1e6,
+        (tot["span"] - tot["busy"]) / tot["n"] / 1e6, tot["busy"] / tot["span"]))
+    return "\n".join(lines)
+
+
 def one_line(metric, value, unit, steps, warmup, ms_per_step, higher, workload, extra):
     d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
          "warmup": warmup, "ms_per_step": ms_per_step, "higher_is_better": higher,
@@ -1037,7 +1203,18 @@ def main():
                     help="first copy MIB MiB device-to-device once (one blit kernel "
                          "reading and writing MIB MiB), as the reference point of "
                          "HBM byte counters")
+    ap.add_argument("--summarize", nargs="+", metavar="ARG",
+                    help="print a profiles/ table from rocprofv3 CSV output and exit: "
+                         "'stats DIR', 'counters TRACE SQ FETCH WRITE' or 'plain-gaps TRACE'")
     args = ap.parse_args()
+    if args.summarize:
+        kind, dirs = args.summarize[0], args.summarize[1:]
+        fn = {"stats": summarize_stats, "counters": summarize_counters,
+              "plain-gaps": summarize_plain_gaps}.get(kind)
+        if fn is None:
+            ap.error("unknown --summarize kind %r" % kind)
+        print(fn(*dirs))
+        return None
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
