@@ -67,9 +67,8 @@ QUERY_LANE_OPS_PER_RAY = 1500
 # GetSkyLuminanceToPoint (camera, point, sun, shadow length) 40 + 24 B,
 # GetSunAndSkyIlluminance (point, normal, sun) 36 + 24 B.
 QUERY_IO_BYTES_PER_RAY = 188
-# Scattering lookups per ray over the three (GetSkyRadiance 1, ToPoint 2),
-# each 16 texels (2 r x 2 mu x 2 mu_s x 2 nu) of S and of S_Mie.
-QUERY_SCATTERING_LOOKUPS = 3
+# A scattering lookup reads 16 texels (2 r x 2 mu x 2 mu_s x 2 nu) of S and
+# of S_Mie.
 QUERY_TEXELS_PER_LOOKUP = 16
 WIDTH, HEIGHT = 1920, 1080
 PROFILES = os.path.join(ROOT, "profiles")
@@ -308,6 +307,23 @@ class Lookups:
         self.n = WIDTH * HEIGHT
         self.half = bool(kw.get("half_precision", False))
         self.luts = 1 if kw.get("combine_scattering_textures", False) else 2
+        # Scattering lookups the batch makes, in units of 16 distinct texel
+        # pairs: GetSkyRadiance returns without one for a camera above the
+        # atmosphere whose ray misses it (functions.glsl:1715-1729), and
+        # GetSkyRadianceToPoint makes two per ray. A lookup from a camera
+        # above the atmosphere is made at (or clamped to) the top radius,
+        # whose two r taps are the same LUT layer: 8 distinct pairs.
+        cam = self.q["camera"].astype("float64")
+        v = self.q["view_ray"].astype("float64")
+        r = (cam * cam).sum(1) ** 0.5
+        rmu = (cam * v).sum(1)
+        rt = kw["top_radius"] / 1000.0
+        disc = rmu * rmu - r * r + rt * rt
+        hits = (disc >= 0.0) & (-rmu - abs(disc) ** 0.5 > 0.0)
+        above = r > rt
+        sky = (~above).sum() + 0.5 * (above & hits).sum()
+        to_point = 2.0 * self.n - 0.5 * above.sum()
+        self.scattering_lookups = float(sky + to_point)
         self.q = configs.sky_queries(seed, self.n, kw["bottom_radius"] / 1000.0,
                                      kw["top_radius"] / 1000.0)
         self.dev = {k: atmo.DeviceArray.from_host(v) for k, v in self.q.items()}
@@ -336,7 +352,9 @@ class Lookups:
         """Roofline of one batch of the three lookups. On random rays no two
         rays share a texel, so every texel a lookup reads (the 16 taps of S
         and of S_Mie, in the LUTs' storage format, 8 B per texel FP16, 16 B
-        FP32) is read from beyond the caches when the LUTs exceed them (the
+        FP32; GetSkyRadiance makes one lookup unless its ray misses the
+        atmosphere, GetSkyRadianceToPoint two) is read from beyond the
+        caches when the LUTs exceed them (the
         high-resolution LUTs do: 268 MB FP16, 537 MB FP32, against 256 MB of
         Infinity Cache); those bytes plus the rays' inputs and outputs are
         the algorithmic bytes. Transmittance and irradiance LUTs (1 MB, 64
@@ -345,19 +363,22 @@ class Lookups:
         per line, so their rate is set against the measured line rate of a
         random 128-byte gather over a buffer of the copy's size."""
         texel = 8 if self.half else 16
-        texel_bytes = QUERY_SCATTERING_LOOKUPS * QUERY_TEXELS_PER_LOOKUP * self.luts * texel
+        texel_bytes = (self.scattering_lookups * QUERY_TEXELS_PER_LOOKUP * self.luts * texel /
+                       self.n)
         per_ray = QUERY_IO_BYTES_PER_RAY + texel_bytes
         achieved = per_ray * self.n / batch_s / 1e9
         ideal_io = max(QUERY_BYTES_PER_RAY * self.n / (PEAK_HBM_GBS * 1e9),
                        QUERY_LANE_OPS_PER_RAY * self.n / (PEAK_VALU_TOPS * 1e12))
-        out = {"bound": "hbm", "bytes_per_ray": per_ray,
-               "io_bytes_per_ray": QUERY_IO_BYTES_PER_RAY, "texel_bytes_per_ray": texel_bytes,
+        out = {"bound": "hbm", "bytes_per_ray": round(per_ray, 1),
+               "io_bytes_per_ray": QUERY_IO_BYTES_PER_RAY,
+               "texel_bytes_per_ray": round(texel_bytes, 1),
+               "scattering_lookups_per_ray": round(self.scattering_lookups / self.n, 4),
                "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                "frac": round(achieved / PEAK_HBM_GBS, 4),
                "io_only_frac": round(ideal_io / batch_s, 4)}
         if self.luts == 2:
-            lines = QUERY_SCATTERING_LOOKUPS * (2 if self.half else 4)
-            out["lines_per_ray"] = lines
+            lines = self.scattering_lookups / self.n * (2 if self.half else 4)
+            out["lines_per_ray"] = round(lines, 4)
             out["lines_per_s"] = round(lines * self.n / batch_s, 1)
             rate = MEASURED_PEAKS.get("gather_lines_per_s_2_25gib")
             if rate:
