@@ -84,6 +84,39 @@ def test_full_size_batch_equals_alone():
         m.close()
 
 
+def test_full_workload_batch_equals_alone():
+    # All 64 atmospheres of the batched workload (seed 2024) at their drawn
+    # sizes and orders, batched in one graph as the benchmark runs them:
+    # every LUT of every model equals the model initialised alone, bit for
+    # bit (compared by digest, so that the 64 models' LUTs need not be held
+    # on the host at once).
+    import xxhash
+
+    def digest(m):
+        h = xxhash.xxh3_128()
+        for n in NAMES:
+            h.update(np.ascontiguousarray(m.lut(n)))
+        return h.hexdigest()
+
+    cfgs = configs.batch(2024, 64)
+    ref = []
+    for cfg in cfgs:
+        kw, orders = configs.model_kwargs(cfg)
+        m = atmo.Model(**kw)
+        m.init(orders)
+        ref.append(digest(m))
+        m.close()
+    models = [atmo.Model(**configs.model_kwargs(c)[0]) for c in cfgs]
+    b = atmo.Batch(models, [c["orders"] for c in cfgs], use_graph=True)
+    b.launch()
+    atmo.synchronize()
+    got = [digest(m) for m in models]
+    b.close()
+    for m in models:
+        m.close()
+    assert got == ref, [i for i, (a, r) in enumerate(zip(got, ref)) if a != r]
+
+
 def test_batch_holds_its_models():
     """While a batch lives, its models cannot join another batch, drop their
     interleaved lookup copy or be destroyed, and Init keeps the temporaries

@@ -205,6 +205,60 @@ def test_latencies_are_ordered():
     srv.close()
 
 
+def test_full_size_requests_and_query_batches_equal_alone():
+    """The serving workload's shapes: atmospheres of the batched workload at
+    their drawn (full) LUT sizes, served in batches, with 1920x1080-ray query
+    batches of each lookup chained on them; the LUTs and every lookup output
+    equal those of the same model initialised alone, bit for bit."""
+    cfgs = configs.batch(31, 3)
+    n = 1920 * 1080
+    qs = [queries(c, n, 500 + i) for i, c in enumerate(cfgs)]
+    ref = []
+    for c, q in zip(cfgs, qs):
+        kw, orders = configs.model_kwargs(c)
+        m = atmo.Model(**kw)
+        m.init(orders)
+        o = [outputs(n) for _ in range(3)]
+        m.sky_radiance_device(n, q["camera"], q["view_ray"], q["shadow_length"],
+                              q["sun_direction"], *o[0])
+        m.sky_radiance_to_point_device(n, q["camera"], q["point"], q["shadow_length"],
+                                       q["sun_direction"], *o[1])
+        m.sun_and_sky_irradiance_device(n, q["point"], q["normal"], q["sun_direction"],
+                                        *o[2])
+        atmo.synchronize()
+        ref.append(({k: m.lut(k) for k in LUTS}, [a.to_host() for p in o for a in p]))
+        for p in o:
+            for a in p:
+                a.free()
+        m.close()
+    srv = atmo.Server(max_inflight_batches=2, max_batch_models=3, lookup_streams=2)
+    ids, lookup_ids, outs = [], [], []
+    for c, q in zip(cfgs, qs):
+        kw, orders = configs.model_kwargs(c)
+        mid = srv.submit_precompute(num_scattering_orders=orders, **kw)
+        ids.append(mid)
+        o = [outputs(n) for _ in range(3)]
+        lookup_ids.append(srv.submit_lookup(
+            mid, "sky", n, [q["camera"], q["view_ray"], q["shadow_length"],
+                            q["sun_direction"]], o[0]))
+        lookup_ids.append(srv.submit_lookup(
+            mid, "sky_to_point", n, [q["camera"], q["point"], q["shadow_length"],
+                                     q["sun_direction"]], o[1]))
+        lookup_ids.append(srv.submit_lookup(
+            mid, "sun_and_sky", n, [q["point"], q["normal"], q["sun_direction"]], o[2]))
+        outs.append(o)
+    done = srv.wait(ids + lookup_ids)
+    assert all(done[i]["status"] == 0 for i in ids + lookup_ids), done
+    for i, mid in enumerate(ids):
+        m = srv.model(mid)
+        for k in LUTS:
+            assert np.array_equal(m.lut(k), ref[i][0][k]), (i, k)
+        got = [a.to_host() for p in outs[i] for a in p]
+        for j, (a, b) in enumerate(zip(got, ref[i][1])):
+            assert np.array_equal(a, b), (i, j)
+    srv.close()
+
+
 def test_illuminance_requests_in_mix_equal_alone():
     """Illuminance-mode requests (15 wavelengths: 5 Precompute iterations
     run as parallel jobs with their own accumulators), FP32 and FP16, batched
