@@ -1045,7 +1045,10 @@ __global__ __launch_bounds__(kB) void k_multiple_scattering(
 
   // Per-texel constants: the texel's nu taps (fixed for all samples,
   // functions.glsl:967-973) and r mu_s.
-  int slot_of[kTpt], x_of[kTpt], c0_of[kTpt], c1_of[kTpt];
+  // c0_of, c1_of: LDS byte offsets of the texel's two nu slices in a
+  // blended row (the slice's first texel).
+  int slot_of[kTpt], x_of[kTpt];
+  uint32_t c0_of[kTpt], c1_of[kTpt];
   float ws_of[kTpt];
   double nu_of[kTpt], rmus_of[kTpt];
   float acc[kTpt][3];
@@ -1056,7 +1059,7 @@ __global__ __launch_bounds__(kB) void k_multiple_scattering(
     slot_of[j] = (slot < rpb && row0 + slot < nrows) ? slot : -1;
     x_of[j] = f - slot * W;
     acc[j][0] = acc[j][1] = acc[j][2] = 0.0f;
-    c0_of[j] = c1_of[j] = 0;
+    c0_of[j] = c1_of[j] = 0u;
     ws_of[j] = 0.0f;
     nu_of[j] = rmus_of[j] = 0.0;
     if (slot_of[j] < 0) continue;
@@ -1078,8 +1081,8 @@ __global__ __launch_bounds__(kB) void k_multiple_scattering(
     s0 = s0 < 0 ? 0 : (s0 > nnu - 1 ? nnu - 1 : s0);
     s1 = s1 < 0 ? 0 : (s1 > nnu - 1 ? nnu - 1 : s1);
     ws_of[j] = static_cast<float>(tcx - fl);
-    c0_of[j] = slot * W + s0 * nmus;
-    c1_of[j] = slot * W + s1 * nmus;
+    c0_of[j] = static_cast<uint32_t>(slot * W + s0 * nmus) * sizeof(float3);
+    c1_of[j] = static_cast<uint32_t>(slot * W + s1 * nmus) * sizeof(float3);
     nu_of[j] = nu;
     rmus_of[j] = r * mu_s;
   }
@@ -1153,8 +1156,14 @@ __global__ __launch_bounds__(kB) void k_multiple_scattering(
       float wx;
       const int i0 = split_coord(xin, &wx);
       const int i1 = min(i0 + 1, nmus - 1);
-      const float3 p = B[c0_of[j] + i0], q = B[c0_of[j] + i1];
-      const float3 u = B[c1_of[j] + i0], v = B[c1_of[j] + i1];
+      // Byte offsets of the mu_s taps: 24-bit multiplies (i0, i1 < MU_S).
+      const uint32_t b0 = __umul24(static_cast<uint32_t>(i0), sizeof(float3));
+      const uint32_t b1 = __umul24(static_cast<uint32_t>(i1), sizeof(float3));
+      const char* const Bc = reinterpret_cast<const char*>(B);
+      const float3 p = *reinterpret_cast<const float3*>(Bc + c0_of[j] + b0);
+      const float3 q = *reinterpret_cast<const float3*>(Bc + c0_of[j] + b1);
+      const float3 u = *reinterpret_cast<const float3*>(Bc + c1_of[j] + b0);
+      const float3 v = *reinterpret_cast<const float3*>(Bc + c1_of[j] + b1);
       const float ws = ws_of[j];
       const float lo0 = fmaf(wx, q.x - p.x, p.x), hi0 = fmaf(wx, v.x - u.x, u.x);
       const float lo1 = fmaf(wx, q.y - p.y, p.y), hi1 = fmaf(wx, v.y - u.y, u.y);
