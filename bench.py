@@ -296,6 +296,26 @@ def _progress(msg):
 LOOKUP_NAMES = ("sky_luminance", "sky_luminance_to_point", "sun_and_sky_illuminance")
 
 
+def scattering_lookups(q, top_radius):
+    """Scattering lookups that a batch of the three lookups makes on the
+    rays q (length units), in units of 16 distinct texels of each scattering
+    LUT: GetSkyRadiance returns without one for a camera above the
+    atmosphere whose ray misses it (functions.glsl:1715-1729), and
+    GetSkyRadianceToPoint makes two per ray. A lookup from a camera above
+    the atmosphere is made at (or clamped to) the top radius, whose two r
+    taps are the same LUT layer: 8 distinct texels."""
+    cam = q["camera"].astype("float64")
+    v = q["view_ray"].astype("float64")
+    r = (cam * cam).sum(1) ** 0.5
+    rmu = (cam * v).sum(1)
+    disc = rmu * rmu - r * r + top_radius * top_radius
+    hits = (disc >= 0.0) & (-rmu - abs(disc) ** 0.5 > 0.0)
+    above = r > top_radius
+    sky = (~above).sum() + 0.5 * (above & hits).sum()
+    to_point = 2.0 * len(r) - 0.5 * above.sum()
+    return float(sky + to_point)
+
+
 class Lookups:
     """One 1920x1080 batch of each luminance lookup (GetSkyLuminance,
     GetSkyLuminanceToPoint, GetSunAndSkyIlluminance) on a model, inputs
@@ -307,25 +327,9 @@ class Lookups:
         self.n = WIDTH * HEIGHT
         self.half = bool(kw.get("half_precision", False))
         self.luts = 1 if kw.get("combine_scattering_textures", False) else 2
-        # Scattering lookups the batch makes, in units of 16 distinct texel
-        # pairs: GetSkyRadiance returns without one for a camera above the
-        # atmosphere whose ray misses it (functions.glsl:1715-1729), and
-        # GetSkyRadianceToPoint makes two per ray. A lookup from a camera
-        # above the atmosphere is made at (or clamped to) the top radius,
-        # whose two r taps are the same LUT layer: 8 distinct pairs.
-        cam = self.q["camera"].astype("float64")
-        v = self.q["view_ray"].astype("float64")
-        r = (cam * cam).sum(1) ** 0.5
-        rmu = (cam * v).sum(1)
-        rt = kw["top_radius"] / 1000.0
-        disc = rmu * rmu - r * r + rt * rt
-        hits = (disc >= 0.0) & (-rmu - abs(disc) ** 0.5 > 0.0)
-        above = r > rt
-        sky = (~above).sum() + 0.5 * (above & hits).sum()
-        to_point = 2.0 * self.n - 0.5 * above.sum()
-        self.scattering_lookups = float(sky + to_point)
         self.q = configs.sky_queries(seed, self.n, kw["bottom_radius"] / 1000.0,
                                      kw["top_radius"] / 1000.0)
+        self.scattering_lookups = scattering_lookups(self.q, kw["top_radius"] / 1000.0)
         self.dev = {k: atmo.DeviceArray.from_host(v) for k, v in self.q.items()}
         self.outs = [atmo.DeviceArray((self.n, 3)) for _ in range(6)]
         self.ev = [atmo.Event() for _ in range(4)]
