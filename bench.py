@@ -1033,6 +1033,7 @@ def run_headline(args):
 # Summaries of rocprofv3 outputs (CSV) as the markdown tables of profiles/:
 #   python bench.py --summarize stats DIR
 #   python bench.py --summarize counters TRACE_DIR SQ_DIR FETCH_DIR WRITE_DIR
+#   python bench.py --summarize utilization TRACE_DIR SQ_DIR
 #   python bench.py --summarize plain-gaps TRACE_DIR
 # ---------------------------------------------------------------------------
 def _csv_rows(d, suffix):
@@ -1115,6 +1116,55 @@ def summarize_counters(trace_dir, sq_dir, fetch_dir, write_dir, probe_mib=256):
             fmt(valu / waves if valu and waves else None, 0), fmt(avg(sq, k, "SQ_INSTS_LDS"), 0),
             fmt(avg(sq, k, "SQ_ACTIVE_INST_VALU"), 0), fmt(avg(sq, k, "GRBM_GUI_ACTIVE"), 0),
             fmt(fmb, 1), fmt(wmb, 1), fmt(gbs, 0), fmt(vt, 2)))
+    return "\n".join(lines)
+
+
+UTILIZATION_COUNTERS = ("SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_VALU2 "
+                        "SQ_LDS_IDX_ACTIVE SQ_LDS_BANK_CONFLICT SQ_WAIT_INST_LDS SQ_WAVE_CYCLES "
+                        "GRBM_GUI_ACTIVE")
+
+
+def summarize_utilization(trace_dir, sq_dir):
+    """Issue and LDS utilization per kernel from one rocprofv3 --pmc run of
+    UTILIZATION_COUNTERS (summed over the 8 XCDs by rocprofv3): cycles =
+    GRBM_GUI_ACTIVE / 8; SIMD quad-cycles = 1024 SIMDs x cycles / 4; VALU
+    issue = (SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2) / SIMD quad-cycles,
+    the share of SIMD quad-cycles issuing at least one VALU instruction;
+    dual = 2 SQ_ACTIVE_INST_VALU2 / SQ_ACTIVE_INST_VALU, the share of VALU
+    instructions issued in pairs; LDS busy = SQ_LDS_IDX_ACTIVE / (cycles x
+    256 CUs); bank conflicts = SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE; LDS
+    wait = SQ_WAIT_INST_LDS / SQ_WAVE_CYCLES."""
+    trace, sq = _kernel_stats(trace_dir), _counters(sq_dir)
+
+    def avg(k, n):
+        v = sq.get(k, {}).get(n)
+        return sum(v.values()) / len(v) if v else None
+
+    pct = lambda x: "-" if x is None else "%.0f%%" % (100.0 * x)
+    lines = ["| kernel | dispatches (trace) | avg ms (trace) | VALU instr / wave | VALU issue | "
+             "dual | LDS busy | LDS bank conflicts | LDS wait / wave-cycles |",
+             "|---|---|---|---|---|---|---|---|---|"]
+    keys = sorted((k for k in sq if not k.startswith("__amd")),
+                  key=lambda k: -float(trace.get(k, {}).get("TotalDurationNs", 0) or 0))
+    for k in keys:
+        t = trace.get(k)
+        g, waves, valu = avg(k, "GRBM_GUI_ACTIVE"), avg(k, "SQ_WAVES"), avg(k, "SQ_INSTS_VALU")
+        act, act2 = avg(k, "SQ_ACTIVE_INST_VALU"), avg(k, "SQ_ACTIVE_INST_VALU2")
+        lds, conf = avg(k, "SQ_LDS_IDX_ACTIVE"), avg(k, "SQ_LDS_BANK_CONFLICT")
+        wait, wcyc = avg(k, "SQ_WAIT_INST_LDS"), avg(k, "SQ_WAVE_CYCLES")
+        cycles = g / 8.0 if g else None
+        quad = 1024.0 * cycles / 4.0 if cycles else None
+        issue = (act - act2) / quad if quad and act is not None and act2 is not None else None
+        dual = 2.0 * act2 / act if act and act2 is not None else None
+        busy = lds / (cycles * 256.0) if cycles and lds is not None else None
+        conflicts = conf / lds if lds and conf is not None else None
+        lw = wait / wcyc if wcyc and wait is not None else None
+        lines.append("| %s | %d | %s | %s | %s | %s | %s | %s | %s |" % (
+            k, int(t["Calls"]) if t else 0,
+            "%.3f" % (float(t["AverageNs"]) / 1e6) if t else "-",
+            "%.0f" % (valu / waves) if valu and waves else "-", pct(issue), pct(dual),
+            pct(busy), "-" if conflicts is None else "%.1f%%" % (100.0 * conflicts),
+            "-" if lw is None else "%.1f%%" % (100.0 * lw)))
     return "\n".join(lines)
 
 
@@ -1230,11 +1280,13 @@ def main():
                          "HBM byte counters")
     ap.add_argument("--summarize", nargs="+", metavar="ARG",
                     help="print a profiles/ table from rocprofv3 CSV output and exit: "
-                         "'stats DIR', 'counters TRACE SQ FETCH WRITE' or 'plain-gaps TRACE'")
+                         "'stats DIR', 'counters TRACE SQ FETCH WRITE', 'utilization TRACE "
+                         "SQ' or 'plain-gaps TRACE'")
     args = ap.parse_args()
     if args.summarize:
         kind, dirs = args.summarize[0], args.summarize[1:]
         fn = {"stats": summarize_stats, "counters": summarize_counters,
+              "utilization": summarize_utilization,
               "plain-gaps": summarize_plain_gaps}.get(kind)
         if fn is None:
             ap.error("unknown --summarize kind %r" % kind)

@@ -249,9 +249,7 @@ void atmo_batch_destroy(atmo_batch* batch);
  * when its job table is built. Classes never tuned or set use the defaults
  * (4 nu per thread in blocks of 128 threads when MU x ceil(NU / 4) <= 256,
  * else 256; the smallest multiple-scattering tile that holds a row (1 x 256,
- * 2 x 256, then 2 texels x 512 threads); ray order once
- * the R x MU x MU_S x NU RGB float density LUT reaches 64 MiB, r-major
- * below). */
+ * 2 x 256, then 2 texels x 512 threads); ray order). */
 typedef struct atmo_tiles {
   int sd_nu_per_thread;
   int sd_block_threads;

@@ -100,11 +100,14 @@ ATMO_HD int ms_default_tile(int w) {
 // nearly the same source rows of the scattering density: in ray order the
 // blocks resident on one XCD share their source rows in its L2.
 constexpr int kMsRowOrders = 3;
-// Default: ray order once the scattering-density LUT the pass reads
-// (R x MU x W RGB float texels) reaches kMsRayOrderBytes, r-major below.
-constexpr double kMsRayOrderBytes = 64.0 * 1048576.0;
+// Default: ray order at every size. At the reference sizes (a 12.6 MB
+// delta J per iteration) it fetches 0.25 GB per launch of 5 iterations
+// against 1.05 GB in r-major order, and runs 1.4% faster.
 ATMO_HD int ms_default_row_order(int nr, int nmu, int w) {
-  return static_cast<double>(nr) * nmu * w * 12.0 >= kMsRayOrderBytes ? 2 : 0;
+  (void)nr;
+  (void)nmu;
+  (void)w;
+  return 2;
 }
 
 // Blocks one job needs for each pass.

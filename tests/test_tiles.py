@@ -40,16 +40,13 @@ def test_defaults_follow_row_length():
     assert ms(WIDE) == (2, 512)
 
 
-def test_default_row_order_follows_lut_bytes():
-    # r-major below a 64 MiB scattering-density LUT (RGB float texels), ray
-    # order from there.
-    assert atmo.get_tiles()["ms_row_order"] == "r-major"  # 32 x 128 x 32 x 8: 12 MiB
-    assert atmo.get_tiles(dict(scattering_r=64, scattering_mu_s=48))["ms_row_order"] == \
-        "r-major"
-    big = dict(scattering_r=64, scattering_mu=256, scattering_mu_s=64, scattering_nu=16)
-    assert atmo.get_tiles(big)["ms_row_order"] == "ray"  # 192 MiB
-    assert atmo.get_tiles(dict(big, scattering_nu=8))["ms_row_order"] == "ray"  # 96 MiB
-    assert atmo.get_tiles(dict(big, scattering_nu=4))["ms_row_order"] == "r-major"  # 48 MiB
+def test_default_row_order_is_ray_order():
+    # Ray order at every size (at the reference sizes it fetches a quarter of
+    # the r-major order's bytes).
+    for z in (None, dict(scattering_r=64, scattering_mu_s=48),
+              dict(scattering_r=64, scattering_mu=256, scattering_mu_s=64, scattering_nu=16),
+              dict(scattering_r=4, scattering_mu=16, scattering_mu_s=4, scattering_nu=4)):
+        assert atmo.get_tiles(z)["ms_row_order"] == "ray"
 
 
 def test_set_get_by_size_class():
@@ -72,7 +69,9 @@ def test_set_get_by_size_class():
     assert atmo.get_tiles(SMALL)["ms_row_order"] == "mu-major"
     atmo.set_tiles(SMALL, 4, 256, 2, 256, ms_row_order="ray")
     assert atmo.get_tiles(SMALL)["ms_row_order"] == "ray"
-    assert atmo.get_tiles(other)["ms_row_order"] == "r-major"
+    atmo.set_tiles(SMALL, 4, 256, 2, 256, ms_row_order="r-major")
+    assert atmo.get_tiles(SMALL)["ms_row_order"] == "r-major"
+    assert atmo.get_tiles(other)["ms_row_order"] == "ray"
 
 
 @pytest.mark.parametrize("shape", [(3, 256, 1, 256), (4, 96, 1, 256), (4, 256, 3, 256),
@@ -104,7 +103,7 @@ def test_save_load_round_trip(tmp_path):
     assert after == before
     lines = [l for l in path.read_text().splitlines() if not l.startswith("#")]
     assert sorted(lines) == sorted(["sd 64 4 2 128", "sd 128 16 4 64",
-                                    "ms 16 64 16 4 2 512 0", "ms 32 128 64 16 2 512 2"])
+                                    "ms 16 64 16 4 2 512 2", "ms 32 128 64 16 2 512 2"])
 
 
 @pytest.mark.parametrize("text", ["sd 64 4 3 256\n", "ms 32 128 64 16 1 256 0\n",
