@@ -425,19 +425,28 @@ struct D4 {
 };
 
 // Texel sources of the lookups: FP32 texels, or FP16 texels with per-channel
-// inverse scales.
+// inverse scales. The scales are powers of two, so a lookup sums the stored
+// values and scales its sums once (scale()): multiplying by a power of two
+// commutes with rounding, and the result is the sum of the scaled texels bit
+// for bit (the physical values are far from overflow and underflow).
 struct F32Texels {
   const float4* p;
   ATMO_HD float4 operator[](int64_t i) const { return gload(p, i); }
+  ATMO_HD float4 scale() const { return make_float4(1.0f, 1.0f, 1.0f, 1.0f); }
 };
 struct F16Texels {
   const uint2* p;
   float4 inv;
-  ATMO_HD float4 operator[](int64_t i) const {
-    float4 v = unpack_half4(gload(p, i));
-    return make_float4(v.x * inv.x, v.y * inv.y, v.z * inv.z, v.w * inv.w);
-  }
+  ATMO_HD float4 operator[](int64_t i) const { return unpack_half4(gload(p, i)); }
+  ATMO_HD float4 scale() const { return inv; }
 };
+ATMO_HD D4 scaled(D4 a, float4 s) {
+  return D4{a.x * static_cast<double>(s.x), a.y * static_cast<double>(s.y),
+            a.z * static_cast<double>(s.z), a.w * static_cast<double>(s.w)};
+}
+ATMO_HD float4 scaled(float4 a, float4 s) {
+  return make_float4(a.x * s.x, a.y * s.y, a.z * s.z, a.w * s.w);
+}
 
 // The 16 taps of GetScattering (functions.glsl:958-976) with the GL
 // texture() of each nu slice, a trilinear (mu_s, mu, r) filter: f(texel
@@ -494,7 +503,7 @@ template <class Texels>
 ATMO_HD D4 s_lookup_d(const Job& j, const Texels& lut, const SCoords& c) {
   D4 acc{0.0, 0.0, 0.0, 0.0};
   s_taps(j, c, [&](int64_t i, double wt) { acc_d4(&acc, wt, lut[i]); });
-  return acc;
+  return scaled(acc, lut.scale());
 }
 
 // The lookup copy of S and SM (Job::Q), in 128-byte blocks. For each (r,
@@ -584,17 +593,20 @@ struct F32Quads {
     *s = gload(q, 8 * quad + 2 * k);
     *m = gload(q, 8 * quad + 2 * k + 1);
   }
+  ATMO_HD float4 scale_s() const { return make_float4(1.0f, 1.0f, 1.0f, 1.0f); }
+  ATMO_HD float4 scale_m() const { return make_float4(1.0f, 1.0f, 1.0f, 1.0f); }
 };
+// FP16 quads: stored values, scaled after the sums as for F16Texels.
 struct F16Quads {
   const uint4* q;  // 4 uint4 per quad: S and SM of a texel as 4 x FP16 each
   float4 inv_s, inv_m;
   ATMO_HD void load(int64_t quad, int k, float4* s, float4* m) const {
     const uint4 u = gload(q, 4 * quad + k);
-    const float4 a = unpack_half4(make_uint2(u.x, u.y));
-    const float4 b = unpack_half4(make_uint2(u.z, u.w));
-    *s = make_float4(a.x * inv_s.x, a.y * inv_s.y, a.z * inv_s.z, a.w * inv_s.w);
-    *m = make_float4(b.x * inv_m.x, b.y * inv_m.y, b.z * inv_m.z, b.w * inv_m.w);
+    *s = unpack_half4(make_uint2(u.x, u.y));
+    *m = unpack_half4(make_uint2(u.z, u.w));
   }
+  ATMO_HD float4 scale_s() const { return inv_s; }
+  ATMO_HD float4 scale_m() const { return inv_m; }
 };
 template <class Quads>
 ATMO_HD void s_lookup_quads_d(const Job& j, const Quads& lut, const SCoords& c, D4* s,
@@ -608,8 +620,8 @@ ATMO_HD void s_lookup_quads_d(const Job& j, const Quads& lut, const SCoords& c, 
       acc_d4(&am, w[k], vm);
     }
   });
-  *s = as;
-  *m = am;
+  *s = scaled(as, lut.scale_s());
+  *m = scaled(am, lut.scale_m());
 }
 
 // The same lookups with FP32 weights and sums, for lookups whose result is
@@ -627,7 +639,7 @@ template <class Texels>
 ATMO_HD D4 s_lookup_f(const Job& j, const Texels& lut, const SCoords& c) {
   float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   s_taps<float>(j, c, [&](int64_t i, float wt) { acc_f4(&acc, wt, lut[i]); });
-  return to_d4(acc);
+  return to_d4(scaled(acc, lut.scale()));
 }
 
 template <class Quads>
@@ -642,8 +654,8 @@ ATMO_HD void s_lookup_quads_f(const Job& j, const Quads& lut, const SCoords& c, 
       acc_f4(&am, w[k], vm);
     }
   });
-  *s = to_d4(as);
-  *m = to_d4(am);
+  *s = to_d4(scaled(as, lut.scale_s()));
+  *m = to_d4(scaled(am, lut.scale_m()));
 }
 
 // Real: the type of the weights and sums (double: exact FP64 weights;
