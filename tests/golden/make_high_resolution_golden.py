@@ -3,7 +3,7 @@
 workload (the spectral atmosphere with ozone, 15 precomputed wavelengths,
 LUTs 512x128 / 64x256x64x16 / 128x32), sampled at seeded texels.
 
-The full chain takes about an hour and a half of host time on 8 threads, too
+The full chain takes about 3.6 hours of host time on 6 threads, too
 long for a test run, so it is computed once here and the GPU tests compare
 the product's LUTs (FP32 and FP16 storage) with it:
   - scattering (RGBA) and single_mie_scattering (RGB) at N_TEXELS texels of

@@ -5,7 +5,7 @@ atmosphere with ozone, 15 precomputed wavelengths).
 Two checks:
 
 - End to end: a full FP64 Init (8 orders, 5 Precompute iterations) takes
-  about an hour and a half of host time, so it was computed once by
+  about 3.6 hours of host time on 6 threads, so it was computed once by
   tests/golden/make_high_resolution_golden.py; the GPU's final LUTs, with
   FP32 and with FP16 storage, are compared with it at its 10000 sampled
   texels of the scattering LUTs and over the whole irradiance LUT. The final
