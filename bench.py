@@ -403,6 +403,23 @@ class Lookups:
             "lookup_roofline_frac": rl["frac"],
         }
 
+    def host_buffers(self, model, reps=2):
+        """The same three lookups from and to host (numpy) arrays through the
+        host wrappers: device buffers allocated, inputs copied in, outputs
+        copied back, per call. Rays per second over the three, best of
+        reps."""
+        q, n = self.q, self.n
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            model.sky_luminance(q["camera"], q["view_ray"], q["shadow_length"],
+                                q["sun_direction"])
+            model.sky_luminance_to_point(q["camera"], q["point"], q["shadow_length"],
+                                         q["sun_direction"])
+            model.sun_and_sky_illuminance(q["point"], q["normal"], q["sun_direction"])
+            best = min(best, time.perf_counter() - t0)
+        return round(3.0 * n / best, 1)
+
     def free(self):
         for d in list(self.dev.values()) + self.outs:
             d.free()
@@ -517,6 +534,7 @@ def section_high_resolution(args, steps, warmup, with_lookups=True, keep_luts=Fa
     stream.synchronize()
     _progress("high resolution: warm-up done, %d timed steps" % steps)
     pre_ms, step_ms, wall, launches = timed_inits(atmo, m, orders, stream, steps, lk)
+    host_rays = lk.host_buffers(m) if lk is not None else None
     resident, temporaries = m.footprint()
     luts = None
     if keep_luts:
@@ -544,6 +562,9 @@ def section_high_resolution(args, steps, warmup, with_lookups=True, keep_luts=Fa
     }
     if lk is not None:
         res.update(lk.summary())
+        # Not the headline rate: host arrays in and out (PCIe and
+        # allocations included).
+        res["sky_rays_per_s_host_buffers"] = host_rays
         lk.free()
     meta = dict(cfg=cfg, steps=steps, n_s=n_s, iters=iters, orders=orders, luts=luts)
     return res, launches, meta
