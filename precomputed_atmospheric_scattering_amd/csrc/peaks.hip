@@ -16,6 +16,12 @@
 // The gathers read one 128-byte block per lane at a hashed block index (the
 // access pattern of a sky lookup on random rays: every block a different
 // line, none reused).
+//
+// atmo_peaks_flush(stream) writes a 512 MiB buffer on a stream: launches
+// enqueued after it find none of the data of earlier launches in the L2s or
+// the 256 MB Infinity Cache, and their start is not delayed by the host's
+// enqueue time (the GPU is busy with the fill meanwhile), so HIP events
+// around them time the kernels alone.
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -115,6 +121,12 @@ __global__ __launch_bounds__(kThreads) void k_gather(const float4* __restrict__ 
     s += v.x + v.y + v.z + v.w;
   }
   out[i] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill(float4* __restrict__ p, size_t n4) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < n4;
+       i += static_cast<size_t>(gridDim.x) * kThreads)
+    p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 struct Timer {
@@ -234,6 +246,24 @@ done:
   if (sink != nullptr) (void)hipFree(sink);
   for (int k = 0; k < n && k < 7; ++k) out[k] = r[k];
   return static_cast<int>(err);
+}
+
+// Fills the flush buffer on 'stream' (a hipStream_t; NULL = the default
+// stream). The buffer is allocated by the first call and kept until the
+// process ends. Returns 0 or the HIP error code.
+int atmo_peaks_flush(void* stream) {
+  static float4* buf = nullptr;
+  const size_t kBytes = size_t(512) << 20;
+  if (buf == nullptr) {
+    const hipError_t e = hipMalloc(&buf, kBytes);
+    if (e != hipSuccess) {
+      buf = nullptr;
+      return static_cast<int>(e);
+    }
+  }
+  hipLaunchKernelGGL(k_fill, dim3(256 * 16), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), buf, kBytes / sizeof(float4));
+  return static_cast<int>(hipGetLastError());
 }
 
 }  // extern "C"

@@ -32,13 +32,14 @@ def test_library_exports_every_declared_symbol():
 
 def test_peaks_tool_loads():
     # The measurement library of bench.py (on-device rate ceilings) is built
-    # beside libatmo.so and exports its one entry point; it rejects a NULL
+    # beside libatmo.so and exports its two entry points; it rejects a NULL
     # output array before touching the device.
     lib = C.CDLL(os.path.join(ROOT, "precomputed_atmospheric_scattering_amd",
                               "libatmo_peaks.so"))
     lib.atmo_peaks_measure.restype = C.c_int
     lib.atmo_peaks_measure.argtypes = [C.POINTER(C.c_double), C.c_int]
     assert lib.atmo_peaks_measure(None, 7) != 0
+    assert hasattr(lib, "atmo_peaks_flush")
 
 
 def params(**over):
