@@ -273,8 +273,13 @@ ATMO_HD float4 unpack_half4(uint2 u) {
 // exponent of each channel and rounds its mantissa to nearest (even) at 12
 // bits: relative error at most 2^-13 at any magnitude (plain FP16 would lose
 // the small texels of high orders, the night side, and channels far below
-// the texel's largest), in 8 bytes instead of 12. Channel c's 21 bits are
-// bits 21c .. 21c + 20 of the 64-bit word (x low, y high).
+// the texel's largest), in 8 bytes instead of 12. The layout puts each
+// channel where one or two instructions recover its float bits (the
+// multiple-scattering pass decodes four texels per texel and sample):
+//   x = c0 << 11 | c2 >> 10        c0 in x[31:11], c2[20:10] in x[10:0]
+//   y = c2 << 22 | c1 << 1         c2[9:0] in y[31:22], c1 in y[21:1]
+// so c0 = x masked, c1 = y << 10, and c2 = the 64-bit word (x high, y low)
+// shifted right by 11 (one funnel shift), masked.
 ATMO_HD uint32_t f21_bits(float v) {
   uint32_t b;
   __builtin_memcpy(&b, &v, sizeof(b));
@@ -283,12 +288,13 @@ ATMO_HD uint32_t f21_bits(float v) {
 }
 ATMO_HD uint2 pack_f21x3(float r, float g, float b) {
   const uint32_t c0 = f21_bits(r), c1 = f21_bits(g), c2 = f21_bits(b);
-  return make_uint2(c0 | (c1 << 21), (c1 >> 11) | (c2 << 10));
+  return make_uint2((c0 << 11) | (c2 >> 10), (c2 << 22) | (c1 << 1));
 }
 ATMO_HD float3 unpack_f21x3(uint2 u) {
-  const uint32_t b0 = u.x << 11;
-  const uint32_t b1 = ((u.x >> 21) | (u.y << 11)) << 11;
-  const uint32_t b2 = (u.y >> 10) << 11;
+  const uint32_t b0 = u.x & 0xFFFFF800u;
+  const uint32_t b1 = u.y << 10;
+  const uint32_t b2 =
+      static_cast<uint32_t>(((static_cast<uint64_t>(u.x) << 32) | u.y) >> 11) & 0xFFFFF800u;
   float3 v;
   __builtin_memcpy(&v.x, &b0, sizeof(float));
   __builtin_memcpy(&v.y, &b1, sizeof(float));
