@@ -333,8 +333,15 @@ ATMO_HD float min1(float v) { return v < 1.0f ? v : 1.0f; }
 // Clamp to [0, hi]; NaN maps to 0. Used before every float -> index
 // conversion so that indices stay in bounds whatever the input.
 ATMO_HD float clamp_index_coord(float t, float hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // One v_med3_f32 (the median of t, 0 and hi >= 0); with a NaN operand it
+  // returns the minimum of the other two, 0. fminf(fmaxf()) takes three
+  // instructions here: hi is canonicalized before the min.
+  return __builtin_amdgcn_fmed3f(t, 0.0f, hi);
+#else
   // fmaxf returns the non-NaN operand.
   return fminf(fmaxf(t, 0.0f), hi);
+#endif
 }
 
 // functions.glsl:493-519
