@@ -40,6 +40,37 @@ def test_cie_table_published_values():
     assert cie(360.0, 2) == 0.0 and cie(830.0, 1) == 0.0
 
 
+# CIE 1931 2-degree colour-matching functions at single wavelengths, as the
+# CIE publishes them (CIE 15:2004 tables, 5 nm steps): a check of the shared
+# table against an outside source, value by value, so that a wrong or shifted
+# entry fails even where the integrals above still hold.
+CIE_1931_PUBLISHED = {
+    380.0: (0.001368, 0.000039, 0.006450),
+    400.0: (0.014310, 0.000396, 0.067850),
+    420.0: (0.134380, 0.004000, 0.645600),
+    440.0: (0.348280, 0.023000, 1.747060),
+    450.0: (0.336200, 0.038000, 1.772110),
+    460.0: (0.290800, 0.060000, 1.669200),
+    480.0: (0.095640, 0.139020, 0.812950),
+    500.0: (0.004900, 0.323000, 0.272000),
+    520.0: (0.063270, 0.710000, 0.078250),
+    540.0: (0.290400, 0.954000, 0.020300),
+    560.0: (0.594500, 0.995000, 0.003900),
+    580.0: (0.916300, 0.870000, 0.001650),
+    600.0: (1.062200, 0.631000, 0.000800),
+    620.0: (0.854450, 0.381000, 0.000190),
+    640.0: (0.447900, 0.175000, 0.000020),
+    650.0: (0.283500, 0.107000, 0.000000),
+    700.0: (0.011359, 0.004102, 0.000000),
+}
+
+
+@pytest.mark.parametrize("lam", sorted(CIE_1931_PUBLISHED))
+def test_cie_table_matches_published_samples(lam):
+    for col, want in zip((1, 2, 3), CIE_1931_PUBLISHED[lam]):
+        assert cie(lam, col) == pytest.approx(want, rel=1e-4, abs=1e-6), (lam, col)
+
+
 def test_xyz_to_srgb_maps_d65_white_to_one():
     # sRGB (IEC 61966-2-1): D65 white XYZ = (0.9505, 1.0, 1.0890) -> (1, 1, 1).
     m = np.array([[3.2406, -1.5372, -0.4986], [-0.9689, 1.8758, 0.0415],
