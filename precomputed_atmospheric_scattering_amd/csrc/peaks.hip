@@ -249,8 +249,10 @@ done:
 }
 
 // Fills the flush buffer on 'stream' (a hipStream_t; NULL = the default
-// stream). The buffer is allocated by the first call and kept until the
-// process ends. Returns 0 or the HIP error code.
+// stream). The buffer is allocated on the current device by the first call
+// and kept until the process ends; later calls must use that device, from
+// one thread at a time (a measurement tool). Returns 0 or the HIP error
+// code.
 int atmo_peaks_flush(void* stream) {
   static float4* buf = nullptr;
   const size_t kBytes = size_t(512) << 20;
