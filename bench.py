@@ -47,6 +47,13 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (packed FMA = 2 flops per lane)
 # FMA lanes per clock (two VALU instructions of a wave per 4 clocks per SIMD,
 # or one packed). Every VALU instruction of a wave counts 64 lane-ops.
 PEAK_VALU_TOPS = PEAK_FP32_TFLOPS / 2.0
+# FP arithmetic (add, mul, FMA) lane-instructions per second when they issue
+# one per SIMD per 4 clocks: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz. On
+# MI355X two FP32 arithmetic instructions do not issue together (csrc/
+# peaks.hip measures FMA + add pairs at the single rate, FMA + integer or
+# conversion pairs at twice it), and packed FP32 is not used here, so this
+# is the ceiling of a kernel made of FP32 (or FP64) arithmetic.
+PEAK_FP_ARITH_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 PEAK_HBM_GBS = 8000.0
 # Algorithmic work per integration sample (VALU lane-ops, transcendentals
 # weighted 4, loop invariants hoisted): frozen estimates of the reference's
@@ -73,6 +80,7 @@ QUERY_TEXELS_PER_LOOKUP = 16
 WIDTH, HEIGHT = 1920, 1080
 PROFILES = os.path.join(ROOT, "profiles")
 COUNTERS_HIGH_RES = os.path.join(PROFILES, "counters_high_resolution.md")
+VALU_MIX_HIGH_RES = os.path.join(PROFILES, "valu_mix_high_resolution.md")
 # Pass -> kernel of the counter summary (the transmittance pass's launches
 # are two kernels, the optical depths and the LUT, and carry no counters).
 PASS_KERNELS = dict(single_scattering="k_single_scattering",
@@ -137,6 +145,19 @@ def counter_figures(path=COUNTERS_HIGH_RES):
             out[k] = dict(valu=float(row["VALU instr / dispatch"]),
                           fetch=float(row["fetch MB / dispatch"]) * 1e6,
                           write=float(row["write MB / dispatch"]) * 1e6)
+        except (KeyError, ValueError):
+            continue
+    return out
+
+
+def valu_mix_figures(path=VALU_MIX_HIGH_RES):
+    """Per kernel: FP32 and FP64 arithmetic (add, mul, FMA) wave-instructions
+    per dispatch, from the committed instruction-mix summary of the
+    high-resolution workload."""
+    out = {}
+    for k, row in profile_table(path).items():
+        try:
+            out[k] = float(row["FP32 arith / dispatch"]) + float(row["FP64 arith / dispatch"])
         except (KeyError, ValueError):
             continue
     return out
@@ -434,7 +455,8 @@ class Lookups:
 MEASURED_PEAKS = {}
 PEAK_KEYS = ("fp32_fma_lane_instr_per_s", "valu_dual_issue_lane_instr_per_s",
              "fp64_fma_lane_instr_per_s", "hbm_read_bytes_per_s", "hbm_copy_bytes_per_s",
-             "gather_lines_per_s_1gib", "gather_lines_per_s_2_25gib")
+             "gather_lines_per_s_1gib", "gather_lines_per_s_2_25gib",
+             "fp32_fma_add_lane_instr_per_s", "fp32_fma_int_lane_instr_per_s")
 
 
 _PEAKS_LIB = []
@@ -466,7 +488,8 @@ def measure_peaks():
     """Microbenchmarks of the rate ceilings on this GPU: FP32 FMA issue
     (plain, non-packed instructions), the issue rate of an FMA/add/fract mix
     that the SIMDs issue two at a time, FP64 FMA issue, HBM streaming read
-    and copy, and random 128-byte line gathers over 1 GiB and 2.25 GiB."""
+    and copy, random 128-byte line gathers over 1 GiB and 2.25 GiB, and the
+    issue rates of FMA + FP32 add and FMA + integer instruction pairs."""
     import ctypes as C
     lib = peaks_lib()
     out = (C.c_double * len(PEAK_KEYS))()
@@ -482,6 +505,8 @@ def measure_peaks():
         "hbm_copy_gb_per_s": round(out[4] / 1e9, 1),
         "gather_lines_per_s_1gib": round(out[5], 1),
         "gather_lines_per_s_2_25gib": round(out[6], 1),
+        "fp32_fma_plus_add_tops": round(out[7] / 1e12, 2),
+        "fp32_fma_plus_int_tops": round(out[8] / 1e12, 2),
         "unit_note": "TOP/s are VALU lane-instructions per second (an FMA counts 1)",
     }
 
@@ -602,6 +627,7 @@ def pass_figures(launches, steps, n_s, iters):
     ms per atmosphere, VALU lane-op rate and HBM rate, each as a fraction of
     its MI355X peak."""
     cnt = counter_figures()
+    mix = valu_mix_figures()
     merged = {}
     for k, v in launches.items():
         merged.setdefault("scattering_density" if k.startswith("scattering_density") else k,
@@ -619,6 +645,10 @@ def pass_figures(launches, steps, n_s, iters):
             e["hbm_bytes_per_launch"] = round(c["fetch"] + c["write"])
             e["hbm_gb_per_s"] = round((c["fetch"] + c["write"]) / avg_s / 1e9, 1)
             e["hbm_frac"] = round((c["fetch"] + c["write"]) / avg_s / 1e9 / PEAK_HBM_GBS, 4)
+        fa = mix.get(PASS_KERNELS.get(name, ""))
+        if fa is not None:
+            e["fp_arith_tops"] = round(fa * 64.0 / avg_s / 1e12, 3)
+            e["fp_arith_frac"] = round(fa * 64.0 / avg_s / 1e12 / PEAK_FP_ARITH_TOPS, 4)
         out[name] = e
     if "scattering_density" in out:
         # The frozen per-sample estimate of the reference's work, for scale.
@@ -648,6 +678,14 @@ def roofline(passes):
         m = measured / 1e12 if valu else measured / 1e9
         a = p["valu_tops"] if valu else p["hbm_gb_per_s"]
         extra = {"peak_measured": round(m, 2), "frac_of_measured_peak": round(a / m, 4)}
+    fp_arith = {}
+    if "fp_arith_tops" in p:
+        fp_arith = {"fp_arith": {
+            "achieved": p["fp_arith_tops"], "peak": round(PEAK_FP_ARITH_TOPS, 2),
+            "unit": "TOP/s (FP32/FP64 add, mul and FMA lane-instructions; one per SIMD "
+                    "per 4 clocks: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz)",
+            "frac": p["fp_arith_frac"],
+            "counters": os.path.relpath(VALU_MIX_HIGH_RES, ROOT)}}
     return dict({
         "kernel": PASS_KERNELS[name],
         "bound": "valu" if valu else "hbm",
@@ -661,7 +699,7 @@ def roofline(passes):
         "valu_frac": p["valu_frac"],
         "hbm_frac": p["hbm_frac"],
         "counters": os.path.relpath(COUNTERS_HIGH_RES, ROOT),
-    }, **extra)
+    }, **extra, **fp_arith)
 
 
 def section_spectral(args, steps, warmup):
@@ -1082,6 +1120,7 @@ def run_headline(args):
 #   python bench.py --summarize stats DIR
 #   python bench.py --summarize counters TRACE_DIR SQ_DIR FETCH_DIR WRITE_DIR
 #   python bench.py --summarize utilization TRACE_DIR SQ_DIR
+#   python bench.py --summarize valu-mix TRACE_DIR MIX32_DIR MIX64_DIR
 #   python bench.py --summarize plain-gaps TRACE_DIR
 # ---------------------------------------------------------------------------
 def _csv_rows(d, suffix):
@@ -1216,6 +1255,50 @@ def summarize_utilization(trace_dir, sq_dir):
     return "\n".join(lines)
 
 
+MIX32_COUNTERS = ("SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VALU_ADD_F32 SQ_INSTS_VALU_MUL_F32 "
+                  "SQ_INSTS_VALU_FMA_F32 SQ_INSTS_VALU_TRANS_F32 SQ_INSTS_VALU_CVT "
+                  "SQ_INSTS_VALU_INT32")
+MIX64_COUNTERS = ("SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 "
+                  "SQ_INSTS_VALU_FMA_F64 SQ_INSTS_VALU_TRANS_F64 SQ_INSTS_VALU_INT64")
+
+
+def summarize_valu_mix(trace_dir, mix32_dir, mix64_dir):
+    """The VALU instruction mix per kernel from two rocprofv3 --pmc runs
+    (MIX32_COUNTERS, MIX64_COUNTERS): FP32 and FP64 arithmetic (add + mul +
+    FMA), transcendental, conversion and integer instructions per dispatch,
+    as shares of SQ_INSTS_VALU, and the FP arithmetic lane-instruction rate
+    over the trace's average duration against PEAK_FP_ARITH_TOPS."""
+    trace, m32, m64 = _kernel_stats(trace_dir), _counters(mix32_dir), _counters(mix64_dir)
+
+    def avg(c, k, n):
+        v = c.get(k, {}).get(n)
+        return sum(v.values()) / len(v) if v else 0.0
+
+    lines = ["| kernel | dispatches (trace) | avg ms (trace) | VALU instr / dispatch | "
+             "FP32 arith / dispatch | FP64 arith / dispatch | FP32 arith | FP64 arith | "
+             "transcendental | conversions | integer | FP arith Tlane-op/s | of single issue |",
+             "|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    keys = sorted((k for k in m32 if not k.startswith("__amd")),
+                  key=lambda k: -float(trace.get(k, {}).get("TotalDurationNs", 0) or 0))
+    for k in keys:
+        t = trace.get(k)
+        ms = float(t["AverageNs"]) / 1e6 if t else None
+        valu = avg(m32, k, "SQ_INSTS_VALU")
+        f32 = sum(avg(m32, k, "SQ_INSTS_VALU_%s_F32" % o) for o in ("ADD", "MUL", "FMA"))
+        f64 = sum(avg(m64, k, "SQ_INSTS_VALU_%s_F64" % o) for o in ("ADD", "MUL", "FMA"))
+        trans = avg(m32, k, "SQ_INSTS_VALU_TRANS_F32") + avg(m64, k, "SQ_INSTS_VALU_TRANS_F64")
+        cvt = avg(m32, k, "SQ_INSTS_VALU_CVT")
+        ints = avg(m32, k, "SQ_INSTS_VALU_INT32") + avg(m64, k, "SQ_INSTS_VALU_INT64")
+        share = lambda x: "%.0f%%" % (100.0 * x / valu) if valu else "-"
+        rate = (f32 + f64) * 64 / (ms * 1e-3) / 1e12 if ms else None
+        lines.append("| %s | %d | %s | %.0f | %.0f | %.0f | %s | %s | %s | %s | %s | %s | %s |" % (
+            k, int(t["Calls"]) if t else 0, "%.3f" % ms if ms else "-", valu, f32, f64,
+            share(f32), share(f64), share(trans), share(cvt), share(ints),
+            "-" if rate is None else "%.2f" % rate,
+            "-" if rate is None else "%.2f" % (rate / PEAK_FP_ARITH_TOPS)))
+    return "\n".join(lines)
+
+
 def summarize_plain_gaps(trace_dir, seed=2024):
     """Where the plain one-at-a-time Init loop of the batched workload spends
     its time, from a kernel trace (--kernel-trace --hip-runtime-trace) of
@@ -1329,12 +1412,12 @@ def main():
     ap.add_argument("--summarize", nargs="+", metavar="ARG",
                     help="print a profiles/ table from rocprofv3 CSV output and exit: "
                          "'stats DIR', 'counters TRACE SQ FETCH WRITE', 'utilization TRACE "
-                         "SQ' or 'plain-gaps TRACE'")
+                         "SQ', 'valu-mix TRACE MIX32 MIX64' or 'plain-gaps TRACE'")
     args = ap.parse_args()
     if args.summarize:
         kind, dirs = args.summarize[0], args.summarize[1:]
         fn = {"stats": summarize_stats, "counters": summarize_counters,
-              "utilization": summarize_utilization,
+              "utilization": summarize_utilization, "valu-mix": summarize_valu_mix,
               "plain-gaps": summarize_plain_gaps}.get(kind)
         if fn is None:
             ap.error("unknown --summarize kind %r" % kind)

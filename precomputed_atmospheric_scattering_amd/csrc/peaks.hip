@@ -11,7 +11,11 @@
 //   out[3] HBM streaming read, bytes/s (float4 loads over 2 GiB);
 //   out[4] HBM copy, bytes read + written per second (1 GiB to 1 GiB);
 //   out[5] random 128-byte line gathers over 1 GiB, lines/s;
-//   out[6] random 128-byte line gathers over 2.25 GiB, lines/s.
+//   out[6] random 128-byte line gathers over 2.25 GiB, lines/s;
+//   out[7] VALU issue rate of one v_fma_f32 and one v_add_f32 per chain
+//          step, lane-instructions/s (FP32 arithmetic with FP32 arithmetic);
+//   out[8] VALU issue rate of one v_fma_f32 and one v_lshl_add_u32 per
+//          chain step, lane-instructions/s (FP32 arithmetic with integer).
 //
 // The gathers read one 128-byte block per lane at a hashed block index (the
 // access pattern of a sky lookup on random rays: every block a different
@@ -62,6 +66,47 @@ __global__ __launch_bounds__(kThreads) void k_mix32(float* out, int iters, float
   float s = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; ++k) s += a[k] + t[k];
+  if (s == 12345.678f) out[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_fma_add32(float* out, int iters, float b, float c) {
+  float a[8], t[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = threadIdx.x * 1e-3f + k;
+    t[k] = 0.5f + k * 1e-2f;
+  }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      a[k] = fmaf(a[k], b, c);
+      t[k] = t[k] + c;
+    }
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += a[k] + t[k];
+  if (s == 12345.678f) out[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_fma_int(float* out, int iters, float b, float c) {
+  float a[8];
+  unsigned q[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = threadIdx.x * 1e-3f + k;
+    q[k] = threadIdx.x + k;
+  }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      a[k] = fmaf(a[k], b, c);
+      q[k] = (q[k] << 5) + 77u;
+    }
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += a[k] + static_cast<float>(q[k]);
   if (s == 12345.678f) out[blockIdx.x] = s;
 }
 
@@ -178,12 +223,12 @@ struct Timer {
 
 extern "C" {
 
-// Fills out[0..n) (n <= 7) with the figures listed above. Returns 0, or the
+// Fills out[0..n) (n <= 9) with the figures listed above. Returns 0, or the
 // HIP error code of the first failing call. Allocates 2.25 GiB + 32 MiB of
 // device memory for the duration of the call.
 int atmo_peaks_measure(double* out, int n) {
   if (out == nullptr || n < 0) return static_cast<int>(hipErrorInvalidValue);
-  double r[7] = {0, 0, 0, 0, 0, 0, 0};
+  double r[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   hipError_t err = hipSuccess;
   Timer tm;
   float4* buf = nullptr;
@@ -214,6 +259,16 @@ int atmo_peaks_measure(double* out, int n) {
                          reinterpret_cast<double*>(sink), kIters, 0.999, 1e-3);
     }, &ms));
     r[2] = lane_steps / (ms * 1e-3);
+    PEAKS_TRY(tm.best_ms(3, [&](hipStream_t s) {
+      hipLaunchKernelGGL(k_fma_add32, dim3(kBlocks), dim3(kThreads), 0, s, sink, kIters,
+                         0.999f, 1e-3f);
+    }, &ms));
+    r[7] = 2.0 * lane_steps / (ms * 1e-3);
+    PEAKS_TRY(tm.best_ms(3, [&](hipStream_t s) {
+      hipLaunchKernelGGL(k_fma_int, dim3(kBlocks), dim3(kThreads), 0, s, sink, kIters, 0.999f,
+                         1e-3f);
+    }, &ms));
+    r[8] = 2.0 * lane_steps / (ms * 1e-3);
   }
   {
     const size_t read_bytes = size_t(2) << 30;
@@ -244,7 +299,7 @@ int atmo_peaks_measure(double* out, int n) {
 done:
   if (buf != nullptr) (void)hipFree(buf);
   if (sink != nullptr) (void)hipFree(sink);
-  for (int k = 0; k < n && k < 7; ++k) out[k] = r[k];
+  for (int k = 0; k < n && k < 9; ++k) out[k] = r[k];
   return static_cast<int>(err);
 }
 

@@ -81,6 +81,38 @@ def test_summaries(tmp_path):
     assert row[13].strip() == "32.00"
 
 
+def test_valu_mix_summary(tmp_path):
+    trace, m32, m64 = (tmp_path / n for n in ("trace", "m32", "m64"))
+    for d in (trace, m32, m64):
+        d.mkdir()
+    write_csv(trace / "run_kernel_stats.csv",
+              ["Name", "Calls", "TotalDurationNs", "AverageNs", "Percentage", "MinNs", "MaxNs",
+               "StdDev"],
+              [['"k_a"', 1, 1000000, 1000000, 100.0, 1000000, 1000000, 0]])
+    cols = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+    k = '"void atmo::k_a<4>(int)"'
+    write_csv(m32 / "run_counter_collection.csv", cols,
+              [[1, k, "SQ_INSTS_VALU", 1000], [1, k, "SQ_INSTS_VALU_FMA_F32", 500],
+               [1, k, "SQ_INSTS_VALU_ADD_F32", 100], [1, k, "SQ_INSTS_VALU_MUL_F32", 100],
+               [1, k, "SQ_INSTS_VALU_CVT", 50], [1, k, "SQ_INSTS_VALU_INT32", 150]])
+    write_csv(m64 / "run_counter_collection.csv", cols,
+              [[1, k, "SQ_INSTS_VALU", 1000], [1, k, "SQ_INSTS_VALU_FMA_F64", 100]])
+    table = bench.summarize_valu_mix(str(trace), str(m32), str(m64))
+    row = [l for l in table.splitlines() if l.startswith("| k_a |")][0].split("|")
+    # 700 FP32 + 100 FP64 arithmetic wave instructions x 64 lanes in 1 ms.
+    assert row[5].strip() == "700" and row[6].strip() == "100"
+    assert row[7].strip() == "70%" and row[10].strip() == "5%" and row[11].strip() == "15%"
+    rate = 800 * 64 / 1e-3 / 1e12
+    assert row[12].strip() == "%.2f" % rate
+    assert row[13].strip() == "%.2f" % (rate / bench.PEAK_FP_ARITH_TOPS)
+
+
+def test_fp_arith_peak():
+    # One FP arithmetic instruction per SIMD per 4 clocks: 256 CUs x 4 SIMDs
+    # x 64 lanes / 4 clocks x 2.4 GHz.
+    assert abs(bench.PEAK_FP_ARITH_TOPS - 39.3216) < 1e-9
+
+
 def test_more_than_one_gpu_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True)
