@@ -293,11 +293,52 @@ __device__ __forceinline__ void tsun_lookup(const Job& J, const SsSample<double>
   o[2] = fma(wy1, fma(w, double(d.z) - c.z, double(c.z)), wy0 * fma(w, double(b.z) - a.z, double(a.z)));
 }
 
+// The outputs of one single-scattering texel: the delta Rayleigh and Mie
+// LUTs, and the scattering accumulators (FP64 in illuminance mode).
+template <bool kF64, class Real>
+__device__ __forceinline__ void ss_store(const Job& J, int64_t t_idx, double dx,
+                                         const Real (&ar)[3], const Real (&am)[3]) {
+  double ray[3], mie[3];
+  for (int c = 0; c < 3; ++c) {
+    ray[c] = static_cast<double>(ar[c]) * (dx * J.solar[c] * J.rayleigh[c]);
+    mie[c] = static_cast<double>(am[c]) * (dx * J.solar[c] * J.mie_sca[c]);
+  }
+  gstore(J.dR, t_idx, make_float4(static_cast<float>(ray[0]), static_cast<float>(ray[1]),
+                                  static_cast<float>(ray[2]), 0.0f));
+  gstore(J.dM, t_idx, make_float4(static_cast<float>(mie[0]), static_cast<float>(mie[1]),
+                                  static_cast<float>(mie[2]), 0.0f));
+  if (kF64) {
+    const D4 lr = apply_lum_d(J.lum, ray), lm = apply_lum_d(J.lum, mie);
+    store_d4(J.S64, t_idx, D4{lr.x, lr.y, lr.z, lm.x});
+    if (!J.combined) store_d4(J.SM64, t_idx, D4{lm.x, lm.y, lm.z, 0.0});
+    return;
+  }
+  const float3 rayf = make_float3(static_cast<float>(ray[0]), static_cast<float>(ray[1]),
+                                  static_cast<float>(ray[2]));
+  const float3 mief = make_float3(static_cast<float>(mie[0]), static_cast<float>(mie[1]),
+                                  static_cast<float>(mie[2]));
+  const float3 lr = apply_lum(J.lum, rayf);
+  const float3 lm = apply_lum(J.lum, mief);
+  gstore(J.S, t_idx, make_float4(lr.x, lr.y, lr.z, lm.x));
+  if (!J.combined) gstore(J.SM, t_idx, make_float4(lm.x, lm.y, lm.z, 0.0f));
+}
+
+// Texels per thread of the single-scattering pass for packed rows of W
+// texels: 2 once a row fills two blocks (texels x and x + kBlock of a row
+// then share each sample's transmittance rows), else 1.
+__host__ __device__ inline int ss_texels_per_thread(int w) {
+  return w >= 4 * kBlock ? 4 : (w >= 2 * kBlock ? 2 : 1);
+}
+
 // kF64: the kernel serves the jobs with FP64 accumulators (illuminance
 // mode) and computes the sun-side lookups and the sums in FP64; otherwise it
-// serves the others. Both are launched over the same blocks when a launch
-// holds both kinds; each returns at once on the other's jobs.
-template <bool kF64>
+// serves the others. kTpt: texels per thread, ss_texels_per_thread(W) of the
+// job (texels x and x + kBlock of a row for 2: their sun lookups read the
+// same two transmittance rows per sample, so a wave has twice the gathers in
+// flight and the second's lines are often in L1 already). The instances are
+// launched over the same blocks when a launch holds several kinds; each
+// returns at once on the others' jobs.
+template <bool kF64, int kTpt>
 __global__ __launch_bounds__(kBlock) void k_single_scattering(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
     int njobs) {
@@ -306,9 +347,10 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
   const int ji = find_job(first_block, njobs, blockIdx.x);
   const Job& J = jobs[ji];
   if ((J.acc64 != 0) != kF64) return;
-  const int lb = blockIdx.x - first_block[ji];
   const int W = J.nnu * J.nmus;
-  const int tiles = (W + kBlock - 1) / kBlock;
+  if (ss_texels_per_thread(W) != kTpt) return;
+  const int lb = blockIdx.x - first_block[ji];
+  const int tiles = (W + kBlock * kTpt - 1) / (kBlock * kTpt);
   const int tile = lb % tiles;
   const int row = lb / tiles;  // z * nmu + y
   const int y = row % J.nmu, z = row / J.nmu;
@@ -368,54 +410,52 @@ __global__ __launch_bounds__(kBlock) void k_single_scattering(
   }
   __syncthreads();
 
-  const int x = tile * kBlock + tid;
-  if (x >= W) return;
-  const double mu_s = s_col_mus(J, x % J.nmus);
-  const double nu = s_slice_nu(J, x / J.nmus, mu, mu_s);
-  const double rmus = r * mu_s;
-  Real ar[3] = {0, 0, 0}, am[3] = {0, 0, 0};
+  double mu_s[kTpt], nu[kTpt], rmus[kTpt];
+  bool on[kTpt];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < kTpt; ++j) {
+    const int x = tile * kBlock * kTpt + j * kBlock + tid;
+    on[j] = x < W;
+    any = any || on[j];
+    mu_s[j] = on[j] ? s_col_mus(J, x % J.nmus) : 0.0;
+    nu[j] = on[j] ? s_slice_nu(J, x / J.nmus, mu, mu_s[j]) : 0.0;
+    rmus[j] = r * mu_s[j];
+  }
+  if (!any) return;
+  Real ar[kTpt][3], am[kTpt][3];
+#pragma unroll
+  for (int j = 0; j < kTpt; ++j)
+    for (int c = 0; c < 3; ++c) ar[j][c] = am[j][c] = Real(0);
   for (int s = 0; s < kMarch; ++s) {
     const SsSample<Real>& S = smp[s];
-    const double t = fma(S.d, nu, rmus);  // r * mu_s + d * nu = r_d * mu_s_d
-    const Real xs = static_cast<Real>((t + S.rho) * S.inv_r);  // mu_s_d - cos(theta_h)
-    // Clamps as max/min, 2 instructions each (the operands are finite; the
-    // ternary form, which also keeps NaNs, compiles to compares and selects).
-    const Real mus = fmin(fmax(static_cast<Real>(t * S.inv_r), Real(-1)), Real(1));
-    const Real u = fmin(fmax((xs + S.e) * S.inv_span, Real(0)), Real(1));
-    const Real vis = u * u * fma(Real(-2), u, Real(3));
-    Real ts[3];
-    tsun_lookup(J, S, mus, ts);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const Real f = ts[c] * vis;
-      ar[c] = fma(f, S.fr[c], ar[c]);
-      am[c] = fma(f, S.fm[c], am[c]);
+    for (int j = 0; j < kTpt; ++j) {
+      if (!on[j]) continue;
+      const double t = fma(S.d, nu[j], rmus[j]);  // r * mu_s + d * nu = r_d * mu_s_d
+      const Real xs = static_cast<Real>((t + S.rho) * S.inv_r);  // mu_s_d - cos(theta_h)
+      // Clamps as max/min, 2 instructions each (the operands are finite; the
+      // ternary form, which also keeps NaNs, compiles to compares and
+      // selects).
+      const Real mus = fmin(fmax(static_cast<Real>(t * S.inv_r), Real(-1)), Real(1));
+      const Real u = fmin(fmax((xs + S.e) * S.inv_span, Real(0)), Real(1));
+      const Real vis = u * u * fma(Real(-2), u, Real(3));
+      Real ts[3];
+      tsun_lookup(J, S, mus, ts);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const Real f = ts[c] * vis;
+        ar[j][c] = fma(f, S.fr[c], ar[j][c]);
+        am[j][c] = fma(f, S.fm[c], am[j][c]);
+      }
     }
   }
-  double ray[3], mie[3];
-  for (int c = 0; c < 3; ++c) {
-    ray[c] = static_cast<double>(ar[c]) * (dx * J.solar[c] * J.rayleigh[c]);
-    mie[c] = static_cast<double>(am[c]) * (dx * J.solar[c] * J.mie_sca[c]);
+#pragma unroll
+  for (int j = 0; j < kTpt; ++j) {
+    if (!on[j]) continue;
+    const int x = tile * kBlock * kTpt + j * kBlock + tid;
+    ss_store<kF64>(J, static_cast<int64_t>(row) * W + x, dx, ar[j], am[j]);
   }
-  const int64_t t_idx = static_cast<int64_t>(row) * W + x;
-  gstore(J.dR, t_idx, make_float4(static_cast<float>(ray[0]), static_cast<float>(ray[1]),
-                                  static_cast<float>(ray[2]), 0.0f));
-  gstore(J.dM, t_idx, make_float4(static_cast<float>(mie[0]), static_cast<float>(mie[1]),
-                                  static_cast<float>(mie[2]), 0.0f));
-  if (kF64) {
-    const D4 lr = apply_lum_d(J.lum, ray), lm = apply_lum_d(J.lum, mie);
-    store_d4(J.S64, t_idx, D4{lr.x, lr.y, lr.z, lm.x});
-    if (!J.combined) store_d4(J.SM64, t_idx, D4{lm.x, lm.y, lm.z, 0.0});
-    return;
-  }
-  const float3 rayf = make_float3(static_cast<float>(ray[0]), static_cast<float>(ray[1]),
-                                  static_cast<float>(ray[2]));
-  const float3 mief = make_float3(static_cast<float>(mie[0]), static_cast<float>(mie[1]),
-                                  static_cast<float>(mie[2]));
-  const float3 lr = apply_lum(J.lum, rayf);
-  const float3 lm = apply_lum(J.lum, mief);
-  gstore(J.S, t_idx, make_float4(lr.x, lr.y, lr.z, lm.x));
-  if (!J.combined) gstore(J.SM, t_idx, make_float4(lm.x, lm.y, lm.z, 0.0f));
 }
 
 // ---------------------------------------------------------------------------
@@ -1529,7 +1569,8 @@ int blocks_direct_irradiance(const Job& j) {
 }
 int blocks_single_scattering(const Job& j) {
   const int w = j.nnu * j.nmus;
-  return ((w + kBlock - 1) / kBlock) * j.nmu * j.nr;
+  const int per_block = kBlock * ss_texels_per_thread(w);
+  return ((w + per_block - 1) / per_block) * j.nmu * j.nr;
 }
 int blocks_scattering_density(const Job& j) {
   const SdTile t = kSdTile[j.sd_tile];
@@ -1575,13 +1616,23 @@ hipError_t launch_direct_irradiance(const PassLaunch& p, hipStream_t s) {
                      p.jobs, p.first_block, p.njobs);
   return hipGetLastError();
 }
-hipError_t launch_single_scattering(const PassLaunch& p, hipStream_t s) {
+template <int kTpt>
+void launch_ss_tpt(const PassLaunch& p, hipStream_t s) {
+  if (kTpt < ss_texels_per_thread(p.min_w) || kTpt > ss_texels_per_thread(p.max_w)) return;
+  const dim3 g(p.total_blocks), b(kBlock);
   if (p.kinds & kJobsF32)
-    hipLaunchKernelGGL(k_single_scattering<false>, dim3(p.total_blocks), dim3(kBlock), 0, s,
-                       p.jobs, p.first_block, p.njobs);
+    hipLaunchKernelGGL((k_single_scattering<false, kTpt>), g, b, 0, s, p.jobs, p.first_block,
+                       p.njobs);
   if (p.kinds & kJobsAcc64)
-    hipLaunchKernelGGL(k_single_scattering<true>, dim3(p.total_blocks), dim3(kBlock), 0, s,
-                       p.jobs, p.first_block, p.njobs);
+    hipLaunchKernelGGL((k_single_scattering<true, kTpt>), g, b, 0, s, p.jobs, p.first_block,
+                       p.njobs);
+}
+hipError_t launch_single_scattering(const PassLaunch& p, hipStream_t s) {
+  // One instance per kind of job the launch may hold (accumulator precision,
+  // texels per thread from the packed row widths); each skips the others.
+  launch_ss_tpt<1>(p, s);
+  launch_ss_tpt<2>(p, s);
+  launch_ss_tpt<4>(p, s);
   return hipGetLastError();
 }
 template <int kChunk, int kB>
