@@ -333,11 +333,11 @@ __host__ __device__ inline int ss_texels_per_thread(int w) {
 // kF64: the kernel serves the jobs with FP64 accumulators (illuminance
 // mode) and computes the sun-side lookups and the sums in FP64; otherwise it
 // serves the others. kTpt: texels per thread, ss_texels_per_thread(W) of the
-// job (texels x and x + kBlock of a row for 2: their sun lookups read the
-// same two transmittance rows per sample, so a wave has twice the gathers in
-// flight and the second's lines are often in L1 already). The instances are
-// launched over the same blocks when a launch holds several kinds; each
-// returns at once on the others' jobs.
+// job: texels x + k kBlock (k < kTpt) of one row, whose sun lookups read the
+// same two transmittance rows per sample, so a wave keeps kTpt times the
+// gathers in flight and the later texels often find their lines in L1. The
+// instances are launched over the same blocks when a launch holds several
+// kinds; each returns at once on the others' jobs.
 template <bool kF64, int kTpt>
 __global__ __launch_bounds__(kBlock) void k_single_scattering(
     const Job* __restrict__ jobs, const int* __restrict__ first_block,
