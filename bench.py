@@ -24,8 +24,12 @@ same line carries, measured in the same run after the headline steps:
   under a saturating burst;
 - "roofline": the dominant precompute kernel's counter-measured VALU rate
   (instructions from the committed rocprofv3 counter summary, launch times
-  from HIP events in this run) against the FP32 VALU peak; "passes" gives
-  the same per pass, with HBM GB/s;
+  from HIP events in this run) against the FP32 VALU peak, and in
+  "fp_arith" its FP32/FP64 add, mul and FMA rate against their
+  single-issue ceiling; "passes" gives the same per pass, with HBM GB/s;
+- "measured_peaks": on-device rate ceilings of this GPU (csrc/peaks.hip);
+- "lookup_roofline": the lookups' bytes (inputs, outputs and the distinct
+  LUT texels they read) per second against the HBM peak;
 - "cpu_baseline": the FP64 oracle on the host cores, on a bounded sample.
 
 The other --workload values run one section alone (for profiling).
