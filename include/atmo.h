@@ -266,7 +266,7 @@ int atmo_ms_tile_candidates(int* texels_per_thread, int* block_threads,
 /* Times every candidate shape of the two passes on a batch of Earth-like
  * atmospheres of each listed size (enough copies to fill the GPU) on
  * 'stream', and records the fastest shape per size class; a shape replaces
- * the default only if it is at least 3% faster. Classes already in the
+ * the default only if it is at least 10% faster. Classes already in the
  * table are skipped unless retune is nonzero. Synchronous. */
 atmo_status atmo_autotune(const atmo_lut_sizes* sizes, int count, int retune,
                           void* stream);
